@@ -1,0 +1,101 @@
+/*
+ * magace.h -- C-ABI of libmagace.so, the MI355X (gfx950) evaluator of magnetic-ACE
+ * energies, forces and magnetic gradients.
+ *
+ * The first two entry points replace libace's, with identical signatures and semantics,
+ * so LAMMPS pair_spin_ace (or any caller of libace) links against libmagace.so unchanged.
+ * All scalars are passed by reference, all indices are 1-based, arrays are plain C arrays
+ * owned by the caller; nothing is retained between calls.
+ */
+#ifndef MAGACE_H
+#define MAGACE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes (the reference has none: it calls Fortran `stop`). */
+#define MAGACE_OK 0
+#define MAGACE_ERR_INPUT 1  /* bad file, header, sizes, geometry (e.g. very small distance) */
+#define MAGACE_ERR_STATE 2  /* called before initialisation */
+#define MAGACE_ERR_DEVICE 3 /* HIP runtime error */
+
+/* Precision modes */
+#define MAGACE_FP64 0  /* FP64 everywhere; agrees with libace to ~1e-14 relative */
+#define MAGACE_MIXED 1 /* FP32 splines/Y_lm/A/B products, FP64 rho, energy, outputs */
+
+/* Replaces libace acelibini_ (src/ace.F90:10-75; caller lammps/pair_spin_ace.cpp:61,288).
+ * nele must be 1; elenumbers[0] = atomic number + 1; xcnumber 1 = pbe, 2 = lda; reads
+ * "<El>.<xc>.in" from the current directory; smear must match the file to 1e-6.
+ * Runs once per process; later calls are no-ops (src/util.F90:29-33).
+ * On failure magace_last_error() != 0 (the reference stops the process instead). */
+void acelibini_(int *nele, double *rcut, double *rcut_magnetic, double *smear, int *xcnumber,
+                int *elenumbers);
+
+/* Replaces libace acelib_ (src/ace.F90:79-321; caller lammps/pair_spin_ace.cpp:62,545).
+ * Inputs: natom local atoms, nall >= natom, nei pairs; nstart/nstop[natom] inclusive 1-based
+ * pair ranges tiling 1..nei in atom order; nlist (unused by the arithmetic), occ/occlist
+ * (element index, must be 1); dlist[nei] = r, elist[3*nei] = (x_i - x_j)/r;
+ * dlist_momi[natom], elist_momi[3*natom] = |m_i|, m_i/|m_i|; dlist_mom[nei],
+ * elist_mom[3*nei] = |m_j|, m_j/|m_j|.
+ * Outputs: eout[natom] = E_i; flist[3*nei] = dE_i/d(x_i - x_j); tlist[3*nei] = dE_i/dm_j;
+ * tlist0[3*natom] = dE_i/dm_i.  Caller assembly as pair_spin_ace.cpp:587-641:
+ *   f_i -= F_p, f_j += F_p, fm_i -= tlist0_i, fm_j -= tlist_p. */
+void acelib_(int *natom, int *nall, int *nei, int *nlist, int *occ, int *occlist, int *nstart,
+             int *nstop, double *eout, double *dlist, double *elist, double *flist,
+             double *dlist_momi, double *elist_momi, double *dlist_mom, double *elist_mom,
+             double *tlist, double *tlist0);
+
+/* Error state of the last call (no counterpart in the reference, which stops). */
+int magace_last_error(void);
+const char *magace_error_string(void);
+
+/* acelibini_ with an explicit file path and element symbol instead of the CWD lookup
+ * (same checks as src/util.F90:525-597, 420-444).  Replaces any previous potential. */
+int magace_init_file(const char *path, const char *element, double rcut, double rcut_magnetic,
+                     double smear, int xcnumber);
+int magace_finalize(void);
+
+/* Select MAGACE_FP64 or MAGACE_MIXED for subsequent evaluations. */
+int magace_set_precision(int mode);
+int magace_get_precision(void);
+
+/* acelib_ on device-resident arrays (same meaning as acelib_, FP64), with 0-based CSR
+ * offsets[natom+1] instead of nstart/nstop.  Enqueued on `stream` (a hipStream_t, or NULL for
+ * the library's stream); no host synchronisation.  magace_check_errors() synchronises the
+ * stream and reports geometry errors raised by the kernels. */
+int magace_compute_device(int natom, int nei, const int *offsets, const double *dlist,
+                          const double *elist, const double *dlist_momi, const double *elist_momi,
+                          const double *dlist_mom, const double *elist_mom, double *eout,
+                          double *flist, double *tlist, double *tlist0, void *stream);
+int magace_check_errors(void *stream);
+
+/* Caller side of the boundary: PairSpinACE::compute (lammps/pair_spin_ace.cpp:340-669) on
+ * LAMMPS-shaped arrays.  x[nall][3], sp[nall][4] (unit direction, magnitude), type[nall],
+ * map[type] -> element index (must be 0), full neighbour list ilist/numneigh with the
+ * neighbours of atom i at firstneigh[firstneigh_offsets[i] ...].  Accumulates into f and fm
+ * ([nall][3]), energy, virial[6] (xx yy zz xy xz yz) and eatom[nall] (each may be NULL
+ * except f/fm).  Returns 0 or an error code. */
+int magace_pair_compute(int nlocal, int nghost, const double *x, const double *sp, const int *type,
+                        const int *map, int inum, const int *ilist, const int *numneigh,
+                        const int *firstneigh_offsets, const int *firstneigh, int newton_pair,
+                        double *f, double *fm, double *energy, double *virial, double *eatom);
+
+/* Basis introspection: counts of B functions of orders 1..4 of the loaded potential
+ * (the numbers src/util.F90:1831,1929,2235,2596 print). */
+int magace_basis_counts(int *nvar4);
+/* Enumerate the basis of a header without a file: hdr[12] = nb_a nb_m nr2_m l2m nr3_a nr3_m
+ * l3a l3m nr4_a nr4_m l4a l4m.  Writes index tuples (b1index/b2index/b3index/b4index columns)
+ * when cap suffices; returns the count, or a negative error code. */
+int magace_enumerate(const int *hdr, int order, int *out, int cap);
+/* info[0..7] = nz, merged terms, raw m-combinations, raw order 2/3/4, shape class, oA2 */
+int magace_model_info(long long *info, int n);
+/* The merged monomial table rho = Re sum_t w_t prod_s Z[idx[4t+s]] (Z[nz] = 1) that replaces
+ * the per-B m-combination loops of src/b_basis.F90:91-307; copies when cap >= count and
+ * returns the count.  Host-side introspection; needs no GPU. */
+int magace_model_terms(unsigned short *idx, double *w, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

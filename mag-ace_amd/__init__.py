@@ -1,0 +1,14 @@
+"""magace-mi355x: MI355X-native evaluator of magnetic-ACE energies, forces and torques.
+
+The package directory is ``mag-ace_amd/`` (not an importable identifier); it is loaded as
+``magace_amd`` by ``__graft_entry__.load_package()``.  Contents:
+
+* ``libmagace.so``  -- HIP kernels + the libace-compatible C-ABI (include/magace.h)
+* ``capi``          -- ctypes binding of that C-ABI
+* ``systems``       -- synthetic bcc-Fe workloads and the pair_spin_ace array packing
+* ``potgen``        -- synthetic potential files in the reference text format
+* ``pair``          -- Python mirror of the LAMMPS PairSpinACE caller
+"""
+from . import capi, potgen, systems  # noqa: F401
+
+__all__ = ["capi", "potgen", "systems"]

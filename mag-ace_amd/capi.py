@@ -1,0 +1,196 @@
+"""ctypes binding of libmagace.so (the C-ABI declared in include/magace.h).
+
+There is no CPU fallback: if the shared library is missing, or no GPU is visible when a
+compute entry point is called, an exception is raised.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libmagace.so")
+
+FP64, MIXED = 0, 1
+
+_D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_I = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_IP = C.POINTER(C.c_int)
+_DP = C.POINTER(C.c_double)
+
+#: every symbol include/magace.h declares
+EXPORTS = ("acelibini_", "acelib_", "magace_last_error", "magace_error_string", "magace_init_file",
+           "magace_finalize", "magace_set_precision", "magace_get_precision", "magace_compute_device",
+           "magace_check_errors", "magace_basis_counts", "magace_enumerate", "magace_model_info",
+           "magace_pair_compute", "magace_model_terms")
+
+
+class MagaceError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise MagaceError("libmagace.so is not built (run __graft_entry__.build() or "
+                          "python mag-ace_amd/build.py)")
+    L = C.CDLL(LIBPATH, mode=C.RTLD_GLOBAL)
+    L.magace_error_string.restype = C.c_char_p
+    L.magace_init_file.argtypes = [C.c_char_p, C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_int]
+    L.magace_set_precision.argtypes = [C.c_int]
+    L.acelibini_.argtypes = [_IP, _DP, _DP, _DP, _IP, _IP]
+    L.acelibini_.restype = None
+    L.acelib_.argtypes = [_IP, _IP, _IP, _I, _I, _I, _I, _I, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]
+    L.acelib_.restype = None
+    L.magace_compute_device.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 12
+    L.magace_check_errors.argtypes = [C.c_void_p]
+    L.magace_basis_counts.argtypes = [_I]
+    L.magace_enumerate.argtypes = [_I, C.c_int, C.c_void_p, C.c_int]
+    L.magace_model_info.argtypes = [np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), C.c_int]
+    _lib = L
+    return L
+
+
+def _check(rc=None):
+    L = lib()
+    code = L.magace_last_error()
+    if (rc is not None and rc != 0) or code != 0:
+        raise MagaceError(L.magace_error_string().decode() or ("error %s" % rc))
+
+
+def init_file(path, element="Fe", rcut=5.0, rcut_magnetic=4.0, smear=0.2, xc=1):
+    rc = lib().magace_init_file(os.fsencode(path), element.encode(), rcut, rcut_magnetic, smear, xc)
+    _check(rc)
+
+
+def acelibini(nele, rcut, rcut_magnetic, smear, xcnumber, elenumbers):
+    """libace acelibini_ (reads <El>.<xc>.in from the current directory)."""
+    L = lib()
+    n, xc = C.c_int(nele), C.c_int(xcnumber)
+    rc_, rm, sm = C.c_double(rcut), C.c_double(rcut_magnetic), C.c_double(smear)
+    els = (C.c_int * len(elenumbers))(*elenumbers)
+    L.acelibini_(C.byref(n), C.byref(rc_), C.byref(rm), C.byref(sm), C.byref(xc), els)
+    _check()
+
+
+def set_precision(mode):
+    _check(lib().magace_set_precision(int(mode)))
+
+
+def finalize():
+    lib().magace_finalize()
+
+
+def acelib(a: dict):
+    """Call acelib_ on host arrays in the pair_spin_ace layout (see systems.pack_acelib)."""
+    L = lib()
+    natom = int(len(a["nstart"]))
+    nei = int(len(a["dlist"]))
+    nall = int(a.get("nall", natom))
+    eout = np.zeros(max(natom, 1))
+    flist = np.zeros(max(3 * nei, 1))
+    tlist = np.zeros(max(3 * nei, 1))
+    tlist0 = np.zeros(max(3 * natom, 1))
+    ci = lambda v: C.byref(C.c_int(v))
+    pad_i = lambda x: np.ascontiguousarray(x if len(x) else np.zeros(1), dtype=np.int32)
+    pad_d = lambda x: np.ascontiguousarray(x if len(x) else np.zeros(1), dtype=np.float64)
+    L.acelib_(ci(natom), ci(nall), ci(nei), pad_i(a["nlist"]), pad_i(a["occ"]), pad_i(a["occlist"]),
+              pad_i(a["nstart"]), pad_i(a["nstop"]), eout, pad_d(a["dlist"]), pad_d(a["elist"]), flist,
+              pad_d(a["dlist_momi"]), pad_d(a["elist_momi"]), pad_d(a["dlist_mom"]), pad_d(a["elist_mom"]),
+              tlist, tlist0)
+    _check()
+    return dict(eout=eout[:natom], flist=flist[:3 * nei], tlist=tlist[:3 * nei], tlist0=tlist0[:3 * natom])
+
+
+def compute_device(natom, nei, offsets, dlist, elist, dmomi, emomi, dmom, emom, eout, flist, tlist,
+                   tlist0, stream=0):
+    """acelib_ on device pointers (ints).  Enqueued on `stream` (int handle, 0 = library stream)."""
+    rc = lib().magace_compute_device(natom, nei, offsets, dlist, elist, dmomi, emomi, dmom, emom, eout,
+                                     flist, tlist, tlist0, stream or None)
+    _check(rc)
+
+
+def check_errors(stream=0):
+    _check(lib().magace_check_errors(stream or None))
+
+
+def basis_counts():
+    n = np.zeros(4, np.int32)
+    _check(lib().magace_basis_counts(n))
+    return [int(x) for x in n]
+
+
+HDR_KEYS = ("nb_a", "nb_m", "nr2_m", "l2m", "nr3_a", "nr3_m", "l3a", "l3m", "nr4_a", "nr4_m", "l4a", "l4m")
+
+
+def enumerate_basis(header: dict):
+    """Index tuples of the basis of `header` (order -> int array), from the C++ enumeration."""
+    L = lib()
+    h = np.array([header[k] for k in HDR_KEYS], np.int32)
+    out = {}
+    for order, width in ((1, 1), (2, 4), (3, 9), (4, 15)):
+        n = L.magace_enumerate(h, order, None, 0)
+        if n < 0:
+            _check(n)
+        buf = np.zeros(max(n * width, 1), np.int32)
+        L.magace_enumerate(h, order, buf.ctypes.data, buf.size)
+        out[order] = buf[:n * width].reshape(n, width)
+    return out
+
+
+def model_info():
+    v = np.zeros(8, np.int64)
+    _check(lib().magace_model_info(v, 8))
+    keys = ("nz", "terms", "raw", "raw2", "raw3", "raw4", "shape_class", "oA2")
+    return dict(zip(keys, (int(x) for x in v)))
+
+
+def pair_compute(x, sp, ilist, neighbors, nlocal=None, newton_pair=1):
+    """PairSpinACE::compute through the C++ caller (csrc/pair_host.cpp).
+
+    x: (nall, 3); sp: (nall, 4) [unit direction, magnitude]; neighbors: list of int arrays
+    (full list, indices into x).  Returns (energy, f (nall,3), fm (nall,3), virial[6], eatom).
+    """
+    L = lib()
+    L.magace_pair_compute.argtypes = [C.c_int, C.c_int, _D, _D, _I, _I, C.c_int, _I, _I, _I, _I, C.c_int,
+                                      _D, _D, _D, _D, _D]
+    x = np.ascontiguousarray(x, np.float64)
+    sp = np.ascontiguousarray(sp, np.float64)
+    nall = len(x)
+    nlocal = nall if nlocal is None else nlocal
+    numneigh = np.array([len(n) for n in neighbors] + [0] * (nall - len(neighbors)), np.int32)
+    offs = np.zeros(nall, np.int32)
+    offs[1:] = np.cumsum(numneigh)[:-1]
+    flat = np.ascontiguousarray(np.concatenate([np.asarray(n, np.int32) for n in neighbors])
+                                if len(neighbors) else np.zeros(1, np.int32), np.int32)
+    typ = np.ones(nall, np.int32)
+    mp = np.array([-1, 0], np.int32)
+    f = np.zeros((nall, 3))
+    fm = np.zeros((nall, 3))
+    e = np.zeros(1)
+    vir = np.zeros(6)
+    eatom = np.zeros(nall)
+    rc = L.magace_pair_compute(nlocal, nall - nlocal, x, sp, typ, mp, len(ilist),
+                               np.ascontiguousarray(ilist, np.int32), numneigh, offs, flat, int(newton_pair),
+                               f, fm, e, vir, eatom)
+    _check(rc)
+    return float(e[0]), f, fm, vir, eatom
+
+
+def model_terms():
+    L = lib()
+    L.magace_model_terms.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+    n = L.magace_model_terms(None, None, 0)
+    if n < 0:
+        _check(n)
+    idx = np.zeros((max(n, 1), 4), np.uint16)
+    w = np.zeros(max(n, 1))
+    L.magace_model_terms(idx.ctypes.data, w.ctypes.data, n)
+    return idx[:n].astype(np.int64), w[:n]

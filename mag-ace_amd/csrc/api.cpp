@@ -1,0 +1,502 @@
+// api.cpp -- the C-ABI of libmagace.so.
+//
+// acelibini_ / acelib_ keep the exact signatures and semantics of libace's entry points as
+// LAMMPS pair_spin_ace calls them (lammps/pair_spin_ace.cpp:59-62, src/ace.F90:10-21,79-107),
+// so the library drops in for libace.  The remaining magace_* functions are additions:
+// error reporting (the reference calls Fortran `stop` instead), an explicit-path init, the
+// precision switch, and a device-pointer entry point for callers whose arrays already live
+// in HBM.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/magace.h"
+#include "device.hpp"
+#include "magace.hpp"
+
+namespace magace {
+namespace {
+
+std::string g_err;
+int g_errcode = 0;
+
+void set_error(int code, const std::string& msg) {
+    g_errcode = code;
+    g_err = msg;
+    std::fprintf(stderr, "magace: %s\n", msg.c_str());
+}
+void clear_error() {
+    g_errcode = 0;
+    g_err.clear();
+}
+
+const char* kElements[] = {
+    "X",  "H",  "He", "Li", "Be", "B",  "C",  "N",  "O",  "F",  "Ne", "Na", "Mg", "Al", "Si", "P",  "S",
+    "Cl", "Ar", "K",  "Ca", "Sc", "Ti", "V",  "Cr", "Mn", "Fe", "Co", "Ni", "Cu", "Zn", "Ga", "Ge", "As",
+    "Se", "Br", "Kr", "Rb", "Sr", "Y",  "Zr", "Nb", "Mo", "Tc", "Ru", "Rh", "Pd", "Ag", "Cd", "In", "Sn",
+    "Sb", "Te", "I",  "Xe", "Cs", "Ba", "La", "Ce", "Pr", "Nd", "Pm", "Sm", "Eu", "Gd", "Tb", "Dy", "Ho",
+    "Er", "Tm", "Yb", "Lu", "Hf", "Ta", "W",  "Re", "Os", "Ir", "Pt", "Au", "Hg", "Tl", "Pb", "Bi", "Po",
+    "At", "Rn", "Fr", "Ra", "Ac", "Th", "Pa", "U",  "Np", "Pu", "Am", "Cm", "Bk", "Cf", "Es", "Fm", "Md",
+    "No", "Lr"};
+
+template <typename T>
+struct DevTables {
+    T* lut_a = nullptr;
+    T* lut_m = nullptr;
+};
+
+struct Context {
+    Model model;
+    int precision = MAGACE_FP64;
+    int shape_class = 0;
+    DevParams<double> P64;
+    DevParams<float> P32;
+    DevTables<double> t64;
+    DevTables<float> t32;
+    uint16_t* tidx = nullptr;
+    double* tw = nullptr;
+    Buffers buf;
+    // staging for acelib_ host arrays
+    size_t cap_pairs = 0, cap_atoms = 0;
+    int* d_off = nullptr;
+    double *d_dl = nullptr, *d_el = nullptr, *d_dm = nullptr, *d_em = nullptr;
+    double *d_dmi = nullptr, *d_emi = nullptr;
+    double *d_eout = nullptr, *d_fl = nullptr, *d_tl = nullptr, *d_t0 = nullptr;
+    std::vector<int> h_off;
+    hipStream_t stream = nullptr;
+    bool uploaded = false;
+    ~Context();
+};
+
+std::unique_ptr<Context> g_ctx;
+
+#define HIPCHK(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess) {                                                           \
+            set_error(MAGACE_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(_e) + \
+                                             " at " #expr);                               \
+            return MAGACE_ERR_DEVICE;                                                     \
+        }                                                                                 \
+    } while (0)
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+Context::~Context() {
+    dfree(t64.lut_a); dfree(t64.lut_m); dfree(t32.lut_a); dfree(t32.lut_m);
+    dfree(tidx); dfree(tw);
+    dfree(buf.Z); dfree(buf.W); dfree(buf.ehc); dfree(buf.rho1); dfree(buf.dE); dfree(buf.edf); dfree(buf.err);
+    dfree(d_off); dfree(d_dl); dfree(d_el); dfree(d_dm); dfree(d_em); dfree(d_dmi); dfree(d_emi);
+    dfree(d_eout); dfree(d_fl); dfree(d_tl); dfree(d_t0);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+template <typename T>
+void fill_params(DevParams<T>& P, const Model& M) {
+    const Header& h = M.pot.h;
+    P.stride_a = M.spl.atomic_stride;
+    P.stride_m = M.spl.magnetic_stride;
+    P.nlut = M.spl.nlut;
+    P.rs_a = M.spl.rs_a;
+    P.rs_m = M.spl.rs_m;
+    P.nb_a = h.nb_a; P.nb_m = h.nb_m; P.nr2m = h.nr2_m; P.l2m = h.l2m;
+    P.nr3a = h.nr3_a; P.nr3m = h.nr3_m; P.l3a = h.l3a; P.l3m = h.l3m;
+    P.lsa = M.z.lsa; P.lsm = M.z.lsm; P.lsm3 = M.z.lsm3; P.nm3 = M.z.nm3;
+    P.oA2 = M.z.oA2; P.oA0 = M.z.oA0; P.nz = M.z.nz;
+    P.lha = (h.l3a + 1) * (h.l3a + 2) / 2;
+    P.lhm = (h.l2m + 1) * (h.l2m + 2) / 2;
+    P.nterms = (int)M.terms.w.size();
+    P.nfemb = h.nfemb;
+    P.p1a = M.pot.p1[0];
+    P.p1b = M.pot.p1[1];
+    P.ecut = h.ecut;
+    P.decut = h.decut;
+    for (int k = 0; k < 32; ++k) P.c1[k] = k < (int)M.pot.c1.size() ? M.pot.c1[k] : 0.0;
+}
+
+template <typename T>
+int upload_table(T*& dst, const std::vector<double>& src) {
+    std::vector<T> tmp(src.begin(), src.end());
+    HIPCHK(hipMalloc((void**)&dst, tmp.size() * sizeof(T) + 64));
+    HIPCHK(hipMemcpy(dst, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int build_context(Potential&& pot, std::unique_ptr<Context>& out) {
+    auto ctx = std::make_unique<Context>();
+    Model& M = ctx->model;
+    M.pot = std::move(pot);
+    std::string err;
+    if (!check_header(M.pot.h, err)) {
+        set_error(MAGACE_ERR_INPUT, err);
+        return MAGACE_ERR_INPUT;
+    }
+    M.z = make_layout(M.pot.h);
+    if (M.z.nz >= 65535) {
+        set_error(MAGACE_ERR_INPUT, "feature vector too long for 16-bit term indices");
+        return MAGACE_ERR_INPUT;
+    }
+    const Header& h = M.pot.h;
+    ctx->shape_class = supported_nm3_class(M.z.nm3, h.nr2_m, h.nb_a, h.l3a, h.l2m, h.nr3_a);
+    if (ctx->shape_class < 0) {
+        set_error(MAGACE_ERR_INPUT, "potential shape outside the compiled kernel classes");
+        return MAGACE_ERR_INPUT;
+    }
+    if (h.nb_m > 32) {
+        set_error(MAGACE_ERR_INPUT, "nradbase_magnetic > 32");
+        return MAGACE_ERR_INPUT;
+    }
+    M.terms = build_terms(M.pot, M.z);
+    M.spl = build_splines(M.pot);
+    fill_params(ctx->P64, M);
+    fill_params(ctx->P32, M);
+    out = std::move(ctx);
+    return 0;
+}
+
+// device tables are uploaded on first use so the host model can be built without a GPU
+int upload(Context& c) {
+    if (c.uploaded) return 0;
+    const Model& M = c.model;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(upload_legendre_constants());
+    HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    if (upload_table(c.t64.lut_a, M.spl.atomic)) return MAGACE_ERR_DEVICE;
+    if (upload_table(c.t64.lut_m, M.spl.magnetic)) return MAGACE_ERR_DEVICE;
+    if (upload_table(c.t32.lut_a, M.spl.atomic)) return MAGACE_ERR_DEVICE;
+    if (upload_table(c.t32.lut_m, M.spl.magnetic)) return MAGACE_ERR_DEVICE;
+    const size_t nt = M.terms.w.size();
+    HIPCHK(hipMalloc((void**)&c.tidx, nt * 4 * sizeof(uint16_t) + 64));
+    HIPCHK(hipMalloc((void**)&c.tw, nt * sizeof(double) + 64));
+    if (nt) {
+        HIPCHK(hipMemcpy(c.tidx, M.terms.idx.data(), nt * 4 * sizeof(uint16_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c.tw, M.terms.w.data(), nt * sizeof(double), hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMalloc((void**)&c.buf.err, sizeof(int)));
+    HIPCHK(hipMemset(c.buf.err, 0, sizeof(int)));
+    c.P64.lut_a = c.t64.lut_a; c.P64.lut_m = c.t64.lut_m;
+    c.P32.lut_a = c.t32.lut_a; c.P32.lut_m = c.t32.lut_m;
+    c.P64.tidx = c.P32.tidx = c.tidx;
+    c.P64.tw = c.P32.tw = c.tw;
+    c.uploaded = true;
+    return 0;
+}
+
+int ensure_atoms(Context& c, size_t natom) {
+    if (natom <= c.buf.cap_atoms) return 0;
+    const size_t cap = natom + natom / 4 + 16;
+    const size_t nz = (size_t)c.model.z.nz;
+    dfree(c.buf.Z); dfree(c.buf.W); dfree(c.buf.ehc); dfree(c.buf.rho1); dfree(c.buf.dE); dfree(c.buf.edf);
+    HIPCHK(hipMalloc(&c.buf.Z, cap * nz * 2 * sizeof(double)));
+    HIPCHK(hipMalloc(&c.buf.W, cap * nz * 2 * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&c.buf.ehc, cap * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&c.buf.rho1, cap * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&c.buf.dE, cap * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&c.buf.edf, cap * sizeof(double)));
+    c.buf.cap_atoms = cap;
+    return 0;
+}
+
+int run_device(Context& c, int natom, const int* off, const double* dlist, const double* elist,
+               const double* dmomi, const double* emomi, const double* dmom, const double* emom,
+               double* eout, double* flist, double* tlist, double* tlist0, hipStream_t s) {
+    if (int rc = upload(c)) return rc;
+    if (int rc = ensure_atoms(c, (size_t)natom)) return rc;
+    HIPCHK(hipMemsetAsync(c.buf.err, 0, sizeof(int), s));
+    hipError_t e;
+    if (c.precision == MAGACE_FP64)
+        e = Launch<double>::run(c.P64, natom, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist, tlist,
+                                tlist0, c.buf, s, c.shape_class);
+    else
+        e = Launch<float>::run(c.P32, natom, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist, tlist,
+                               tlist0, c.buf, s, c.shape_class);
+    HIPCHK(e);
+    return 0;
+}
+
+int check_kernel_errors(Context& c, hipStream_t s) {
+    int err = 0;
+    HIPCHK(hipMemcpyAsync(&err, c.buf.err, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (err & kErrSmallDistance) {
+        set_error(MAGACE_ERR_INPUT, "Encountered very small distance.");
+        return MAGACE_ERR_INPUT;
+    }
+    if (err & kErrMomentRange) {
+        set_error(MAGACE_ERR_INPUT, "magnetic moment outside [0, rcut_magnetic] (spline table range)");
+        return MAGACE_ERR_INPUT;
+    }
+    return 0;
+}
+
+}  // namespace
+}  // namespace magace
+
+using namespace magace;
+
+extern "C" {
+
+int magace_last_error(void) { return g_errcode; }
+const char* magace_error_string(void) { return g_err.c_str(); }
+
+int magace_init_file(const char* path, const char* element, double rcut, double rcut_magnetic,
+                     double smear, int xcnumber) {
+    clear_error();
+    Potential pot;
+    std::string err;
+    if (!parse_potential(path, element, xcnumber, smear, pot, err)) {
+        set_error(MAGACE_ERR_INPUT, err);
+        return MAGACE_ERR_INPUT;
+    }
+    // cutoff consistency (src/util.F90:420-444)
+    if (pot.h.cut_a > rcut || pot.h.cut_m > rcut_magnetic) {
+        set_error(MAGACE_ERR_INPUT, "Global cutoff too short.");
+        return MAGACE_ERR_INPUT;
+    }
+    pot.rcut = rcut;
+    pot.rcut_m = rcut_magnetic;
+    std::unique_ptr<Context> ctx;
+    int prec = g_ctx ? g_ctx->precision : MAGACE_FP64;
+    if (int rc = build_context(std::move(pot), ctx)) return rc;
+    ctx->precision = prec;
+    g_ctx = std::move(ctx);
+    return 0;
+}
+
+void acelibini_(int* nele, double* rcut, double* rcut_magnetic, double* smear, int* xcnumber,
+                int* elenumbers) {
+    clear_error();
+    if (g_ctx) return;  // initialisation runs once per process (src/util.F90:29-33)
+    if (*nele != 1) {
+        set_error(MAGACE_ERR_INPUT, "Multiple elements require fixing acelibini call first.");
+        return;
+    }
+    if (*xcnumber != 1 && *xcnumber != 2) {
+        set_error(MAGACE_ERR_INPUT, "Unknown XC functional received by acelibini.");
+        return;
+    }
+    const int z1 = elenumbers[0];
+    if (z1 < 1 || z1 > 104) {
+        set_error(MAGACE_ERR_INPUT, "element number out of range");
+        return;
+    }
+    const std::string el = kElements[z1 - 1];
+    const std::string file = el + "." + (*xcnumber == 1 ? "pbe" : "lda") + ".in";
+    magace_init_file(file.c_str(), el.c_str(), *rcut, *rcut_magnetic, *smear, *xcnumber);
+}
+
+int magace_finalize(void) {
+    g_ctx.reset();
+    return 0;
+}
+
+int magace_set_precision(int mode) {
+    if (mode != MAGACE_FP64 && mode != MAGACE_MIXED) {
+        set_error(MAGACE_ERR_INPUT, "precision must be MAGACE_FP64 or MAGACE_MIXED");
+        return MAGACE_ERR_INPUT;
+    }
+    if (!g_ctx) {
+        set_error(MAGACE_ERR_STATE, "library not initialised");
+        return MAGACE_ERR_STATE;
+    }
+    g_ctx->precision = mode;
+    return 0;
+}
+
+int magace_get_precision(void) { return g_ctx ? g_ctx->precision : -1; }
+
+int magace_compute_device(int natom, int nei, const int* offsets, const double* dlist,
+                          const double* elist, const double* dlist_momi, const double* elist_momi,
+                          const double* dlist_mom, const double* elist_mom, double* eout,
+                          double* flist, double* tlist, double* tlist0, void* stream) {
+    clear_error();
+    if (!g_ctx) {
+        set_error(MAGACE_ERR_STATE, "library not initialised");
+        return MAGACE_ERR_STATE;
+    }
+    if (natom < 0 || nei < 0) {
+        set_error(MAGACE_ERR_INPUT, "negative sizes");
+        return MAGACE_ERR_INPUT;
+    }
+    if (int rc = upload(*g_ctx)) return rc;
+    hipStream_t s = stream ? (hipStream_t)stream : g_ctx->stream;
+    if (int rc = run_device(*g_ctx, natom, offsets, dlist, elist, dlist_momi, elist_momi, dlist_mom, elist_mom,
+                            eout, flist, tlist, tlist0, s))
+        return rc;
+    return 0;
+}
+
+int magace_check_errors(void* stream) {
+    if (!g_ctx) {
+        set_error(MAGACE_ERR_STATE, "library not initialised");
+        return MAGACE_ERR_STATE;
+    }
+    if (int rc = upload(*g_ctx)) return rc;
+    return check_kernel_errors(*g_ctx, stream ? (hipStream_t)stream : g_ctx->stream);
+}
+
+void acelib_(int* natom, int* nall, int* nei, int* nlist, int* occ, int* occlist, int* nstart,
+             int* nstop, double* eout, double* dlist, double* elist, double* flist, double* dlist_momi,
+             double* elist_momi, double* dlist_mom, double* elist_mom, double* tlist, double* tlist0) {
+    (void)nlist;  // read but unused by the arithmetic (src/forces.F90:424,754)
+    clear_error();
+    if (!g_ctx) {
+        set_error(MAGACE_ERR_STATE, "acelib_ called before acelibini_");
+        return;
+    }
+    Context& c = *g_ctx;
+    const int N = *natom, P = *nei;
+    if (N < 0 || P < 0 || *nall < N) {
+        set_error(MAGACE_ERR_INPUT, "Incorrect array dimension specifications.");
+        return;
+    }
+    for (int i = 0; i < N; ++i)
+        if (occ[i] != 1) {
+            set_error(MAGACE_ERR_INPUT, "Cannot handle multiple species.");
+            return;
+        }
+    for (int p = 0; p < P; ++p)
+        if (occlist[p] != 1) {
+            set_error(MAGACE_ERR_INPUT, "Cannot handle multiple species.");
+            return;
+        }
+    // pair ranges must tile 1..nei in atom order, as pair_spin_ace builds them
+    // (lammps/pair_spin_ace.cpp:457,524)
+    c.h_off.resize(N + 1);
+    int expect = 1;
+    for (int i = 0; i < N; ++i) {
+        if (nstart[i] != expect || nstop[i] < nstart[i] - 1 || nstop[i] > P) {
+            set_error(MAGACE_ERR_INPUT, "pair ranges must be contiguous and ordered (nstart/nstop)");
+            return;
+        }
+        c.h_off[i] = nstart[i] - 1;
+        expect = nstop[i] + 1;
+    }
+    c.h_off[N] = expect - 1;
+    if (c.h_off[N] != P) {
+        set_error(MAGACE_ERR_INPUT, "Something wrong with neighbor list translation: neicount");
+        return;
+    }
+    auto fail = [&](hipError_t e, const char* what) {
+        if (e == hipSuccess) return false;
+        set_error(MAGACE_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e) + " (" + what + ")");
+        return true;
+    };
+    if ((size_t)P > c.cap_pairs) {
+        dfree(c.d_dl); dfree(c.d_el); dfree(c.d_dm); dfree(c.d_em); dfree(c.d_fl); dfree(c.d_tl);
+        const size_t cap = (size_t)P + P / 4 + 64;
+        if (fail(hipMalloc((void**)&c.d_dl, cap * 8), "alloc") || fail(hipMalloc((void**)&c.d_el, cap * 24), "alloc") ||
+            fail(hipMalloc((void**)&c.d_dm, cap * 8), "alloc") || fail(hipMalloc((void**)&c.d_em, cap * 24), "alloc") ||
+            fail(hipMalloc((void**)&c.d_fl, cap * 24), "alloc") || fail(hipMalloc((void**)&c.d_tl, cap * 24), "alloc"))
+            return;
+        c.cap_pairs = cap;
+    }
+    if ((size_t)N + 1 > c.cap_atoms) {
+        dfree(c.d_off); dfree(c.d_dmi); dfree(c.d_emi); dfree(c.d_eout); dfree(c.d_t0);
+        const size_t cap = (size_t)N + N / 4 + 64;
+        if (fail(hipMalloc((void**)&c.d_off, cap * 4), "alloc") || fail(hipMalloc((void**)&c.d_dmi, cap * 8), "alloc") ||
+            fail(hipMalloc((void**)&c.d_emi, cap * 24), "alloc") || fail(hipMalloc((void**)&c.d_eout, cap * 8), "alloc") ||
+            fail(hipMalloc((void**)&c.d_t0, cap * 24), "alloc"))
+            return;
+        c.cap_atoms = cap;
+    }
+    if (upload(c)) return;
+    hipStream_t s = c.stream;
+    const auto H2D = hipMemcpyHostToDevice;
+    if (fail(hipMemcpyAsync(c.d_off, c.h_off.data(), (N + 1) * 4, H2D, s), "copy") ||
+        fail(hipMemcpyAsync(c.d_dl, dlist, (size_t)P * 8, H2D, s), "copy") ||
+        fail(hipMemcpyAsync(c.d_el, elist, (size_t)P * 24, H2D, s), "copy") ||
+        fail(hipMemcpyAsync(c.d_dm, dlist_mom, (size_t)P * 8, H2D, s), "copy") ||
+        fail(hipMemcpyAsync(c.d_em, elist_mom, (size_t)P * 24, H2D, s), "copy") ||
+        fail(hipMemcpyAsync(c.d_dmi, dlist_momi, (size_t)N * 8, H2D, s), "copy") ||
+        fail(hipMemcpyAsync(c.d_emi, elist_momi, (size_t)N * 24, H2D, s), "copy"))
+        return;
+    if (run_device(c, N, c.d_off, c.d_dl, c.d_el, c.d_dmi, c.d_emi, c.d_dm, c.d_em, c.d_eout, c.d_fl, c.d_tl,
+                   c.d_t0, s))
+        return;
+    const auto D2H = hipMemcpyDeviceToHost;
+    if (fail(hipMemcpyAsync(eout, c.d_eout, (size_t)N * 8, D2H, s), "copy") ||
+        fail(hipMemcpyAsync(flist, c.d_fl, (size_t)P * 24, D2H, s), "copy") ||
+        fail(hipMemcpyAsync(tlist, c.d_tl, (size_t)P * 24, D2H, s), "copy") ||
+        fail(hipMemcpyAsync(tlist0, c.d_t0, (size_t)N * 24, D2H, s), "copy"))
+        return;
+    if (check_kernel_errors(c, s)) return;
+}
+
+int magace_basis_counts(int* nvar) {
+    if (!g_ctx) {
+        set_error(MAGACE_ERR_STATE, "library not initialised");
+        return MAGACE_ERR_STATE;
+    }
+    const Basis& B = g_ctx->model.pot.basis;
+    nvar[0] = (int)B.b1.size(); nvar[1] = (int)B.b2.size(); nvar[2] = (int)B.b3.size(); nvar[3] = (int)B.b4.size();
+    return 0;
+}
+
+int magace_enumerate(const int* hdr, int order, int* out, int cap) {
+    clear_error();
+    Header h;
+    int* f[12] = {&h.nb_a, &h.nb_m, &h.nr2_m, &h.l2m, &h.nr3_a, &h.nr3_m, &h.l3a, &h.l3m,
+                  &h.nr4_a, &h.nr4_m, &h.l4a, &h.l4m};
+    for (int i = 0; i < 12; ++i) *f[i] = hdr[i];
+    std::string err;
+    if (!check_header(h, err)) {
+        set_error(MAGACE_ERR_INPUT, err);
+        return -MAGACE_ERR_INPUT;
+    }
+    const Basis B = enumerate_basis(h);
+    auto copy = [&](const auto& v, int width) {
+        const int n = (int)v.size();
+        if (out && cap >= n * width)
+            for (int k = 0; k < n; ++k)
+                for (int c = 0; c < width; ++c) out[k * width + c] = v[k][c];
+        return n;
+    };
+    switch (order) {
+        case 1: return copy(B.b1, 1);
+        case 2: return copy(B.b2, 4);
+        case 3: return copy(B.b3, 9);
+        case 4: return copy(B.b4, 15);
+    }
+    set_error(MAGACE_ERR_INPUT, "order must be 1..4");
+    return -MAGACE_ERR_INPUT;
+}
+
+int magace_model_terms(uint16_t* idx, double* w, int cap) {
+    if (!g_ctx) {
+        set_error(MAGACE_ERR_STATE, "library not initialised");
+        return -MAGACE_ERR_STATE;
+    }
+    const TermList& T = g_ctx->model.terms;
+    const int n = (int)T.w.size();
+    if (idx && w && cap >= n) {
+        std::memcpy(idx, T.idx.data(), (size_t)n * 4 * sizeof(uint16_t));
+        std::memcpy(w, T.w.data(), (size_t)n * sizeof(double));
+    }
+    return n;
+}
+
+int magace_model_info(long long* info, int n) {
+    if (!g_ctx) {
+        set_error(MAGACE_ERR_STATE, "library not initialised");
+        return MAGACE_ERR_STATE;
+    }
+    const Model& M = g_ctx->model;
+    long long v[8] = {(long long)M.z.nz, (long long)M.terms.w.size(), (long long)M.terms.raw, M.terms.n2,
+                      M.terms.n3, M.terms.n4, g_ctx->shape_class, (long long)M.z.oA2};
+    for (int i = 0; i < n && i < 8; ++i) info[i] = v[i];
+    return 0;
+}
+
+}  // extern "C"

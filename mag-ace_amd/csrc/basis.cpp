@@ -1,0 +1,327 @@
+// basis.cpp -- basis enumeration, Clebsch-Gordan couplings and the merged term list.
+//
+// Enumeration follows the selection rules of setuptables/bcreate (src/util.F90:1801-2814,
+// 4673-4960); couplings follow setupcg/setuphighercouplings (src/util.F90:4149-4388).
+// Instead of the reference's per-B m-combination loops with stored dB/dA
+// (src/b_basis.F90:91-307), every B function is expanded once, at init, into monomials
+// w * prod Z[i] of the per-atom feature vector Z = [A | A2 | A0]; identical monomials are
+// merged and complex-conjugate partners folded, so the device evaluates
+// rho = sum_k c_k B_k and its gradient with one pass over a flat table.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <map>
+#include <unordered_map>
+
+#include "magace.hpp"
+
+namespace magace {
+
+bool check_header(const Header& h, std::string& err) {
+    auto fail = [&](const char* m) { err = m; return false; };
+    if (h.nr5_a || h.nr5_m || h.nr6_a || h.nr6_m) return fail("body order > 4 is not supported");
+    if (h.ndensity != 1) return fail("ndensity must be 1");
+    if (h.nfemb < 0 || h.nfemb > 2) return fail("embedding function not implemented (nfemb 0..2)");
+    // limit checks of src/util.F90:233-272
+    if (h.l3a < h.l4a) return fail(" l3max_atomic needs to be largest.");
+    if (h.l2m < h.l3m || h.l2m < h.l4m) return fail(" l2max_magnetic needs to be largest.");
+    if (h.nr3_a < h.nr4_a) return fail(" nradial_atomic needs to be largest.");
+    if (h.nr2_m < h.nr3_m || h.nr2_m < h.nr4_m) return fail(" nradial_magnetic needs to be largest.");
+    if (h.nr3_a > h.nb_a) return fail(" nradial_atomic < nradbase_atomic required");
+    if (h.nr2_m > h.nb_m) return fail(" nradial_magnetic < nradbase_magnetic required");
+    if (h.l2m > 12 || h.l3a > 12) return fail("angular momentum above 12 is not supported");
+    if (h.nb_a > 32 || h.nb_m > 32) return fail("more than 32 radial base functions");
+    for (int t : {h.radtype_a, h.radtype_m})
+        if (t < 3 || t > 8) return fail("radtype must be 3..8 (0-2 stop in the reference)");
+    return true;
+}
+
+namespace {
+
+// bcreate ordering key for one element (src/util.F90:4728-4730)
+struct Keyer {
+    int lmax, nradial;
+    long operator()(int l, int nr) const { return (long)((lmax + 1) + l) * (nradial + 1) + nr; }
+};
+
+bool triangle_even(int a, int b, int c) {
+    return a + b >= c && std::abs(a - b) <= c && (a + b + c) % 2 == 0;
+}
+
+// couplings L of ((l0 l1)L (l2 l3)L)0 allowed by bcreate rank 4 (src/util.F90:4810-4857)
+std::vector<int> rank4_couplings(const Keyer& key, const int nr[4], const int l[4]) {
+    std::vector<int> out;
+    long k[4];
+    for (int i = 0; i < 4; ++i) k[i] = key(l[i], nr[i]);
+    int lmx = *std::max_element(l, l + 4);
+    for (int L = 0; L <= 2 * lmx; ++L) {
+        if (L > l[0] + l[1] || L > l[2] + l[3]) continue;
+        if (L < std::abs(l[0] - l[1]) || L < std::abs(l[2] - l[3])) continue;
+        if (k[0] == k[1] && (L & 1)) continue;
+        if (k[2] == k[3] && (L & 1)) continue;
+        out.push_back(L);
+    }
+    if ((l[0] + l[1] + l[2] + l[3]) & 1) out.clear();
+    return out;
+}
+
+}  // namespace
+
+Basis enumerate_basis(const Header& h) {
+    Basis B;
+    Keyer key{std::max(h.l3a, h.l2m), std::max(h.nr3_a, h.nr2_m)};
+    for (int n = 1; n <= h.nb_m; ++n) B.b1.push_back({n});
+    // order 2: rank-2 magnetic (n0', l) (n1', l), ordered
+    for (int n0 = 1; n0 <= h.nr2_m; ++n0)
+        for (int n1p = 1; n1p <= h.nr2_m; ++n1p)
+            for (int n1 = 1; n1 <= h.nb_a; ++n1)
+                for (int l = 0; l <= h.l2m; ++l)
+                    if (key(l, n0) >= key(l, n1p)) B.b2.push_back({n0, n1p, n1, l});
+    // order 3: pre-order on combined (atomic, magnetic) keys, then rank-2 atomic ordered and
+    // rank-3 magnetic triangle/parity
+    {
+        const int ka_max = h.nr3_a * (h.l3a + 1), km_max = h.nr3_m * (h.l3m + 1);
+        auto nc = [&](int na, int la, int nm, int lm) {
+            return (long)(km_max + 1) * (h.nr3_a * la + na) + (h.nr3_m * lm + nm);
+        };
+        (void)ka_max;
+        for (int n1 = 1; n1 <= h.nr3_a; ++n1)
+        for (int n2 = 1; n2 <= h.nr3_a; ++n2)
+        for (int n0p = 1; n0p <= h.nr3_m; ++n0p)
+        for (int n1p = 1; n1p <= h.nr3_m; ++n1p)
+        for (int n2p = 1; n2p <= h.nr3_m; ++n2p)
+        for (int l1 = 0; l1 <= h.l3a; ++l1)
+        for (int l0p = 0; l0p <= h.l3m; ++l0p)
+        for (int l1p = 0; l1p <= h.l3m; ++l1p)
+        for (int l2p = 0; l2p <= h.l3m; ++l2p) {
+            if (nc(n1, l1, n1p, l1p) < nc(n2, l1, n2p, l2p)) continue;
+            if (key(l1, n1) < key(l1, n2)) continue;
+            if (!triangle_even(l0p, l1p, l2p)) continue;
+            B.b3.push_back({n1, n2, n0p, n1p, n2p, l1, l0p, l1p, l2p});
+        }
+    }
+    // order 4
+    {
+        const int km_max = h.nr4_m * (h.l4m + 1);
+        auto nc = [&](int na, int la, int nm, int lm) {
+            return (long)(km_max + 1) * (h.nr4_a * la + na) + (h.nr4_m * lm + nm);
+        };
+        for (int n1 = 1; n1 <= h.nr4_a; ++n1)
+        for (int n2 = 1; n2 <= h.nr4_a; ++n2)
+        for (int n3 = 1; n3 <= h.nr4_a; ++n3)
+        for (int m0 = 1; m0 <= h.nr4_m; ++m0)
+        for (int m1 = 1; m1 <= h.nr4_m; ++m1)
+        for (int m2 = 1; m2 <= h.nr4_m; ++m2)
+        for (int m3 = 1; m3 <= h.nr4_m; ++m3)
+        for (int l1 = 0; l1 <= h.l4a; ++l1)
+        for (int l2 = 0; l2 <= h.l4a; ++l2)
+        for (int l3 = 0; l3 <= h.l4a; ++l3)
+        for (int k0 = 0; k0 <= h.l4m; ++k0)
+        for (int k1 = 0; k1 <= h.l4m; ++k1)
+        for (int k2 = 0; k2 <= h.l4m; ++k2)
+        for (int k3 = 0; k3 <= h.l4m; ++k3) {
+            long c1 = nc(n1, l1, m1, k1), c2 = nc(n2, l2, m2, k2), c3 = nc(n3, l3, m3, k3);
+            if (!(c1 >= c2 && c2 >= c3)) continue;
+            if (!triangle_even(l1, l2, l3)) continue;
+            const int nr[4] = {m0, m1, m2, m3}, ll[4] = {k0, k1, k2, k3};
+            for (int L : rank4_couplings(key, nr, ll))
+                B.b4.push_back({n1, n2, n3, m0, m1, m2, m3, l1, l2, l3, k0, k1, k2, k3, L});
+        }
+    }
+    return B;
+}
+
+// Clebsch-Gordan <j1 m1 j2 m2 | J M> by the Racah sum (src/util.F90:4176-4249)
+double clebsch_gordan(int j1, int j2, int J, int m1, int m2, int M) {
+    static double fac[200];
+    static bool init = false;
+    if (!init) {
+        fac[0] = 1.0;
+        for (int i = 1; i < 200; ++i) fac[i] = fac[i - 1] * (double)i;
+        init = true;
+    }
+    if (j1 + j2 < J || std::abs(j1 - j2) > J) return 0.0;
+    if (std::abs(m1) > j1 || std::abs(m2) > j2 || std::abs(M) > J || m1 + m2 != M) return 0.0;
+    double pre = (double)(2 * J + 1) * fac[j1 + j2 - J] * fac[j1 - j2 + J] * fac[j2 - j1 + J] /
+                 fac[j1 + j2 + J + 1] *
+                 (fac[J + M] * fac[J - M] * fac[j1 + m1] * fac[j1 - m1] * fac[j2 + m2] * fac[j2 - m2]);
+    double s = 0.0;
+    for (int k = 0; k <= j1 + j2 + J + 1; ++k) {
+        int a = j1 + j2 - J - k, b = j1 - m1 - k, c = j2 + m2 - k, d = J - j2 + m1 + k, e = J - j1 - m2 + k;
+        if (a < 0 || b < 0 || c < 0 || d < 0 || e < 0) continue;
+        double t = 1.0 / (fac[k] * fac[a] * fac[b] * fac[c] * fac[d] * fac[e]);
+        s += (k & 1) ? -t : t;
+    }
+    return std::sqrt(pre) * s;
+}
+
+ZLayout make_layout(const Header& h) {
+    ZLayout z;
+    z.lsa = (h.l3a + 1) * (h.l3a + 1);
+    z.lsm = (h.l2m + 1) * (h.l2m + 1);
+    z.lsm3 = (h.l3m + 1) * (h.l3m + 1);
+    z.nm3 = h.nr3_m * z.lsm3;
+    z.nr2m_ = h.nr2_m;
+    z.oA = 0;
+    z.oA2 = h.nr3_a * z.lsa * z.nm3;
+    z.oA0 = z.oA2 + h.nb_a * h.nr2_m * z.lsm;
+    z.nz = z.oA0 + h.nr2_m * z.lsm;
+    return z;
+}
+
+namespace {
+
+// complex-conjugate partner of a Z entry: X_{-m,-m'} = (-1)^(m+m') conj(X_{m,m'})
+// (src/a_basis.F90:303-342, spherical.F90:263-280)
+struct Conj {
+    std::vector<int> partner;
+    std::vector<int> sign;
+};
+
+Conj conj_map(const Header& h, const ZLayout& z) {
+    Conj c;
+    c.partner.assign(z.nz + 1, 0);
+    c.sign.assign(z.nz + 1, 1);
+    for (int n = 0; n < h.nr3_a; ++n)
+        for (int l = 0; l <= h.l3a; ++l)
+            for (int m = -l; m <= l; ++m)
+                for (int np = 0; np < h.nr3_m; ++np)
+                    for (int lp = 0; lp <= h.l3m; ++lp)
+                        for (int mp = -lp; mp <= lp; ++mp) {
+                            int x = z.a(n, l, m, np, lp, mp);
+                            c.partner[x] = z.a(n, l, -m, np, lp, -mp);
+                            c.sign[x] = ((m + mp) & 1) ? -1 : 1;
+                        }
+    for (int n = 0; n < h.nb_a; ++n)
+        for (int np = 0; np < h.nr2_m; ++np)
+            for (int lp = 0; lp <= h.l2m; ++lp)
+                for (int mp = -lp; mp <= lp; ++mp) {
+                    int x = z.a2(n, np, lp, mp);
+                    c.partner[x] = z.a2(n, np, lp, -mp);
+                    c.sign[x] = (mp & 1) ? -1 : 1;
+                }
+    for (int np = 0; np < h.nr2_m; ++np)
+        for (int lp = 0; lp <= h.l2m; ++lp)
+            for (int mp = -lp; mp <= lp; ++mp) {
+                int x = z.a0(np, lp, mp);
+                c.partner[x] = z.a0(np, lp, -mp);
+                c.sign[x] = (mp & 1) ? -1 : 1;
+            }
+    c.partner[z.nz] = z.nz;
+    return c;
+}
+
+uint64_t pack4(std::array<int, 4> v) {
+    std::sort(v.begin(), v.end());
+    return ((uint64_t)v[0] << 48) | ((uint64_t)v[1] << 32) | ((uint64_t)v[2] << 16) | (uint64_t)v[3];
+}
+
+}  // namespace
+
+TermList build_terms(const Potential& pot, const ZLayout& z) {
+    const Header& h = pot.h;
+    const Basis& B = pot.basis;
+    const int one = z.nz;  // padding factor Z[nz] = 1
+    std::unordered_map<uint64_t, double> acc;
+    acc.reserve(1 << 16);
+    TermList T;
+    auto emit = [&](double w, int a, int b, int c, int d) {
+        if (w == 0.0) return;
+        acc[pack4({a, b, c, d})] += w;
+        ++T.raw;
+    };
+    // order 2: B2 = Re sum_m C(l l 0; m -m) A0[n0'][l m] A2[n1][n1'][l -m]  (b_basis.F90:95-124)
+    for (size_t k = 0; k < B.b2.size(); ++k) {
+        const auto& b = B.b2[k];
+        const double c = pot.c2[k];
+        const int n0 = b[0] - 1, n1p = b[1] - 1, n1 = b[2] - 1, l = b[3];
+        for (int m = -l; m <= l; ++m) {
+            ++T.n2;
+            emit(c * clebsch_gordan(l, l, 0, m, -m, 0), z.a0(n0, l, m), z.a2(n1, n1p, l, -m), one, one);
+        }
+    }
+    // order 3 (b_basis.F90:146-205, couplings util.F90:4292-4333)
+    for (size_t k = 0; k < B.b3.size(); ++k) {
+        const auto& b = B.b3[k];
+        const double c = pot.c3[k];
+        const int n1 = b[0] - 1, n2 = b[1] - 1, n0p = b[2] - 1, n1p = b[3] - 1, n2p = b[4] - 1;
+        const int l1 = b[5], l0p = b[6], l1p = b[7], l2p = b[8];
+        for (int m1 = -l1; m1 <= l1; ++m1) {
+            const double ca = clebsch_gordan(l1, l1, 0, m1, -m1, 0);
+            for (int q0 = -l0p; q0 <= l0p; ++q0)
+                for (int q1 = -l1p; q1 <= l1p; ++q1) {
+                    const int q2 = -(q0 + q1);
+                    if (std::abs(q2) > l2p) continue;
+                    ++T.n3;
+                    const double cm = clebsch_gordan(l0p, l1p, l2p, q0, q1, -q2) *
+                                      clebsch_gordan(l2p, l2p, 0, q2, -q2, 0);
+                    emit(c * ca * cm, z.a0(n0p, l0p, q0), z.a(n1, l1, m1, n1p, l1p, q1),
+                         z.a(n2, l1, -m1, n2p, l2p, q2), one);
+                }
+        }
+    }
+    // order 4 (b_basis.F90:229-301, couplings util.F90:4343-4388)
+    for (size_t k = 0; k < B.b4.size(); ++k) {
+        const auto& b = B.b4[k];
+        const double c = pot.c4[k];
+        const int n1 = b[0] - 1, n2 = b[1] - 1, n3 = b[2] - 1;
+        const int p0 = b[3] - 1, p1 = b[4] - 1, p2 = b[5] - 1, p3 = b[6] - 1;
+        const int l1 = b[7], l2 = b[8], l3 = b[9], k0 = b[10], k1 = b[11], k2 = b[12], k3 = b[13], L = b[14];
+        for (int m1 = -l1; m1 <= l1; ++m1)
+            for (int m2 = -l2; m2 <= l2; ++m2) {
+                const int m3 = -(m1 + m2);
+                if (std::abs(m3) > l3) continue;
+                const double ca = clebsch_gordan(l1, l2, l3, m1, m2, -m3) * clebsch_gordan(l3, l3, 0, m3, -m3, 0);
+                for (int M = -L; M <= L; ++M)
+                    for (int q0 = -k0; q0 <= k0; ++q0) {
+                        const int q1 = M - q0;
+                        if (std::abs(q1) > k1) continue;
+                        for (int q2 = -k2; q2 <= k2; ++q2) {
+                            const int q3 = -M - q2;
+                            if (std::abs(q3) > k3) continue;
+                            ++T.n4;
+                            const double cm = clebsch_gordan(k0, k1, L, q0, q1, M) *
+                                              clebsch_gordan(k2, k3, L, q2, q3, -M) *
+                                              clebsch_gordan(L, L, 0, M, -M, 0);
+                            emit(c * ca * cm, z.a0(p0, k0, q0), z.a(n1, l1, m1, p1, k1, q1),
+                                 z.a(n2, l2, m2, p2, k2, q2), z.a(n3, l3, m3, p3, k3, q3));
+                        }
+                    }
+            }
+    }
+    // fold complex-conjugate partners: Re(w P) + Re(w' s conj P) = (w + s w') Re P
+    const Conj cj = conj_map(h, z);
+    std::map<uint64_t, double> folded;
+    for (const auto& kv : acc) {
+        const uint64_t key = kv.first;
+        std::array<int, 4> v = {(int)(key >> 48), (int)((key >> 32) & 0xffff), (int)((key >> 16) & 0xffff),
+                                (int)(key & 0xffff)};
+        int s = 1;
+        std::array<int, 4> vb;
+        for (int i = 0; i < 4; ++i) {
+            vb[i] = cj.partner[v[i]];
+            s *= cj.sign[v[i]];
+        }
+        const uint64_t kb = pack4(vb);
+        if (kb == key) {
+            // a self-conjugate monomial with s = -1 is purely imaginary: Re P == 0
+            if (s == 1) folded[key] += kv.second;
+        } else if (kb < key) {
+            folded[kb] += s * kv.second;
+        } else {
+            folded[key] += kv.second;
+        }
+    }
+    for (const auto& kv : folded) {
+        if (kv.second == 0.0) continue;
+        const uint64_t key = kv.first;
+        T.idx.push_back((uint16_t)(key >> 48));
+        T.idx.push_back((uint16_t)((key >> 32) & 0xffff));
+        T.idx.push_back((uint16_t)((key >> 16) & 0xffff));
+        T.idx.push_back((uint16_t)(key & 0xffff));
+        T.w.push_back(kv.second);
+    }
+    return T;
+}
+
+}  // namespace magace

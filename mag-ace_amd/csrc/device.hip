@@ -1,0 +1,901 @@
+// device.hip -- HIP kernels (gfx950) for the magnetic-ACE energy / force / torque path.
+//
+// One evaluation (the work of libace's acelib_, src/ace.F90:280-286) is three launches:
+//
+//   k_density  (prepA, src/a_basis.F90:59-359)   one workgroup per atom.  Pair radial
+//              functions and Y_lm of the bond and of the neighbour moment are staged in LDS
+//              for a chunk of neighbours; threads then own (lm, n-block) columns of A and
+//              (l'm', n-block) columns of A2 and accumulate in registers.  Writes
+//              Z = [A | A2 | A0] per atom, the core sum and the B1 part of rho.
+//   k_adjoint  (prepB + forceweights, src/b_basis.F90:62-307, src/forces.F90:84-349)
+//              one workgroup per atom.  Z and dZ in LDS; each thread walks monomials
+//              w*Z[a]Z[b]Z[c]Z[d] of the merged term list, accumulating rho and scattering
+//              the holomorphic gradient with LDS atomics; then E_i, dE/drho and the scaled
+//              gradient W = dE/dZ are written.
+//   k_forces   (forcecompute + torquecompute, src/forces.F90:418-623, 688-961) one lane per
+//              pair.  Splines and Y_lm are recomputed in registers (nothing per pair is
+//              stored between launches); F_p = dE_i/du_p and T_p = dE_i/dm_j are
+//              contractions of W with the pair's derivatives; lane 0 adds T0_i = dE_i/dm_i.
+//
+// T = double (FP64 path) or float (mixed path: FP32 splines, Y_lm, A, B products; FP64 for
+// rho, the energy and all outputs).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "device.hpp"
+
+namespace magace {
+
+constexpr int kLmaxC = 12;
+constexpr int kLhC = (kLmaxC + 1) * (kLmaxC + 2) / 2;
+
+// Legendre recursion coefficients (src/spherical.F90:18-49): a_lm, b_lm (m <= l-2), c_l, d_l
+__constant__ double c_alm_d[kLhC], c_blm_d[kLhC], c_cl_d[kLmaxC + 1], c_dl_d[kLmaxC + 1];
+__constant__ float c_alm_f[kLhC], c_blm_f[kLhC], c_cl_f[kLmaxC + 1], c_dl_f[kLmaxC + 1];
+
+template <typename T> struct LegC;
+template <> struct LegC<double> {
+    __device__ static double a(int k) { return c_alm_d[k]; }
+    __device__ static double b(int k) { return c_blm_d[k]; }
+    __device__ static double c(int l) { return c_cl_d[l]; }
+    __device__ static double d(int l) { return c_dl_d[l]; }
+};
+template <> struct LegC<float> {
+    __device__ static float a(int k) { return c_alm_f[k]; }
+    __device__ static float b(int k) { return c_blm_f[k]; }
+    __device__ static float c(int l) { return c_cl_f[l]; }
+    __device__ static float d(int l) { return c_dl_f[l]; }
+};
+
+hipError_t upload_legendre_constants() {
+    double a[kLhC] = {0}, b[kLhC] = {0}, c[kLmaxC + 1] = {0}, d[kLmaxC + 1] = {0};
+    float af[kLhC], bf[kLhC], cf[kLmaxC + 1], df[kLmaxC + 1];
+    for (int l = 1; l <= kLmaxC; ++l) {
+        const double lsq = (double)(l * l), l1 = 4.0 * lsq - 1.0, l2 = lsq - 2.0 * l + 1.0;
+        for (int m = 0; m <= l - 2; ++m) {
+            const double msq = (double)(m * m);
+            const int k = l * (l + 1) / 2 + m;
+            a[k] = std::sqrt(l1 / (lsq - msq));
+            b[k] = -std::sqrt((l2 - msq) / (4.0 * l2 - 1.0));
+        }
+        c[l] = -std::sqrt(1.0 + 0.5 / (double)l);
+        d[l] = std::sqrt(2.0 * (double)(l - 1) + 3.0);
+    }
+    for (int i = 0; i < kLhC; ++i) af[i] = (float)a[i], bf[i] = (float)b[i];
+    for (int i = 0; i <= kLmaxC; ++i) cf[i] = (float)c[i], df[i] = (float)d[i];
+    hipError_t e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_alm_d), a, sizeof a))) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_blm_d), b, sizeof b))) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_cl_d), c, sizeof c))) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_dl_d), d, sizeof d))) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_alm_f), af, sizeof af))) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_blm_f), bf, sizeof bf))) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_cl_f), cf, sizeof cf))) return e;
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_dl_f), df, sizeof df);
+}
+
+// ---------------------------------------------------------------- small helpers
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ void cfma(cplx<T>& acc, cplx<T> a, cplx<T> b) {
+    acc.re = fma(a.re, b.re, fma(-a.im, b.im, acc.re));
+    acc.im = fma(a.re, b.im, fma(a.im, b.re, acc.im));
+}
+template <typename T>
+__device__ __forceinline__ T re_mul(cplx<T> a, cplx<T> b) {  // Re(a*b)
+    return fma(a.re, b.re, -a.im * b.im);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cconj_sign(cplx<T> a, int m) {  // (-1)^m conj(a)
+    return (m & 1) ? cplx<T>{-a.re, a.im} : cplx<T>{a.re, -a.im};
+}
+
+// cubic Hermite row entry: value and derivative (d/dr) at fractional position w
+template <typename T>
+__device__ __forceinline__ void horner(const T* __restrict__ c, T w, T rs, T& v, T& dv) {
+    const T c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
+    v = fma(w, fma(w, fma(w, c3, c2), c1), c0);
+    dv = fma(w, fma(w, (T)3 * c3, (T)2 * c2), c1) * rs;
+}
+
+// Spline row and fraction for the atomic table (src/radial.F90:68-84).
+// Returns nullptr when the pair is beyond the table (all functions zero).
+template <typename T>
+__device__ __forceinline__ const T* atomic_row(const DevParams<T>& P, double r, T& w, int* err) {
+    const double x = r * P.rs_a;
+    const double fl = floor(x);
+    if (fl <= 0.0) { atomicOr(err, kErrSmallDistance); return nullptr; }
+    if (fl >= (double)P.nlut) return nullptr;
+    const int nl = (int)fl;
+    w = (T)(x - fl);
+    return P.lut_a + (size_t)(nl - 1) * P.stride_a;
+}
+// magnetic table (src/radial.F90:126-158): nl == 0 -> every function is 1, derivative 0
+// (returned as row == nullptr with unit == true)
+template <typename T>
+__device__ __forceinline__ const T* magnetic_row(const DevParams<T>& P, double mu, T& w, bool& unit,
+                                                 int* err) {
+    unit = false;
+    const double x = mu * P.rs_m;
+    const double fl = floor(x);
+    if (fl < 0.0 || fl > (double)P.nlut) { atomicOr(err, kErrMomentRange); return nullptr; }
+    if (fl == 0.0) { unit = true; return nullptr; }
+    w = (T)(x - fl);
+    return P.lut_m + (size_t)((int)fl - 1) * P.stride_m;
+}
+
+// Normalised associated Legendre functions P[lh], dP[lh] = dP/dz for l <= L, m <= l
+// (src/spherical.F90:55-155); lh = l(l+1)/2 + m.  Y_lm(x,y,z) = P[lh] (x+iy)^m.
+template <typename T, int L>
+__device__ __forceinline__ void legendre(T z, T* P, T* dP) {
+    P[0] = (T)1;
+    dP[0] = (T)0;
+    if (L == 0) return;
+    const T sq3 = (T)1.73205080756887719318, sq3o2 = (T)1.22474487139158894067;
+    P[1] = sq3 * z;
+    dP[1] = sq3;
+    P[2] = -sq3o2;
+    dP[2] = (T)0;
+#pragma unroll
+    for (int l = 2; l <= L; ++l) {
+#pragma unroll
+        for (int m = 0; m <= l - 2; ++m) {
+            const int k = l * (l + 1) / 2 + m, k1 = (l - 1) * l / 2 + m, k2 = (l - 2) * (l - 1) / 2 + m;
+            const T a = LegC<T>::a(k), b = LegC<T>::b(k);
+            P[k] = a * fma(z, P[k1], b * P[k2]);
+            dP[k] = a * (P[k1] + fma(z, dP[k1], b * dP[k2]));
+        }
+        const int kp = (l - 1) * l / 2 + (l - 1);
+        const T t = LegC<T>::d(l) * P[kp];
+        P[l * (l + 1) / 2 + l - 1] = t * z;
+        dP[l * (l + 1) / 2 + l - 1] = t;
+        P[l * (l + 1) / 2 + l] = LegC<T>::c(l) * P[kp];
+        dP[l * (l + 1) / 2 + l] = (T)0;
+    }
+}
+
+// Y_lm (m >= 0) and its tangential gradient (src/spherical.F90:185-260), from the Legendre
+// values and the phase powers ph[m] = (x+iy)^m.
+template <typename T>
+__device__ __forceinline__ void ylm_grad(int m, T Plm, T dPlm, cplx<T> phm, cplx<T> phm1, T x, T y,
+                                         T z, cplx<T>& Y, cplx<T> (&g)[3]) {
+    Y = {Plm * phm.re, Plm * phm.im};
+    cplx<T> gx, gy, gz;
+    if (m == 0) {
+        gx = {0, 0};
+        gy = {0, 0};
+        gz = {dPlm, 0};
+    } else {
+        const T s = (T)m * Plm;
+        gx = {s * phm1.re, s * phm1.im};
+        gy = {-gx.im, gx.re};
+        gz = {dPlm * phm.re, dPlm * phm.im};
+    }
+    const cplx<T> rdy = {x * gx.re + y * gy.re + z * gz.re, x * gx.im + y * gy.im + z * gz.im};
+    g[0] = {gx.re - rdy.re * x, gx.im - rdy.im * x};
+    g[1] = {gy.re - rdy.re * y, gy.im - rdy.im * y};
+    g[2] = {gz.re - rdy.re * z, gz.im - rdy.im * z};
+}
+
+// half set of Y_lm (m >= 0) written to out[lh]
+template <typename T, int L>
+__device__ __forceinline__ void ylm_half(T x, T y, T z, int lmax, cplx<T>* out) {
+    T P[(L + 1) * (L + 2) / 2], dP[(L + 1) * (L + 2) / 2];
+    legendre<T, L>(z, P, dP);
+    cplx<T> ph = {1, 0};
+    const cplx<T> e = {x, y};
+#pragma unroll
+    for (int m = 0; m <= L; ++m) {
+#pragma unroll
+        for (int l = m; l <= L; ++l) {
+            if (l <= lmax) {
+                const int k = l * (l + 1) / 2 + m;
+                out[k] = {P[k] * ph.re, P[k] * ph.im};
+            }
+        }
+        ph = cmul(ph, e);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ cplx<T> yfull(const cplx<T>* half, int l, int m) {
+    const cplx<T> v = half[l * (l + 1) / 2 + (m < 0 ? -m : m)];
+    return m < 0 ? cconj_sign(v, -m) : v;
+}
+
+// ============================================================================ k_density
+// Shapes: NB n-block of A columns, NM3X max magnetic columns of A, NB2 n-block of A2,
+// NR2X max magnetic radial index of A2.
+template <typename T, int L, int CH, int NB, int NM3X, int NB2, int NR2X>
+__global__ void __launch_bounds__(256) k_density(DevParams<T> P, const int* __restrict__ off,
+                                                const double* __restrict__ dlist,
+                                                const double* __restrict__ elist,
+                                                const double* __restrict__ dmomi,
+                                                const double* __restrict__ emomi,
+                                                const double* __restrict__ dmom,
+                                                const double* __restrict__ emom,
+                                                cplx<T>* __restrict__ Zout, double* __restrict__ ehc_out,
+                                                double* __restrict__ rho1_out, int* err) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int atom = blockIdx.x;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int nfa = P.nr3a * (P.l3a + 1), nfm = P.nr2m * (P.l2m + 1);
+    // LDS carve-up (sizes mirror device_density_lds on the host)
+    cplx<T>* sY = (cplx<T>*)smem;                        // [CH][lha]
+    cplx<T>* sYm = sY + CH * P.lha;                       // [CH][lhm]
+    cplx<T>* sH = sYm + CH * P.lhm;                       // [CH][nm3]
+    cplx<T>* sYc = sH + CH * P.nm3;                       // [lhm] central moment
+    T* sR = (T*)(sYc + P.lhm);                            // [CH][nfa]  R_{l,n}
+    T* sG = sR + CH * nfa;                                // [CH][nb_a]
+    T* sM = sG + CH * P.nb_a;                             // [CH][nfm]  M_{l',n'}
+    T* sCr = sM + CH * nfm;                               // [CH]
+    T* sMc = sCr + CH;                                    // [nfm + nb_m] central M and g^m
+    double* sEhc = (double*)(((uintptr_t)(sMc + nfm + P.nb_m) + 15) & ~(uintptr_t)15);
+
+    const int p0 = off[atom], nn = off[atom + 1] - p0;
+    Zout += (size_t)atom * P.nz;
+
+    // central-atom magnetic functions: A0 and B1 (src/a_basis.F90:69-106, b_basis.F90:62-83)
+    if (tid == 0) {
+        const double mu = dmomi[atom];
+        T w;
+        bool unit;
+        const T* row = magnetic_row(P, mu, w, unit, err);
+        double rho1 = 0.0;
+        for (int k = 0; k < P.nb_m + nfm; ++k) {
+            T v = unit ? (T)1 : (T)0, dv;
+            if (row) horner(row + 4 * k, w, (T)P.rs_m, v, dv);
+            if (k < P.nb_m) {
+                rho1 += P.c1[k] * ((double)v - 1.0);
+                sMc[nfm + k] = v;
+            } else {
+                sMc[k - P.nb_m] = v;
+            }
+        }
+        rho1_out[atom] = rho1;
+        ylm_half<T, L>((T)emomi[3 * atom], (T)emomi[3 * atom + 1], (T)emomi[3 * atom + 2], P.l2m, sYc);
+        *sEhc = 0.0;
+    }
+    __syncthreads();
+    for (int t = tid; t < P.nr2m * P.lsm; t += nthr) {
+        const int np = t / P.lsm, lm = t % P.lsm;
+        int l = 0;
+        while ((l + 1) * (l + 1) <= lm) ++l;
+        const int m = lm - l * l - l;
+        const cplx<T> y = yfull(sYc, l, m);
+        const T mv = sMc[l * P.nr2m + np];
+        Zout[P.oA0 + np * P.lsm + lm] = {mv * y.re, mv * y.im};
+    }
+
+    const int nblkA = (P.nr3a + NB - 1) / NB, nblkA2 = (P.nb_a + NB2 - 1) / NB2;
+    const int nTA = P.lsa * nblkA, nTA2 = P.lsm * nblkA2;
+    for (int round = 0; round * nthr < nTA + nTA2; ++round) {
+        const int task = round * nthr + tid;
+        cplx<T> acc[NB > NB2 ? NB : NB2][NM3X > NR2X ? NM3X : NR2X];
+#pragma unroll
+        for (int a = 0; a < (NB > NB2 ? NB : NB2); ++a)
+#pragma unroll
+            for (int b = 0; b < (NM3X > NR2X ? NM3X : NR2X); ++b) acc[a][b] = {0, 0};
+        // decode task
+        const bool isA = task < nTA, isA2 = !isA && task < nTA + nTA2;
+        int lm = 0, l = 0, m = 0, n0 = 0;
+        if (isA) { lm = task % P.lsa; n0 = (task / P.lsa) * NB; }
+        if (isA2) { lm = (task - nTA) % P.lsm; n0 = ((task - nTA) / P.lsm) * NB2; }
+        while ((l + 1) * (l + 1) <= lm) ++l;
+        m = lm - l * l - l;
+
+        for (int c0 = 0; c0 < nn; c0 += CH) {
+            const int nj = nn - c0 < CH ? nn - c0 : CH;
+            __syncthreads();
+            if (tid < CH) {  // atomic side of pair c0+tid
+                const int j = tid;
+                if (j < nj) {
+                    const int p = p0 + c0 + j;
+                    T w;
+                    const T* row = atomic_row(P, dlist[p], w, err);
+                    for (int k = 0; k < P.nb_a; ++k) {
+                        T v = 0, dv;
+                        if (row) horner(row + 4 * k, w, (T)P.rs_a, v, dv);
+                        sG[j * P.nb_a + k] = v;
+                    }
+                    for (int k = 0; k < nfa; ++k) {
+                        T v = 0, dv;
+                        if (row) horner(row + 4 * (P.nb_a + k), w, (T)P.rs_a, v, dv);
+                        sR[j * nfa + k] = v;
+                    }
+                    T cr = 0, dcr;
+                    if (row) horner(row + 4 * (P.nb_a + nfa), w, (T)P.rs_a, cr, dcr);
+                    sCr[j] = cr;
+                    ylm_half<T, L>((T)elist[3 * p], (T)elist[3 * p + 1], (T)elist[3 * p + 2], P.l3a,
+                                   sY + j * P.lha);
+                } else {
+                    sCr[j] = 0;
+                }
+            } else if (tid < 2 * CH) {  // magnetic side of pair c0+j
+                const int j = tid - CH;
+                if (j < nj) {
+                    const int p = p0 + c0 + j;
+                    T w;
+                    bool unit;
+                    const T* row = magnetic_row(P, dmom[p], w, unit, err);
+                    for (int k = 0; k < nfm; ++k) {
+                        T v = unit ? (T)1 : (T)0, dv;
+                        if (row) horner(row + 4 * (P.nb_m + k), w, (T)P.rs_m, v, dv);
+                        sM[j * nfm + k] = v;
+                    }
+                    cplx<T>* yh = sYm + j * P.lhm;
+                    ylm_half<T, L>((T)emom[3 * p], (T)emom[3 * p + 1], (T)emom[3 * p + 2], P.l2m, yh);
+                    // H_q = M_{l',n'} Y_{l'm'} for the magnetic columns of A (l' <= l3m)
+                    for (int q = 0; q < P.nm3; ++q) {
+                        const int np = q / P.lsm3, lmp = q % P.lsm3;
+                        int lp = 0;
+                        while ((lp + 1) * (lp + 1) <= lmp) ++lp;
+                        const int mp = lmp - lp * lp - lp;
+                        const cplx<T> y = yfull(yh, lp, mp);
+                        const T mv = sM[j * nfm + lp * P.nr2m + np];
+                        sH[j * P.nm3 + q] = {mv * y.re, mv * y.im};
+                    }
+                }
+            }
+            __syncthreads();
+            if (round == 0 && tid == 0) {
+                double s = 0.0;
+                for (int j = 0; j < nj; ++j) s += (double)sCr[j];
+                *sEhc += s;
+            }
+            if (isA) {
+                // A[n][lm][q] += R_{l,n} Y_lm H_q   (src/a_basis.F90:248-291)
+                for (int j = 0; j < nj; ++j) {
+                    const cplx<T> y = yfull(sY + j * P.lha, l, m);
+                    cplx<T> h[NM3X];
+#pragma unroll
+                    for (int q = 0; q < NM3X; ++q) h[q] = q < P.nm3 ? sH[j * P.nm3 + q] : cplx<T>{0, 0};
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) {
+                        const int n = n0 + b;
+                        const T r = n < P.nr3a ? sR[j * nfa + l * P.nr3a + n] : (T)0;
+                        const cplx<T> ry = {r * y.re, r * y.im};
+#pragma unroll
+                        for (int q = 0; q < NM3X; ++q) cfma(acc[b][q], ry, h[q]);
+                    }
+                }
+            } else if (isA2) {
+                // A2[n][n'][l'm'] += g_n M_{l',n'} Y_{l'm'}   (src/a_basis.F90:193-201)
+                for (int j = 0; j < nj; ++j) {
+                    const cplx<T> y = yfull(sYm + j * P.lhm, l, m);
+                    cplx<T> my[NR2X];
+#pragma unroll
+                    for (int q = 0; q < NR2X; ++q) {
+                        const T mv = q < P.nr2m ? sM[j * nfm + l * P.nr2m + q] : (T)0;
+                        my[q] = {mv * y.re, mv * y.im};
+                    }
+#pragma unroll
+                    for (int b = 0; b < NB2; ++b) {
+                        const int n = n0 + b;
+                        const T g = n < P.nb_a ? sG[j * P.nb_a + n] : (T)0;
+#pragma unroll
+                        for (int q = 0; q < NR2X; ++q) {
+                            acc[b][q].re = fma(g, my[q].re, acc[b][q].re);
+                            acc[b][q].im = fma(g, my[q].im, acc[b][q].im);
+                        }
+                    }
+                }
+            }
+        }
+        if (isA) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int n = n0 + b;
+                if (n < P.nr3a)
+#pragma unroll
+                    for (int q = 0; q < NM3X; ++q)
+                        if (q < P.nm3) Zout[(n * P.lsa + lm) * P.nm3 + q] = acc[b][q];
+            }
+        } else if (isA2) {
+#pragma unroll
+            for (int b = 0; b < NB2; ++b) {
+                const int n = n0 + b;
+                if (n < P.nb_a)
+#pragma unroll
+                    for (int q = 0; q < NR2X; ++q)
+                        if (q < P.nr2m) Zout[P.oA2 + (n * P.nr2m + q) * P.lsm + lm] = acc[b][q];
+            }
+        }
+    }
+    __syncthreads();
+    if (tid == 0) ehc_out[atom] = *sEhc;
+}
+
+size_t density_lds_bytes(const DevParams<double>& Pd, int CH, size_t tsize) {
+    const int nfa = Pd.nr3a * (Pd.l3a + 1), nfm = Pd.nr2m * (Pd.l2m + 1);
+    size_t cb = 2 * tsize;
+    size_t n = cb * ((size_t)CH * Pd.lha + (size_t)CH * Pd.lhm + (size_t)CH * Pd.nm3 + Pd.lhm) +
+               tsize * ((size_t)CH * nfa + (size_t)CH * Pd.nb_a + (size_t)CH * nfm + CH + nfm + Pd.nb_m);
+    return ((n + 15) & ~(size_t)15) + 16;
+}
+
+// ============================================================================ k_adjoint
+__device__ __forceinline__ void lds_add(double* p, double v) { atomicAdd(p, v); }
+__device__ __forceinline__ void lds_add(float* p, float v) { atomicAdd(p, v); }
+
+template <typename T>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    if (ln == 0) red[wv] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += red[i];
+    return s;  // valid on thread 0
+}
+
+// embedding F(rho) and F'(rho) (src/forces.F90:1134-1233, nfemb 0..2)
+__device__ inline void embed(int nfemb, double rho, double wpre, double mexp, double& F, double& dF) {
+    if (nfemb == 0) { F = rho; dF = 1.0; return; }
+    const double w = 10.0, eps = 1e-10;
+    const double s = nfemb == 2 ? 1.0 + rho : rho;
+    if (fabs(s) > eps) {
+        double y1 = w * s * s;
+        const double g = y1 > 30.0 ? 0.0 : exp(-y1);
+        const double omg = 1.0 - g, a = fabs(s), sg = s >= 0.0 ? 1.0 : -1.0;
+        y1 = pow(omg * a, mexp);
+        const double y2 = mexp * g * a;
+        F = nfemb == 2 ? wpre * (sg * (y1 + y2) - 1.0) : wpre * sg * (y1 + y2);
+        const double dg = -2.0 * w * s * g, da = sg;
+        const double dy = fabs(y1) < eps ? 0.0 : mexp * y1 / (omg * a);
+        dF = wpre * sg * (dy * (-dg * a + omg * da) + mexp * (dg * a + g * da));
+    } else {
+        F = nfemb == 2 ? wpre * mexp * (s - 1.0) : wpre * mexp * rho;
+        dF = wpre * mexp;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_adjoint(DevParams<T> P, const cplx<T>* __restrict__ Zin,
+                                                cplx<T>* __restrict__ Wout,
+                                                const double* __restrict__ ehc,
+                                                const double* __restrict__ rho1,
+                                                double* __restrict__ eout, double* __restrict__ dEout,
+                                                double* __restrict__ edfout) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<T>* sZ = (cplx<T>*)smem;           // [nz+1], sZ[nz] = 1
+    cplx<T>* sW = sZ + (P.nz + 1);          // [nz+1]
+    double* red = (double*)(sW + (P.nz + 1));
+    __shared__ double s_scale;
+    const int atom = blockIdx.x, tid = threadIdx.x;
+    Zin += (size_t)atom * P.nz;
+    for (int x = tid; x < P.nz; x += blockDim.x) {
+        sZ[x] = Zin[x];
+        sW[x] = {0, 0};
+    }
+    if (tid == 0) {
+        sZ[P.nz] = {1, 0};
+        sW[P.nz] = {0, 0};
+    }
+    __syncthreads();
+    double rho = 0.0;
+    const uint2* idx = (const uint2*)P.tidx;
+    for (int t = tid; t < P.nterms; t += blockDim.x) {
+        const uint2 pk = idx[t];
+        const int i0 = pk.x & 0xffff, i1 = pk.x >> 16, i2 = pk.y & 0xffff, i3 = pk.y >> 16;
+        const T w = (T)P.tw[t];
+        const cplx<T> z0 = sZ[i0], z1 = sZ[i1], z2 = sZ[i2], z3 = sZ[i3];
+        const cplx<T> p01 = cmul(z0, z1), p23 = cmul(z2, z3);
+        rho += (double)P.tw[t] * (double)re_mul(p01, p23);
+        const cplx<T> wp23 = {w * p23.re, w * p23.im}, wp01 = {w * p01.re, w * p01.im};
+        const cplx<T> d0 = cmul(wp23, z1), d1 = cmul(wp23, z0);
+        lds_add(&sW[i0].re, d0.re); lds_add(&sW[i0].im, d0.im);
+        lds_add(&sW[i1].re, d1.re); lds_add(&sW[i1].im, d1.im);
+        if (i2 != P.nz) {
+            const cplx<T> d2 = cmul(wp01, z3);
+            lds_add(&sW[i2].re, d2.re); lds_add(&sW[i2].im, d2.im);
+        }
+        if (i3 != P.nz) {
+            const cplx<T> d3 = cmul(wp01, z2);
+            lds_add(&sW[i3].re, d3.re); lds_add(&sW[i3].im, d3.im);
+        }
+    }
+    const double tot = block_sum<T>(rho, red);
+    if (tid == 0) {
+        // E_i = f_in(e_hc) F(rho) + e_hc (src/forces.F90:1104-1128)
+        const double r = tot + rho1[atom];
+        double F, dF;
+        embed(P.nfemb, r, P.p1a, P.p1b, F, dF);
+        const double e = ehc[atom], elow = P.ecut - P.decut;
+        double fc = 1.0, dfc = 0.0;
+        if (e >= P.ecut) { fc = 0.0; dfc = 0.0; }
+        else if (e > elow) {
+            const double pi = 3.14159265358979323846;
+            fc = 0.5 * (1.0 + cos(pi * (e - elow) / P.decut));
+            dfc = -0.5 * sin(pi * (e - elow) / P.decut) * pi / P.decut;
+        }
+        eout[atom] = fc * F + e;
+        dEout[atom] = fc * dF;
+        edfout[atom] = dfc * F + 1.0;
+        s_scale = fc * dF;
+    }
+    __syncthreads();
+    const T sc = (T)s_scale;
+    Wout += (size_t)atom * P.nz;
+    for (int x = tid; x < P.nz; x += blockDim.x) Wout[x] = {sc * sW[x].re, sc * sW[x].im};
+}
+
+// ============================================================================ k_forces
+// Legendre rows streamed one l at a time (src/spherical.F90:55-155): row l of P_lm and
+// dP_lm/dz for m = 0..l from rows l-1 and l-2.  m is unrolled, l is a runtime loop, so the
+// rows stay in registers without indexing them dynamically.
+template <typename T, int L>
+struct LegRows {
+    T P[L + 1], dP[L + 1], P1[L + 1], dP1[L + 1], P2[L + 1], dP2[L + 1];
+    __device__ __forceinline__ void step(int l, T z) {
+#pragma unroll
+        for (int m = 0; m <= L; ++m) {
+            P2[m] = P1[m]; dP2[m] = dP1[m];
+            P1[m] = P[m]; dP1[m] = dP[m];
+        }
+        if (l == 0) {
+#pragma unroll
+            for (int m = 0; m <= L; ++m) { P[m] = (T)0; dP[m] = (T)0; }
+            P[0] = (T)1;
+            return;
+        }
+        if (l == 1) {
+            P[0] = (T)1.73205080756887719318 * z;
+            dP[0] = (T)1.73205080756887719318;
+            if (L >= 1) { P[L >= 1 ? 1 : 0] = -(T)1.22474487139158894067; dP[L >= 1 ? 1 : 0] = (T)0; }
+            return;
+        }
+        const int base = l * (l + 1) / 2;
+#pragma unroll
+        for (int m = 0; m <= L; ++m) {
+            if (m <= l - 2) {
+                const T a = LegC<T>::a(base + m), b = LegC<T>::b(base + m);
+                const T pm = a * fma(z, P1[m], b * P2[m]);
+                dP[m] = a * (P1[m] + fma(z, dP1[m], b * dP2[m]));
+                P[m] = pm;
+            } else if (m == l - 1) {
+                const T t = LegC<T>::d(l) * P1[m];
+                P[m] = t * z;
+                dP[m] = t;
+            } else if (m == l) {
+                P[m] = LegC<T>::c(l) * P1[m > 0 ? m - 1 : 0];
+                dP[m] = (T)0;
+            }
+        }
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ void conj_all(cplx<T>& Y, cplx<T> (&g)[3], int am) {
+    Y = cconj_sign(Y, am);
+    g[0] = cconj_sign(g[0], am);
+    g[1] = cconj_sign(g[1], am);
+    g[2] = cconj_sign(g[2], am);
+}
+
+// lane = pair.  LA/LM: compile-time bounds of l3a/l2m; NM3X: magnetic columns of A;
+// NBAX: atomic radial base bound; NRAX: nr3a bound; NR2X: nr2m bound.
+template <typename T, int LA, int LM, int NM3X, int NBAX, int NRAX, int NR2X>
+__global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __restrict__ off,
+                                               const double* __restrict__ dlist,
+                                               const double* __restrict__ elist,
+                                               const double* __restrict__ dmomi,
+                                               const double* __restrict__ emomi,
+                                               const double* __restrict__ dmom,
+                                               const double* __restrict__ emom,
+                                               const cplx<T>* __restrict__ Win,
+                                               const double* __restrict__ dEin,
+                                               const double* __restrict__ edfin,
+                                               double* __restrict__ flist, double* __restrict__ tlist,
+                                               double* __restrict__ tlist0, int* err) {
+    const int atom = blockIdx.x, lane = threadIdx.x;
+    const int p0 = off[atom], nn = off[atom + 1] - p0;
+    const cplx<T>* __restrict__ W = Win + (size_t)atom * P.nz;
+    const int nfa = P.nr3a * (P.l3a + 1);
+    const double edf = edfin[atom];
+
+    for (int c0 = 0; c0 < nn; c0 += 64) {
+        const int j = c0 + lane;
+        const bool active = j < nn;
+        const int p = p0 + (active ? j : 0);
+        T Fx = 0, Fy = 0, Fz = 0, Tx = 0, Ty = 0, Tz = 0;
+        T wa = 0;
+        const T* rowa = active ? atomic_row(P, dlist[p], wa, err) : nullptr;
+        if (rowa) {
+            const T r = (T)dlist[p], invr = (T)1 / r;
+            const T ex = (T)elist[3 * p], ey = (T)elist[3 * p + 1], ez = (T)elist[3 * p + 2];
+            const double mu_d = dmom[p];
+            const T invmu = (T)(1.0 / mu_d);
+            const T sx = (T)emom[3 * p], sy = (T)emom[3 * p + 1], sz = (T)emom[3 * p + 2];
+            T wm = 0;
+            bool unit = false;
+            const T* rowm = magnetic_row(P, mu_d, wm, unit, err);
+            const T rsa = (T)P.rs_a, rsm = (T)P.rs_m;
+            cplx<T> phm[LM + 1];
+            phm[0] = {1, 0};
+#pragma unroll
+            for (int m = 1; m <= LM; ++m) phm[m] = cmul(phm[m - 1], cplx<T>{sx, sy});
+            // H_q = M_{l',n'}(mu_j) Y_{l'm'}(s_j) for the magnetic columns of A
+            cplx<T> H[NM3X], U[NM3X];
+#pragma unroll
+            for (int q = 0; q < NM3X; ++q) {
+                H[q] = {0, 0};
+                U[q] = {0, 0};
+            }
+            {
+                LegRows<T, LM> Lr;
+                for (int lp = 0; lp <= P.l3m; ++lp) {
+                    Lr.step(lp, sz);
+#pragma unroll
+                    for (int am = 0; am <= LM; ++am) {
+                        if (am > lp) break;
+                        const cplx<T> y0 = {Lr.P[am] * phm[am].re, Lr.P[am] * phm[am].im};
+#pragma unroll
+                        for (int sg = 0; sg < 2; ++sg) {
+                            if (sg == 1 && am == 0) break;
+                            const int mp = sg ? -am : am;
+                            const cplx<T> y = sg ? cconj_sign(y0, am) : y0;
+                            for (int np = 0; np < P.nr3m; ++np) {
+                                T v = unit ? (T)1 : (T)0, dv = 0;
+                                if (rowm) horner(rowm + 4 * (P.nb_m + lp * P.nr2m + np), wm, rsm, v, dv);
+                                const int q = np * P.lsm3 + lp * lp + lp + mp;
+#pragma unroll
+                                for (int qq = 0; qq < NM3X; ++qq)
+                                    if (qq == q) H[qq] = {v * y.re, v * y.im};
+                            }
+                        }
+                    }
+                }
+            }
+            // ---- atomic loop: F from A and U_q = sum W_A R Y (src/forces.F90:497-620, 836-957)
+            {
+                cplx<T> pha[LA + 1];
+                pha[0] = {1, 0};
+#pragma unroll
+                for (int m = 1; m <= LA; ++m) pha[m] = cmul(pha[m - 1], cplx<T>{ex, ey});
+                LegRows<T, LA> Lr;
+                for (int l = 0; l <= P.l3a; ++l) {
+                    Lr.step(l, ez);
+                    T R[NRAX], dR[NRAX];
+#pragma unroll
+                    for (int n = 0; n < NRAX; ++n) {
+                        R[n] = 0;
+                        dR[n] = 0;
+                        if (n < P.nr3a) horner(rowa + 4 * (P.nb_a + l * P.nr3a + n), wa, rsa, R[n], dR[n]);
+                    }
+#pragma unroll
+                    for (int am = 0; am <= LA; ++am) {
+                        if (am > l) break;
+                        cplx<T> Y0, g0[3];
+                        ylm_grad<T>(am, Lr.P[am], Lr.dP[am], pha[am], pha[am > 0 ? am - 1 : 0], ex, ey, ez, Y0, g0);
+#pragma unroll
+                        for (int sg = 0; sg < 2; ++sg) {
+                            if (sg == 1 && am == 0) break;
+                            cplx<T> Y = Y0, g[3] = {g0[0], g0[1], g0[2]};
+                            if (sg) conj_all(Y, g, am);
+                            const int lm = l * l + l + (sg ? -am : am);
+#pragma unroll
+                            for (int n = 0; n < NRAX; ++n) {
+                                if (n >= P.nr3a) break;
+                                const cplx<T>* Wn = W + (n * P.lsa + lm) * P.nm3;
+                                cplx<T> V = {0, 0};
+                                const cplx<T> ry = {R[n] * Y.re, R[n] * Y.im};
+#pragma unroll
+                                for (int q = 0; q < NM3X; ++q) {
+                                    if (q >= P.nm3) break;
+                                    const cplx<T> wq = Wn[q];
+                                    cfma(V, wq, H[q]);
+                                    cfma(U[q], wq, ry);
+                                }
+                                // Re[V (R' Y e + R/r grad Y)]
+                                const T a = dR[n] * re_mul(V, Y), b = R[n] * invr;
+                                Fx += fma(a, ex, b * re_mul(V, g[0]));
+                                Fy += fma(a, ey, b * re_mul(V, g[1]));
+                                Fz += fma(a, ez, b * re_mul(V, g[2]));
+                            }
+                        }
+                    }
+                }
+            }
+            // ---- magnetic loop: A2 part of F and T, A part of T (src/forces.F90:460-491, 793-833)
+            {
+                T ga[NBAX], dga[NBAX];
+#pragma unroll
+                for (int n = 0; n < NBAX; ++n) {
+                    ga[n] = 0;
+                    dga[n] = 0;
+                    if (n < P.nb_a) horner(rowa + 4 * n, wa, rsa, ga[n], dga[n]);
+                }
+                cplx<T> V2[NBAX];
+#pragma unroll
+                for (int n = 0; n < NBAX; ++n) V2[n] = {0, 0};
+                LegRows<T, LM> Lr;
+                for (int lp = 0; lp <= P.l2m; ++lp) {
+                    Lr.step(lp, sz);
+                    T Mv[NR2X], dMv[NR2X];
+#pragma unroll
+                    for (int np = 0; np < NR2X; ++np) {
+                        Mv[np] = unit ? (T)1 : (T)0;
+                        dMv[np] = 0;
+                        if (rowm && np < P.nr2m) horner(rowm + 4 * (P.nb_m + lp * P.nr2m + np), wm, rsm, Mv[np], dMv[np]);
+                    }
+#pragma unroll
+                    for (int am = 0; am <= LM; ++am) {
+                        if (am > lp) break;
+                        cplx<T> Y0, g0[3];
+                        ylm_grad<T>(am, Lr.P[am], Lr.dP[am], phm[am], phm[am > 0 ? am - 1 : 0], sx, sy, sz, Y0, g0);
+#pragma unroll
+                        for (int sg = 0; sg < 2; ++sg) {
+                            if (sg == 1 && am == 0) break;
+                            cplx<T> Y = Y0, g[3] = {g0[0], g0[1], g0[2]};
+                            if (sg) conj_all(Y, g, am);
+                            const int lmp = lp * lp + lp + (sg ? -am : am);
+#pragma unroll
+                            for (int np = 0; np < NR2X; ++np) {
+                                if (np >= P.nr2m) break;
+                                const cplx<T> H2 = {Mv[np] * Y.re, Mv[np] * Y.im};
+                                cplx<T> U2 = {0, 0};
+                                const cplx<T>* W2 = W + P.oA2 + np * P.lsm + lmp;
+#pragma unroll
+                                for (int n = 0; n < NBAX; ++n) {
+                                    if (n >= P.nb_a) break;
+                                    const cplx<T> wv = W2[(size_t)n * P.nr2m * P.lsm];
+                                    cfma(V2[n], wv, H2);
+                                    U2.re = fma(wv.re, ga[n], U2.re);
+                                    U2.im = fma(wv.im, ga[n], U2.im);
+                                }
+                                if (lp <= P.l3m && np < P.nr3m) {
+                                    const int q = np * P.lsm3 + lmp;
+#pragma unroll
+                                    for (int qq = 0; qq < NM3X; ++qq)
+                                        if (qq == q) {
+                                            U2.re += U[qq].re;
+                                            U2.im += U[qq].im;
+                                        }
+                                }
+                                // d(M Y)/dm = M' s Y + M/mu grad Y
+                                const T a = dMv[np] * re_mul(U2, Y), b = Mv[np] * invmu;
+                                Tx += fma(a, sx, b * re_mul(U2, g[0]));
+                                Ty += fma(a, sy, b * re_mul(U2, g[1]));
+                                Tz += fma(a, sz, b * re_mul(U2, g[2]));
+                            }
+                        }
+                    }
+                }
+                // F from A2 (g'_n e) and the core term (src/forces.F90:464-494)
+                T s = 0;
+#pragma unroll
+                for (int n = 0; n < NBAX; ++n)
+                    if (n < P.nb_a) s = fma(dga[n], V2[n].re, s);
+                T cr, dcr;
+                horner(rowa + 4 * (P.nb_a + nfa), wa, rsa, cr, dcr);
+                s += (T)edf * dcr;
+                Fx = fma(s, ex, Fx);
+                Fy = fma(s, ey, Fy);
+                Fz = fma(s, ez, Fz);
+            }
+        }
+        if (active) {
+            flist[3 * (size_t)p] = (double)Fx;
+            flist[3 * (size_t)p + 1] = (double)Fy;
+            flist[3 * (size_t)p + 2] = (double)Fz;
+            tlist[3 * (size_t)p] = (double)Tx;
+            tlist[3 * (size_t)p + 1] = (double)Ty;
+            tlist[3 * (size_t)p + 2] = (double)Tz;
+        }
+    }
+    // ---- self term T0 = dE_i/dm_i (src/forces.F90:712-749)
+    if (lane == 0) {
+        const double mu_d = dmomi[atom];
+        const T invmu = (T)(1.0 / mu_d);
+        const T sx = (T)emomi[3 * atom], sy = (T)emomi[3 * atom + 1], sz = (T)emomi[3 * atom + 2];
+        T wm = 0;
+        bool unit = false;
+        const T* rowm = magnetic_row(P, mu_d, wm, unit, err);
+        const T rsm = (T)P.rs_m;
+        // B1: dE/dg^m_n = dE/drho c1_n
+        const double dE = dEin[atom];
+        T s1 = 0;
+        for (int n = 0; n < P.nb_m; ++n) {
+            T v = unit ? (T)1 : (T)0, dv = 0;
+            if (rowm) horner(rowm + 4 * n, wm, rsm, v, dv);
+            s1 += (T)(dE * P.c1[n]) * dv;
+        }
+        T Tx = s1 * sx, Ty = s1 * sy, Tz = s1 * sz;
+        cplx<T> phm[LM + 1];
+        phm[0] = {1, 0};
+#pragma unroll
+        for (int m = 1; m <= LM; ++m) phm[m] = cmul(phm[m - 1], cplx<T>{sx, sy});
+        LegRows<T, LM> Lr;
+        for (int lp = 0; lp <= P.l2m; ++lp) {
+            Lr.step(lp, sz);
+#pragma unroll
+            for (int am = 0; am <= LM; ++am) {
+                if (am > lp) break;
+                cplx<T> Y0, g0[3];
+                ylm_grad<T>(am, Lr.P[am], Lr.dP[am], phm[am], phm[am > 0 ? am - 1 : 0], sx, sy, sz, Y0, g0);
+#pragma unroll
+                for (int sg = 0; sg < 2; ++sg) {
+                    if (sg == 1 && am == 0) break;
+                    cplx<T> Y = Y0, g[3] = {g0[0], g0[1], g0[2]};
+                    if (sg) conj_all(Y, g, am);
+                    const int mp = sg ? -am : am;
+                    for (int np = 0; np < P.nr2m; ++np) {
+                        T v = unit ? (T)1 : (T)0, dv = 0;
+                        if (rowm) horner(rowm + 4 * (P.nb_m + lp * P.nr2m + np), wm, rsm, v, dv);
+                        const cplx<T> w0 = W[P.oA0 + np * P.lsm + lp * lp + lp + mp];
+                        const T a = dv * re_mul(w0, Y), b = v * invmu;
+                        Tx += fma(a, sx, b * re_mul(w0, g[0]));
+                        Ty += fma(a, sy, b * re_mul(w0, g[1]));
+                        Tz += fma(a, sz, b * re_mul(w0, g[2]));
+                    }
+                }
+            }
+        }
+        tlist0[3 * (size_t)atom] = (double)Tx;
+        tlist0[3 * (size_t)atom + 1] = (double)Ty;
+        tlist0[3 * (size_t)atom + 2] = (double)Tz;
+    }
+}
+
+// ============================================================================ launch
+// shape classes: 0 = small magnetic A block (nm3 <= 4, nr2m <= 2, nb_a <= 16, nr3a <= 8)
+//                1 = wide magnetic A block  (nm3 <= 27, nr2m <= 5, nb_a <= 16, nr3a <= 8)
+int supported_nm3_class(int nm3, int nr2m, int nb_a, int l3a, int l2m, int nr3a) {
+    if (nb_a > 16) return -1;
+    if (nm3 <= 4 && nr2m <= 2 && l3a <= 6 && l2m <= 6 && nr3a <= 8) return 0;
+    if (nm3 <= 27 && nr2m <= 5 && l3a <= 2 && l2m <= 2 && nr3a <= 4) return 1;
+    return -1;
+}
+
+template <typename T, int L, int CH, int NB, int NM3X, int NB2, int NR2X, int NRAX>
+static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, const double* dlist,
+                            const double* elist, const double* dmomi, const double* emomi,
+                            const double* dmom, const double* emom, double* eout, double* flist,
+                            double* tlist, double* tlist0, Buffers& B, hipStream_t s) {
+    DevParams<double> Pd;  // only the dims are used for LDS sizing
+    Pd.nr3a = P.nr3a; Pd.l3a = P.l3a; Pd.nr2m = P.nr2m; Pd.l2m = P.l2m; Pd.lha = P.lha;
+    Pd.lhm = P.lhm; Pd.nm3 = P.nm3; Pd.nb_a = P.nb_a; Pd.nb_m = P.nb_m;
+    const size_t lds1 = density_lds_bytes(Pd, CH, sizeof(T));
+    auto k1 = k_density<T, L, CH, NB, NM3X, NB2, NR2X>;
+    hipError_t e = hipFuncSetAttribute((const void*)k1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k1, dim3(natom), dim3(256), lds1, s, P, off, dlist, elist, dmomi, emomi, dmom,
+                       emom, (cplx<T>*)B.Z, B.ehc, B.rho1, B.err);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const size_t lds2 = 2 * (size_t)(P.nz + 1) * sizeof(cplx<T>) + 8 * sizeof(double);
+    auto k2 = k_adjoint<T>;
+    e = hipFuncSetAttribute((const void*)k2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k2, dim3(natom), dim3(256), lds2, s, P, (const cplx<T>*)B.Z, (cplx<T>*)B.W, B.ehc,
+                       B.rho1, eout, B.dE, B.edf);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    auto k3 = k_forces<T, L, L, NM3X, 16, NRAX, NR2X>;
+    hipLaunchKernelGGL(k3, dim3(natom), dim3(64), 0, s, P, off, dlist, elist, dmomi, emomi, dmom, emom,
+                       (const cplx<T>*)B.W, B.dE, B.edf, flist, tlist, tlist0, B.err);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t Launch<T>::run(const DevParams<T>& P, int natom, const int* off, const double* dlist,
+                          const double* elist, const double* dmomi, const double* emomi,
+                          const double* dmom, const double* emom, double* eout, double* flist,
+                          double* tlist, double* tlist0, Buffers& B, hipStream_t s, int cls) {
+    if (natom <= 0) return hipSuccess;
+    constexpr int CH = sizeof(T) == 8 ? 32 : 64;
+    if (cls == 0)
+        return run_class<T, 6, CH, 4, 4, 4, 2, 8>(P, natom, off, dlist, elist, dmomi, emomi, dmom, emom, eout,
+                                            flist, tlist, tlist0, B, s);
+    return run_class<T, 2, CH, 1, 27, 2, 5, 4>(P, natom, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist,
+                                         tlist, tlist0, B, s);
+}
+
+template struct Launch<double>;
+template struct Launch<float>;
+
+}  // namespace magace

@@ -1,0 +1,62 @@
+// device.hpp -- device-side parameter block and launch interface of the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace magace {
+
+template <typename T>
+struct cplx {
+    T re, im;
+};
+
+// Everything a kernel needs about the potential, passed by value as a kernel argument.
+template <typename T>
+struct DevParams {
+    const T* lut_a = nullptr;  // atomic spline rows [nlut][stride_a]
+    const T* lut_m = nullptr;  // magnetic spline rows [nlut][stride_m]
+    int stride_a = 0, stride_m = 0, nlut = 0;
+    double rs_a = 0, rs_m = 0;  // nlut / cutoff (index arithmetic stays FP64)
+    int nb_a = 0, nb_m = 0, nr2m = 0, l2m = 0, nr3a = 0, nr3m = 0, l3a = 0, l3m = 0;
+    int lsa = 0, lsm = 0, lsm3 = 0, nm3 = 0, oA2 = 0, oA0 = 0, nz = 0;
+    int lha = 0, lhm = 0;  // (l+1)(l+2)/2 for l3a, l2m
+    const uint16_t* tidx = nullptr;  // 4 per term
+    const double* tw = nullptr;
+    int nterms = 0;
+    int nfemb = 0;
+    double p1a = 1, p1b = 1, ecut = 0, decut = 1;
+    double c1[32] = {0};  // coefficients of B1_n = g^m_n(mu_i) - 1
+};
+
+// Error bits raised by kernels (reported by the host API).
+enum : int {
+    kErrSmallDistance = 1,   // r < rcut/nlut ("Encountered very small distance.")
+    kErrMomentRange = 2,     // |m| outside [0, rcut_magnetic*(1+1/nlut)) (reference reads out of bounds)
+    kErrTooManyNeighbors = 4,
+};
+
+struct Buffers {
+    void* Z = nullptr;       // [natom][nz] complex
+    void* W = nullptr;       // [natom][nz] complex
+    double* ehc = nullptr;   // core sum
+    double* rho1 = nullptr;  // B1 part of rho
+    double* dE = nullptr;    // fcut * F'(rho)
+    double* edf = nullptr;   // d f_in/d e_hc * F + 1
+    int* err = nullptr;
+    size_t cap_atoms = 0;
+};
+
+template <typename T>
+struct Launch {
+    static hipError_t run(const DevParams<T>& P, int natom, const int* offsets, const double* dlist,
+                          const double* elist, const double* dmomi, const double* emomi,
+                          const double* dmom, const double* emom, double* eout, double* flist,
+                          double* tlist, double* tlist0, Buffers& B, hipStream_t s, int nm3_class);
+};
+
+hipError_t upload_legendre_constants();
+int supported_nm3_class(int nm3, int nr2m, int nb_a, int l3a, int l2m, int nr3a);
+
+}  // namespace magace

@@ -1,0 +1,103 @@
+// magace.hpp -- host-side model of a magnetic-ACE potential for the MI355X evaluator.
+//
+// Scope: the per-atom energy / force / magnetic-gradient path of libace
+// (src/ace.F90:79-321 -> prepA, prepB, forceweights, forcecompute, torquecompute),
+// single element, body order <= 4, ndensity 1.  The host side parses the potential file,
+// enumerates the basis, builds Clebsch-Gordan couplings and FP64 spline tables, and
+// compiles everything into flat tables the HIP kernels consume.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace magace {
+
+// Potential-file header (src/util.F90:603-653).
+struct Header {
+    int nb_a = 0, nb_m = 0, nr2_m = 0, l2m = 0;   // "nradbase l2max"
+    int nr3_a = 0, nr3_m = 0, l3a = 0, l3m = 0;   // "nradial3 l3max"
+    int nr4_a = 0, nr4_m = 0, l4a = 0, l4m = 0;   // "nradial4 l4max"
+    int nr5_a = 0, nr5_m = 0, l5a = 0, l5m = 0;
+    int nr6_a = 0, nr6_m = 0, l6a = 0, l6m = 0;
+    int ndensity = 1, npot = 0, nfemb = 0, radtype_a = 3, radtype_m = 7;
+    double lambda_a = 0, lambda_m = 0, cut_a = 0, cut_m = 0, dcut_a = 0, dcut_m = 0;
+    double ecut = 0, decut = 0;
+};
+
+// Enumerated basis: index tuples with the column meaning of b1index/b2index/b3index/b4index.
+struct Basis {
+    std::vector<std::array<int, 1>> b1;   // n0'
+    std::vector<std::array<int, 4>> b2;   // n0' n1' n1 l
+    std::vector<std::array<int, 9>> b3;   // n1 n2 n0' n1' n2' l1 l0' l1' l2'
+    std::vector<std::array<int, 15>> b4;  // n1 n2 n3 n0'..n3' l1 l2 l3 l0'..l3' L01'
+};
+
+struct Potential {
+    Header h;
+    Basis basis;
+    std::vector<double> c1, c2, c3, c4;  // expansion coefficients, one per B function
+    std::vector<double> crad_a;          // [nr3_a][l3a+1][nb_a]
+    std::vector<double> crad_m;          // [nr2_m][l2m+1][nb_m]
+    double p1[2] = {1.0, 1.0};           // embedding prefactor / exponent
+    double core[2] = {3.0, 3.0};         // core repulsion prefactor / exponent
+    double rcut = 0, rcut_m = 0;         // global cutoffs from acelibini_
+};
+
+// Per-atom complex feature vector Z = [A | A2 | A0] (all m, m' stored).
+struct ZLayout {
+    int lsa = 0, lsm = 0, lsm3 = 0;  // (l3a+1)^2, (l2m+1)^2, (l3m+1)^2
+    int nm3 = 0;                     // nr3_m * lsm3 (magnetic columns of A)
+    int oA = 0, oA2 = 0, oA0 = 0, nz = 0;
+    int a(int n, int l, int m, int np, int lp, int mp) const {
+        return oA + (n * lsa + l * l + l + m) * nm3 + np * lsm3 + lp * lp + lp + mp;
+    }
+    int a2(int n, int np, int lp, int mp) const { return oA2 + (n * nr2m_ + np) * lsm + lp * lp + lp + mp; }
+    int a0(int np, int lp, int mp) const { return oA0 + np * lsm + lp * lp + lp + mp; }
+    int nr2m_ = 0;
+};
+
+// rho = Re sum_t w_t prod_{s<4} Z[i_ts] with Z[nz] == 1 padding lower-degree terms.
+struct TermList {
+    std::vector<uint16_t> idx;   // 4 per term
+    std::vector<double> w;
+    int n2 = 0, n3 = 0, n4 = 0;  // raw (before merging) m-combination counts per order
+    int raw = 0;
+};
+
+// Spline tables in node-major layout: row n holds [4 coeffs] for every function family.
+struct Splines {
+    int nlut = 10000;                    // src/radial.F90:43
+    int nfa = 0, nfm = 0;                // functions per atomic / magnetic row
+    int atomic_stride = 0, magnetic_stride = 0;  // doubles per row
+    double rs_a = 0, rs_m = 0;           // nlut / cutoff
+    // atomic row: [g_1..g_nb_a | R_{n,l} (n major, l minor) | core], each 4 coefficients
+    std::vector<double> atomic;
+    // magnetic row: [g^m_1..g^m_nb_m | M_{n',l'} (n' major)], each 4 coefficients
+    std::vector<double> magnetic;
+};
+
+struct Model {
+    Potential pot;
+    ZLayout z;
+    TermList terms;
+    Splines spl;
+};
+
+// potential.cpp
+bool parse_potential(const std::string& path, const std::string& element, int xc, double smear,
+                     Potential& pot, std::string& err);
+// basis.cpp
+bool check_header(const Header& h, std::string& err);
+Basis enumerate_basis(const Header& h);
+double clebsch_gordan(int j1, int j2, int J, int m1, int m2, int M);
+ZLayout make_layout(const Header& h);
+TermList build_terms(const Potential& pot, const ZLayout& z);
+// splines.cpp
+void radial_basis(int radtype, double lambda, int nb, double cut, double dcut, double r,
+                  double* g, double* dg);
+void core_repulsion(double r, double pre, double lambda, double cut, double& cr, double& dcr);
+Splines build_splines(const Potential& pot);
+
+}  // namespace magace

@@ -9,7 +9,8 @@ Random parameters (documented, deterministic from ``seed`` with numpy default_rn
   * expansion coefficients c ~ N(0, 1e-2) (standard deviation 1e-2), one per B function;
   * atomic radial coefficients crad_atomic ~ N(0, 1)/sqrt(nradbase_atomic);
   * magnetic radial coefficients crad_magnetic ~ N(0, 1)/sqrt(nradbase_magnetic);
-  * core repulsion off (prefactor 0), embedding linear (nfemb 0), ndensity 1.
+  * by default core repulsion off (prefactor 0), embedding linear (nfemb 0), ndensity 1;
+    headers may override par (p1a, p1b), core (core_pre, core_lambda), ecut/decut.
 """
 from __future__ import annotations
 
@@ -18,7 +19,7 @@ import numpy as np
 BASE = dict(nr5_a=0, nr5_m=0, l5a=0, l5m=0, nr6_a=0, nr6_m=0, l6a=0, l6m=0,
             ndensity=1, npot=5, nfemb=0, radtype_a=3, radtype_m=7,
             lambda_a=5.25, lambda_m=5.25, cut_a=5.0, cut_m=4.0, dcut_a=0.01, dcut_m=0.01,
-            ecut=10000.0, decut=250.0)
+            ecut=10000.0, decut=250.0, p1a=1.0, p1b=1.0, core_pre=0.0, core_lambda=0.0)
 
 #: "nradbase l2max 8 3 2 4 / nradial3 l3max 4 1 4 1 / nradial4 l4max 2 1 2 1" (489 B functions)
 H500 = dict(BASE, nb_a=8, nb_m=3, nr2_m=2, l2m=4, nr3_a=4, nr3_m=1, l3a=4, l3m=1,
@@ -26,7 +27,12 @@ H500 = dict(BASE, nb_a=8, nb_m=3, nr2_m=2, l2m=4, nr3_a=4, nr3_m=1, l3a=4, l3m=1
 #: "10 3 2 6 / 7 1 6 1 / 3 1 3 1" (2043 B functions)
 H2000 = dict(BASE, nb_a=10, nb_m=3, nr2_m=2, l2m=6, nr3_a=7, nr3_m=1, l3a=6, l3m=1,
              nr4_a=3, nr4_m=1, l4a=3, l4m=1)
-#: tiny header for fast unit tests (l3a == l2m as the oracle requires)
+#: the shape of the reference's example potential (example/Fe.pbe.in header lines 8-14:
+#: 16 16 5 2 / 4 3 1 2 / 3 3 0 0, nfemb 1) with synthetic parameters and a nonlinear
+#: embedding exponent
+HFE = dict(BASE, nb_a=16, nb_m=16, nr2_m=5, l2m=2, nr3_a=4, nr3_m=3, l3a=1, l3m=2,
+           nr4_a=3, nr4_m=3, l4a=0, l4m=0, nfemb=1, p1a=1.0, p1b=0.75)
+#: tiny header for fast unit tests
 HTINY = dict(BASE, nb_a=4, nb_m=2, nr2_m=2, l2m=2, nr3_a=2, nr3_m=1, l3a=2, l3m=1,
              nr4_a=2, nr4_m=1, l4a=2, l4m=1)
 
@@ -57,12 +63,12 @@ def render(header: dict, basis: dict, seed: int, element: str = "Fe", xc: str = 
     L.append(" nradial6 l6max %d %d %d %d" % (h["nr6_a"], h["nr6_m"], h["l6a"], h["l6m"]))
     L.append(" ndensity %d" % h["ndensity"])
     L.append(" npot-nfemb-radtype %d %d %d %d" % (h["npot"], h["nfemb"], h["radtype_a"], h["radtype_m"]))
-    for k in (1, 2):
+    for k, v in ((1, h["p1a"]), (2, h["p1b"])):
         L.append(" par %d 1" % k)
-        L.append(" %s 1.0 F" % element)
-    for k in (1, 2):
+        L.append(" %s %s F" % (element, _f(v)))
+    for k, v in ((1, h["core_pre"]), (2, h["core_lambda"])):
         L.append(" core %d 0" % k)
-        L.append(" %s %s 0.0 F" % (element, element))
+        L.append(" %s %s %s F" % (element, element, _f(v)))
     nb_a, nb_m = h["nb_a"], h["nb_m"]
     for nr in range(1, h["nr3_a"] + 1):
         for l in range(0, h["l3a"] + 1):

@@ -1,0 +1,61 @@
+"""Shared test helpers: golden cases, potentials, error metrics."""
+import hashlib
+import os
+
+import numpy as np
+
+import __graft_entry__
+
+PKG = __graft_entry__.load_package()
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+CASES = ("tiny_16", "h500_128", "h2000_16", "hfe_16", "edge_h500core")
+
+HEADERS = {"HTINY": PKG.potgen.HTINY, "H500": PKG.potgen.H500, "H2000": PKG.potgen.H2000,
+           "HFE": PKG.potgen.HFE,
+           "H500CORE": dict(PKG.potgen.H500, core_pre=25.0, core_lambda=0.35, ecut=14.0, decut=8.0,
+                            nfemb=2, p1a=1.0, p1b=0.9)}
+
+
+def potential_text(header_name, seed, enum="product"):
+    h = HEADERS[header_name]
+    if enum == "product":
+        basis = PKG.capi.enumerate_basis(h)
+    else:
+        import oracle as O
+        o = O.Oracle.from_header(h)
+        basis = {k: o.basis(k) for k in (1, 2, 3, 4)}
+    return PKG.potgen.render(h, basis, seed)
+
+
+def write_potential(path, header_name, seed, enum="product", sha=None):
+    text = potential_text(header_name, seed, enum)
+    if sha is not None:
+        assert hashlib.sha256(text.encode()).hexdigest() == sha, "regenerated potential differs"
+    with open(path, "w") as f:
+        f.write(text)
+    return path
+
+
+def load_case(name, tmpdir, enum="product"):
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    a = PKG.systems.pack_acelib(z["pos"], z["cell"], z["sdir"], z["smag"], float(z["rlist"]))
+    pot = write_potential(os.path.join(str(tmpdir), name + ".pbe.in"), str(z["header"]), int(z["pot_seed"]),
+                          enum, str(z["pot_sha256"]))
+    ref = {k: z[k] for k in ("eout", "flist", "tlist", "tlist0")}
+    return dict(a=a, ref=ref, pot=pot, header=str(z["header"]), z=z)
+
+
+def relerr(got, ref):
+    """Norm-wise relative error ||got-ref||_inf / ||ref||_inf."""
+    s = np.abs(ref).max() if np.size(ref) else 0.0
+    d = np.abs(np.asarray(got) - np.asarray(ref)).max() if np.size(ref) else 0.0
+    return d / s if s > 0 else d
+
+
+def per_atom_errors(a, got, ref):
+    """Max |dE_i| (eV), max |dF_i| (eV/A), max |d fm_i| (eV/muB) after caller assembly."""
+    s = PKG.systems
+    e1, f1, m1, _ = s.assemble(a, got["eout"], got["flist"], got["tlist"], got["tlist0"])
+    e0, f0, m0, _ = s.assemble(a, ref["eout"], ref["flist"], ref["tlist"], ref["tlist0"])
+    return (np.abs(got["eout"] - ref["eout"]).max(), np.abs(f1 - f0).max(), np.abs(m1 - m0).max())

@@ -1,0 +1,158 @@
+"""GPU parity: libmagace.so on the MI355X against the reference (golden vectors written by
+the reference libace) and the oracle, through the C-ABI.
+
+Bars: FP64 path <= 1e-10 norm-wise relative error per output array (eout, flist, tlist,
+tlist0); mixed path: force error <= 1e-4 eV/A and magnetic-gradient error <= 1e-4 eV/muB per
+atom after caller assembly, scaled to the output magnitude where the synthetic potentials
+make forces large (tolerances written in each test).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import CASES, PKG, load_case, per_atom_errors, relerr, write_potential
+
+capi = PKG.capi
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+if not torch.cuda.is_available():
+    pytest.skip("no GPU visible", allow_module_level=True)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_fp64_matches_reference(name, tmp_path):
+    say("case", name, "fp64")
+    c = load_case(name, tmp_path)
+    capi.init_file(c["pot"])
+    capi.set_precision(capi.FP64)
+    got = capi.acelib(c["a"])
+    errs = {k: relerr(got[k], c["ref"][k]) for k in ("eout", "flist", "tlist", "tlist0")}
+    say("  rel errors", errs)
+    for k, e in errs.items():
+        assert e <= 1e-10, (k, e)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_mixed_precision_error(name, tmp_path):
+    say("case", name, "mixed")
+    c = load_case(name, tmp_path)
+    capi.init_file(c["pot"])
+    capi.set_precision(capi.MIXED)
+    got = capi.acelib(c["a"])
+    capi.set_precision(capi.FP64)
+    de, df, dm = per_atom_errors(c["a"], got, c["ref"])
+    _, f0, m0, _ = PKG.systems.assemble(c["a"], c["ref"]["eout"], c["ref"]["flist"], c["ref"]["tlist"],
+                                        c["ref"]["tlist0"])
+    fs, ms = np.abs(f0).max(), np.abs(m0).max()
+    say("  max|dE_i| %.3e  max|dF_i| %.3e (scale %.3e)  max|dfm_i| %.3e (scale %.3e)" % (de, df, fs, dm, ms))
+    # absolute 1e-4 or 2e-6 relative to the largest component, whichever is larger
+    assert df <= max(1e-4, 2e-6 * fs)
+    assert dm <= max(1e-4, 2e-6 * ms)
+    assert de <= max(1e-4, 2e-6 * np.abs(c["ref"]["eout"]).max())
+
+
+def test_device_entry_point_matches_acelib(tmp_path):
+    say("case h500_128 device entry")
+    c = load_case("h500_128", tmp_path)
+    a = c["a"]
+    capi.init_file(c["pot"])
+    capi.set_precision(capi.FP64)
+    host = capi.acelib(a)
+    dev = torch.device("cuda:0")
+    t = lambda x, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(x), dtype=dt, device=dev)
+    off = t(a["offsets"], torch.int32)
+    ins = [t(a[k]) for k in ("dlist", "elist", "dlist_momi", "elist_momi", "dlist_mom", "elist_mom")]
+    N, P = a["natom"], len(a["dlist"])
+    outs = [torch.zeros(N, dtype=torch.float64, device=dev), torch.zeros(3 * P, dtype=torch.float64, device=dev),
+            torch.zeros(3 * P, dtype=torch.float64, device=dev), torch.zeros(3 * N, dtype=torch.float64, device=dev)]
+    stream = torch.cuda.current_stream().cuda_stream
+    capi.compute_device(N, P, off.data_ptr(), *[x.data_ptr() for x in ins], *[x.data_ptr() for x in outs],
+                        stream=stream)
+    capi.check_errors(stream)
+    for k, o in zip(("eout", "flist", "tlist", "tlist0"), outs):
+        assert np.array_equal(o.cpu().numpy(), host[k]), k
+
+
+def test_pair_compute_matches_python_assembly(tmp_path):
+    """C++ caller (PairSpinACE::compute restated) against the numpy assembly."""
+    say("case tiny_16 pair_compute")
+    c = load_case("tiny_16", tmp_path)
+    capi.init_file(c["pot"])
+    capi.set_precision(capi.FP64)
+    z = c["z"]
+    pos, cell = z["pos"], z["cell"]
+    # explicit ghost atoms: images of every atom within the list cutoff
+    a = c["a"]
+    L = np.diag(cell)
+    shifts = a["pair_i"] * 0
+    i, j = a["pair_i"], a["pair_j"]
+    d = -(a["elist"].reshape(-1, 3) * a["dlist"][:, None])  # x_j(image) - x_i
+    img = pos[i] + d
+    n = len(pos)
+    ghosts, neigh = [], [[] for _ in range(n)]
+    for p in range(len(i)):
+        if np.allclose(img[p], pos[j[p]]):
+            neigh[i[p]].append(j[p])
+        else:
+            neigh[i[p]].append(n + len(ghosts))
+            ghosts.append(j[p])
+    x = np.concatenate([pos, img[[p for p in range(len(i)) if not np.allclose(img[p], pos[j[p]])]]])
+    owner = np.concatenate([np.arange(n), np.array(ghosts, int)])
+    sp = np.concatenate([z["sdir"][owner], z["smag"][owner][:, None]], axis=1)
+    e, f, fm, vir, eatom = capi.pair_compute(x, sp, np.arange(n), neigh, nlocal=n, newton_pair=1)
+    # fold ghost forces back onto their owners, as LAMMPS reverse communication does
+    ff = np.zeros((n, 3))
+    fmm = np.zeros((n, 3))
+    np.add.at(ff, owner, f)
+    np.add.at(fmm, owner, fm)
+    e0, f0, m0, vir0 = PKG.systems.assemble(a, c["ref"]["eout"], c["ref"]["flist"], c["ref"]["tlist"],
+                                            c["ref"]["tlist0"])
+    assert abs(e - e0) < 1e-10 * max(1, abs(e0))
+    assert np.abs(ff - f0).max() < 1e-10 * max(1, np.abs(f0).max())
+    assert np.abs(fmm - m0).max() < 1e-10 * max(1, np.abs(m0).max())
+    v = np.array([vir0[0, 0], vir0[1, 1], vir0[2, 2], vir0[0, 1], vir0[0, 2], vir0[1, 2]])
+    assert np.abs(vir - v).max() < 1e-10 * max(1, np.abs(v).max())
+
+
+@pytest.mark.parametrize("hname,ncell,spins", [("H500", 10, "uniform"), ("H2000", 5, "cone")])
+def test_larger_system_against_oracle(hname, ncell, spins, tmp_path):
+    """2000-atom bcc Fe (configs[1] shape) and 250 atoms at the lmax-6 header vs the oracle."""
+    say("case", hname, ncell, spins)
+    pot = write_potential(str(tmp_path / "p.in"), hname, 2027)
+    s = PKG.systems.make_system(ncell, 2027, spins)
+    a = PKG.systems.pack_acelib(s["pos"], s["cell"], s["sdir"], s["smag"], 5.0)
+    capi.init_file(pot)
+    capi.set_precision(capi.FP64)
+    got = capi.acelib(a)
+    ref = O.Oracle.load(pot).acelib(a, nthreads=16)
+    for k in ("eout", "flist", "tlist", "tlist0"):
+        assert relerr(got[k], ref[k]) <= 1e-10, k
+
+
+def test_empty_call_and_error_flags(tmp_path):
+    say("case empty / small distance")
+    pot = write_potential(str(tmp_path / "p.in"), "HTINY", 3)
+    capi.init_file(pot)
+    empty = dict(nstart=np.zeros(0, np.int32), nstop=np.zeros(0, np.int32), nlist=np.zeros(0, np.int32),
+                 occ=np.zeros(0, np.int32), occlist=np.zeros(0, np.int32), dlist=np.zeros(0), elist=np.zeros(0),
+                 dlist_momi=np.zeros(0), elist_momi=np.zeros(0), dlist_mom=np.zeros(0), elist_mom=np.zeros(0))
+    out = capi.acelib(empty)
+    assert out["eout"].size == 0
+    # one pair closer than rcut/nlut: the reference stops ("Encountered very small distance.")
+    a = dict(nstart=np.array([1, 2], np.int32), nstop=np.array([1, 1], np.int32), nlist=np.array([2], np.int32),
+             occ=np.ones(2, np.int32), occlist=np.ones(1, np.int32), dlist=np.array([1e-4]),
+             elist=np.array([0.0, 0.0, 1.0]), dlist_momi=np.array([2.2, 2.2]),
+             elist_momi=np.array([0, 0, 1.0, 0, 0, 1.0]), dlist_mom=np.array([2.2]), elist_mom=np.array([0, 0, 1.0]))
+    with pytest.raises(capi.MagaceError, match="small distance"):
+        capi.acelib(a)
+    # the library stays usable
+    a["dlist"] = np.array([2.5])
+    capi.acelib(a)

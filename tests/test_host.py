@@ -1,0 +1,158 @@
+"""CPU tests of the product's host side (no GPU): the C-ABI library, the C++ basis
+enumeration, the merged monomial table, potential-file handling and error paths."""
+import hashlib
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import CASES, GOLDEN, HEADERS, PKG, potential_text, write_potential
+
+capi = PKG.capi
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "magace.h")).read()
+    declared = set(re.findall(r"^\s*(?:void|int|const char \*)\s*\*?\s*(\w+)\s*\(", header, re.M))
+    assert declared == set(capi.EXPORTS)
+    out = subprocess.run(["nm", "-D", "--defined-only", capi.LIBPATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    missing = declared - exported
+    assert not missing, missing
+    L = capi.lib()
+    for name in declared:
+        assert hasattr(L, name)
+
+
+@pytest.mark.parametrize("hname", sorted(HEADERS))
+def test_enumeration_matches_oracle(hname):
+    h = HEADERS[hname]
+    mine = capi.enumerate_basis(h)
+    o = O.Oracle.from_header(h)
+    for k in (1, 2, 3, 4):
+        assert np.array_equal(mine[k], o.basis(k)), (hname, k)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_potential_regenerates_bitwise(name):
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    text = potential_text(str(z["header"]), int(z["pot_seed"]), enum="product")
+    assert hashlib.sha256(text.encode()).hexdigest() == str(z["pot_sha256"])
+
+
+def test_reference_example_potential_counts():
+    path = "/root/reference/example/Fe.pbe.in"
+    if not os.path.exists(path):
+        pytest.skip("reference tree not mounted")
+    capi.init_file(path, rcut=4.5, rcut_magnetic=4.0, smear=0.2)
+    assert capi.basis_counts() == [16, 720, 4932, 495]
+
+
+def _clu1(text):
+    lines = text.splitlines()
+    return [float(lines[i + 1].split()[2]) for i, l in enumerate(lines) if l.split()[:2] == ["clu", "1"]]
+
+
+def _zvector(o, header, a, i, nz, oA2, oA0):
+    """Z = [A | A2 | A0] of atom i from the oracle's splines and spherical harmonics."""
+    h = dict(PKG.potgen.BASE, **header)
+    lsa, lsm, lsm3 = (h["l3a"] + 1) ** 2, (h["l2m"] + 1) ** 2, (h["l3m"] + 1) ** 2
+    nm3 = h["nr3_m"] * lsm3
+    Z = np.zeros(nz + 1, complex)
+    Z[nz] = 1.0
+    lo, hi = a["nstart"][i] - 1, a["nstop"][i]
+    for p in range(lo, hi):
+        g, _ = o.spline(0, a["dlist"][p])
+        R, _ = o.spline(1, a["dlist"][p])          # [n*(l3a+1) + l]
+        M, _ = o.spline(4, a["dlist_mom"][p])      # [n'*(l2m+1) + l']
+        Y, _ = O.ylm(h["l3a"], a["elist"][3 * p:3 * p + 3])
+        Ys, _ = O.ylm(h["l2m"], a["elist_mom"][3 * p:3 * p + 3])
+        for n in range(h["nr3_a"]):
+            for l in range(h["l3a"] + 1):
+                for m in range(-l, l + 1):
+                    lm = l * l + l + m
+                    ry = R[n * (h["l3a"] + 1) + l] * Y[lm]
+                    for npr in range(h["nr3_m"]):
+                        for lp in range(h["l3m"] + 1):
+                            for mp in range(-lp, lp + 1):
+                                q = npr * lsm3 + lp * lp + lp + mp
+                                Z[(n * lsa + lm) * nm3 + q] += ry * M[npr * (h["l2m"] + 1) + lp] * Ys[lp * lp + lp + mp]
+        for n in range(h["nb_a"]):
+            for npr in range(h["nr2_m"]):
+                for lp in range(h["l2m"] + 1):
+                    for mp in range(-lp, lp + 1):
+                        Z[oA2 + (n * h["nr2_m"] + npr) * lsm + lp * lp + lp + mp] += \
+                            g[n] * M[npr * (h["l2m"] + 1) + lp] * Ys[lp * lp + lp + mp]
+    M, _ = o.spline(4, a["dlist_momi"][i])
+    Ys, _ = O.ylm(h["l2m"], a["elist_momi"][3 * i:3 * i + 3])
+    for npr in range(h["nr2_m"]):
+        for lp in range(h["l2m"] + 1):
+            for mp in range(-lp, lp + 1):
+                Z[oA0 + npr * lsm + lp * lp + lp + mp] = M[npr * (h["l2m"] + 1) + lp] * Ys[lp * lp + lp + mp]
+    return Z
+
+
+@pytest.mark.parametrize("name", ["tiny_16", "h2000_16"])
+def test_monomial_table_reproduces_the_reference_energy(name, tmp_path):
+    """The merged table rho = Re sum_t w_t prod Z replaces the B-function loops of
+    src/b_basis.F90:91-307; with linear embedding and no core E_i = rho_i exactly."""
+    from helpers import load_case
+    c = load_case(name, tmp_path)
+    header = HEADERS[c["header"]]
+    capi.init_file(c["pot"])
+    info = capi.model_info()
+    idx, w = capi.model_terms()
+    assert info["terms"] == len(w) and info["terms"] < info["raw"]
+    o = O.Oracle.load(c["pot"])
+    c1 = _clu1(open(c["pot"]).read())
+    for i in (0, 7):
+        Z = _zvector(o, header, c["a"], i, info["nz"], info["oA2"], info["oA2"] + header["nb_a"] * header["nr2_m"] * (header["l2m"] + 1) ** 2)
+        rho = np.real(np.sum(w * np.prod(Z[idx], axis=1)))
+        gm, _ = o.spline(3, c["a"]["dlist_momi"][i])
+        rho += np.dot(c1, gm - 1.0)
+        assert abs(rho - c["ref"]["eout"][i]) < 1e-12 * max(1.0, abs(c["ref"]["eout"][i]))
+
+
+def test_init_error_paths(tmp_path):
+    pot = write_potential(str(tmp_path / "p.in"), "HTINY", 1)
+    with pytest.raises(capi.MagaceError, match="smearing"):
+        capi.init_file(pot, smear=0.3)
+    with pytest.raises(capi.MagaceError, match="element"):
+        capi.init_file(pot, element="Co")
+    with pytest.raises(capi.MagaceError, match="cutoff"):
+        capi.init_file(pot, rcut=4.0)
+    with pytest.raises(capi.MagaceError, match="not found"):
+        capi.init_file(str(tmp_path / "missing.in"))
+    bad = open(pot).read().replace("nradial5 l5max 0 0 0 0", "nradial5 l5max 1 1 0 0")
+    open(tmp_path / "b.in", "w").write(bad)
+    with pytest.raises(capi.MagaceError, match="body order"):
+        capi.init_file(str(tmp_path / "b.in"))
+    capi.init_file(pot)  # a good file still loads after failures
+
+
+def test_acelibini_reads_cwd_and_enforces_single_element(tmp_path):
+    """acelibini_ semantics in a fresh process: <El>.<xc>.in from the CWD, nele == 1."""
+    write_potential(str(tmp_path / "Fe.pbe.in"), "HTINY", 1)
+    code = r'''
+import sys
+sys.path.insert(0, %r)
+import __graft_entry__
+capi = __graft_entry__.load_package().capi
+try:
+    capi.acelibini(2, 5.0, 4.0, 0.2, 1, [27, 28])
+    print("no error")
+except capi.MagaceError as e:
+    print("multi:", e)
+capi.acelibini(1, 5.0, 4.0, 0.2, 1, [27])
+print("counts", capi.basis_counts())
+''' % ROOT
+    p = subprocess.run([sys.executable, "-c", code], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert "multi: Multiple elements" in p.stdout
+    assert "counts [2, 36, 30, 186]" in p.stdout
