@@ -1,0 +1,200 @@
+"""Benchmark: atom-steps/s of E + F + magnetic gradients for 16000-atom bcc Fe on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): bcc Fe 20x20x20 conventional
+cells (16000 atoms, a = 2.83 A, periodic), positions jittered N(0, 0.05 A), ferromagnetic
+spins along z with a random cone angle U[0, 30 deg], magnitudes N(2.2, 0.1) muB, r_c 5.0 A;
+potential header "10 3 2 6 / 7 1 6 1 / 3 1 3 1" (lmax 6, nradmax 10, 2043 B functions),
+coefficients N(0, 1e-2), seed 2028.  A step is one full evaluation (libace acelib_ semantics:
+pair list prebuilt, inputs resident in HBM): energies, pair forces, pair and self magnetic
+gradients.  The mixed path is the headline value; the FP64 path runs on the same inputs and
+the mixed path's error against it is reported.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+import __graft_entry__  # noqa: E402
+
+SEED = 2028
+PEAK = {"mixed": 157.3, "fp64": 78.6}  # TFLOP/s, MI355X published vector FP32 / FP64
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(pkg, pot, a, natom_sample, use_reference=True):
+    """Reference libace (oracle/_ref, serial Fortran) on the first natom_sample atoms of the
+    same workload; falls back to the oracle's C restatement (OpenMP) if the reference
+    binary is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    S = natom_sample
+    P = int(a["offsets"][S])
+    sub = dict(nstart=a["nstart"][:S], nstop=a["nstop"][:S], nlist=a["nlist"][:P], occ=a["occ"][:S],
+               occlist=a["occlist"][:P], dlist=a["dlist"][:P], elist=a["elist"][:3 * P],
+               dlist_momi=a["dlist_momi"][:S], elist_momi=a["elist_momi"][:3 * S], dlist_mom=a["dlist_mom"][:P],
+               elist_mom=a["elist_mom"][:3 * P], nall=S)
+    if use_reference and os.path.exists(O.REF_DRIVER):
+        out = O.RefDriver(pot).run(sub, nreps=2, timeout=900)
+        t = float(np.median(out["seconds"]))
+        return dict(value=S / t, unit="atom-steps/s", cores=1, kind="reference",
+                    sample="%d of the 16000 atoms (%d pairs), reference libace built from its Fortran "
+                           "sources with AMD flang -O2, serial, median of 2 acelib_ calls" % (S, P))
+    nthr = min(16, os.cpu_count() or 1)
+    o = O.Oracle.load(pot)
+    o.acelib(sub, nthreads=nthr)
+    t0 = time.perf_counter()
+    o.acelib(sub, nthreads=nthr)
+    t = time.perf_counter() - t0
+    return dict(value=S / t, unit="atom-steps/s", cores=nthr, kind="port",
+                sample="%d of the 16000 atoms, C restatement of the reference (OpenMP)" % S)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--ncell", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=400)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build evaluates on one GPU only; multi-GPU runs are not part of it "
+              "(see DESIGN.md for how the path would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    pkg = __graft_entry__.load_package()
+    capi, potgen, systems = pkg.capi, pkg.potgen, pkg.systems
+    import importlib
+    flops = importlib.import_module("magace_amd.flops")
+
+    H = potgen.H2000
+    td = tempfile.mkdtemp(prefix="magace_bench_")
+    pot = os.path.join(td, "Fe.pbe.in")
+    basis = capi.enumerate_basis(H)
+    potgen.write(pot, H, basis, SEED)
+    t0 = time.perf_counter()
+    s = systems.make_system(args.ncell, SEED, spins="cone")
+    a = systems.pack_acelib(s["pos"], s["cell"], s["sdir"], s["smag"], 5.0)
+    N, P = int(a["natom"]), int(len(a["dlist"]))
+    log("system: %d atoms, %d pairs (%.1f neighbours/atom), built in %.1f s" %
+        (N, P, P / N, time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    capi.init_file(pot)
+    info = capi.model_info()
+    log("init: %.2f s, %d monomials (from %d m-combinations), feature vector %d" %
+        (time.perf_counter() - t0, info["terms"], info["raw"], info["nz"]))
+
+    T = lambda x, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(x), dtype=dt, device=dev)
+    off = T(a["offsets"], torch.int32)
+    ins = [T(a[k]) for k in ("dlist", "elist", "dlist_momi", "elist_momi", "dlist_mom", "elist_mom")]
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def alloc():
+        return [torch.zeros(N, dtype=torch.float64, device=dev), torch.zeros(3 * P, dtype=torch.float64, device=dev),
+                torch.zeros(3 * P, dtype=torch.float64, device=dev), torch.zeros(3 * N, dtype=torch.float64, device=dev)]
+
+    def step(outs):
+        capi.compute_device(N, P, off.data_ptr(), *[x.data_ptr() for x in ins], *[x.data_ptr() for x in outs],
+                            stream=stream)
+
+    res = {}
+    for prec, mode in (("fp64", capi.FP64), ("mixed", capi.MIXED)):
+        capi.set_precision(mode)
+        outs = alloc()
+        for _ in range(args.warmup):
+            step(outs)
+        capi.check_errors(stream)
+        torch.cuda.synchronize()
+        capi.profile(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step(outs)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        kms, cnt = capi.profile_read()
+        capi.profile(False)
+        capi.check_errors(stream)
+        res[prec] = dict(ms_per_step=1e3 * dt / args.steps, value=N * args.steps / dt,
+                         kernel_ms=(kms / max(cnt, 1)).tolist(),
+                         out={k: o.cpu().numpy() for k, o in zip(("eout", "flist", "tlist", "tlist0"), outs)})
+        log("%s: %.3f ms/step, %.3e atom-steps/s, kernels [density, adjoint, forces] ms %s" %
+            (prec, res[prec]["ms_per_step"], res[prec]["value"], np.round(kms / max(cnt, 1), 4).tolist()))
+
+    o64, o32 = res["fp64"]["out"], res["mixed"]["out"]
+    _, f64, m64, _ = systems.assemble(a, o64["eout"], o64["flist"], o64["tlist"], o64["tlist0"])
+    _, f32, m32, _ = systems.assemble(a, o32["eout"], o32["flist"], o32["tlist"], o32["tlist0"])
+    err = dict(energy_per_atom_max=float(np.abs(o32["eout"] - o64["eout"]).max()),
+               force_max=float(np.abs(f32 - f64).max()), torque_max=float(np.abs(m32 - m64).max()),
+               force_scale=float(np.abs(f64).max()), torque_scale=float(np.abs(m64).max()))
+    log("mixed vs fp64:", err)
+
+    raw = {2: info["raw2"], 3: info["raw3"], 4: info["raw4"]}
+    fm = flops.flops_model(H, basis, raw, P / N)
+    kernels = ("density", "adjoint", "forces")
+    km = res["mixed"]["kernel_ms"]
+    dom = int(np.argmax(km))
+    achieved = fm[kernels[dom]] * N / (km[dom] * 1e-3) / 1e12
+    total_tf = fm["total"] * res["mixed"]["value"] / 1e12
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(pkg, pot, a, min(args.cpu_sample, N))
+            log("cpu baseline:", cpu)
+        except Exception as e:  # the headline line must still be printed
+            log("cpu baseline failed:", e)
+
+    line = {
+        "metric": "atom-steps/s E+F+torques, 16k Fe atoms: mixed vs FP64 @ force err <=1e-4 eV/A",
+        "value": res["mixed"]["value"],
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["mixed"]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64 (f32 splines/Y_lm/A/B products, f64 rho/energy/outputs)",
+        "data": "synthetic (bcc Fe lattice, random coefficients from seed 2028)",
+        "config": {"workload": "bcc Fe %dx%dx%d (%d atoms, %d pairs), cone spins <=30 deg, r_c 5.0 A, "
+                               "header 10 3 2 6 / 7 1 6 1 / 3 1 3 1 (2043 B functions), E+F+dE/dm per step"
+                               % (args.ncell, args.ncell, args.ncell, N, P),
+                   "natoms": N, "pairs": P, "parallelism": "single GPU"},
+        "fp64": {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms_per_step"],
+                 "kernel_ms": res["fp64"]["kernel_ms"]},
+        "mixed_over_fp64": res["mixed"]["value"] / res["fp64"]["value"],
+        "mixed_error_vs_fp64": err,
+        "kernel_ms": dict(zip(kernels, km)),
+        "flops_per_atom_step": fm,
+        "achieved_tflops_total": total_tf,
+        "roofline": {"bound": "valu", "kernel": kernels[dom], "achieved": achieved, "peak": PEAK["mixed"],
+                     "unit": "TFLOP/s", "frac": achieved / PEAK["mixed"], "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["gpu_over_cpu"] = res["mixed"]["value"] / cpu["value"]
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -81,6 +81,13 @@ int magace_pair_compute(int nlocal, int nghost, const double *x, const double *s
                         const int *firstneigh_offsets, const int *firstneigh, int newton_pair,
                         double *f, double *fm, double *energy, double *virial, double *eatom);
 
+/* Per-kernel timing with HIP events recorded on the evaluation stream around each of the
+ * three kernels (density, adjoint, forces).  magace_profile(1) starts recording (and clears);
+ * magace_profile_read() waits for the recorded evaluations and returns the summed
+ * milliseconds per kernel in ms[3] and the number of evaluations in *count. */
+int magace_profile(int enable);
+int magace_profile_read(double *ms, int *count);
+
 /* Basis introspection: counts of B functions of orders 1..4 of the loaded potential
  * (the numbers src/util.F90:1831,1929,2235,2596 print). */
 int magace_basis_counts(int *nvar4);
@@ -88,7 +95,8 @@ int magace_basis_counts(int *nvar4);
  * l3a l3m nr4_a nr4_m l4a l4m.  Writes index tuples (b1index/b2index/b3index/b4index columns)
  * when cap suffices; returns the count, or a negative error code. */
 int magace_enumerate(const int *hdr, int order, int *out, int cap);
-/* info[0..7] = nz, merged terms, raw m-combinations, raw order 2/3/4, shape class, oA2 */
+/* info[0..9] = nz, merged terms, raw m-combinations, raw order 2/3/4, shape class, oA2,
+ * gradient contributions, gradient chunks */
 int magace_model_info(long long *info, int n);
 /* The merged monomial table rho = Re sum_t w_t prod_s Z[idx[4t+s]] (Z[nz] = 1) that replaces
  * the per-B m-combination loops of src/b_basis.F90:91-307; copies when cap >= count and

@@ -24,7 +24,7 @@ _DP = C.POINTER(C.c_double)
 EXPORTS = ("acelibini_", "acelib_", "magace_last_error", "magace_error_string", "magace_init_file",
            "magace_finalize", "magace_set_precision", "magace_get_precision", "magace_compute_device",
            "magace_check_errors", "magace_basis_counts", "magace_enumerate", "magace_model_info",
-           "magace_pair_compute", "magace_model_terms")
+           "magace_pair_compute", "magace_model_terms", "magace_profile", "magace_profile_read")
 
 
 class MagaceError(RuntimeError):
@@ -146,9 +146,9 @@ def enumerate_basis(header: dict):
 
 
 def model_info():
-    v = np.zeros(8, np.int64)
-    _check(lib().magace_model_info(v, 8))
-    keys = ("nz", "terms", "raw", "raw2", "raw3", "raw4", "shape_class", "oA2")
+    v = np.zeros(10, np.int64)
+    _check(lib().magace_model_info(v, 10))
+    keys = ("nz", "terms", "raw", "raw2", "raw3", "raw4", "shape_class", "oA2", "contribs", "chunks")
     return dict(zip(keys, (int(x) for x in v)))
 
 
@@ -194,3 +194,17 @@ def model_terms():
     w = np.zeros(max(n, 1))
     L.magace_model_terms(idx.ctypes.data, w.ctypes.data, n)
     return idx[:n].astype(np.int64), w[:n]
+
+
+def profile(enable=True):
+    _check(lib().magace_profile(1 if enable else 0))
+
+
+def profile_read():
+    """(ms per kernel summed over evaluations [density, adjoint, forces], evaluations)."""
+    L = lib()
+    L.magace_profile_read.argtypes = [_D, C.POINTER(C.c_int)]
+    ms = np.zeros(3)
+    n = C.c_int(0)
+    _check(L.magace_profile_read(ms, C.byref(n)))
+    return ms, n.value

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <array>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -59,8 +60,9 @@ struct Context {
     DevParams<float> P32;
     DevTables<double> t64;
     DevTables<float> t32;
-    uint16_t* tidx = nullptr;
-    double* tw = nullptr;
+    uint4* crec64 = nullptr;
+    uint4* crec32 = nullptr;
+    int2* chunks = nullptr;
     Buffers buf;
     // staging for acelib_ host arrays
     size_t cap_pairs = 0, cap_atoms = 0;
@@ -71,6 +73,10 @@ struct Context {
     std::vector<int> h_off;
     hipStream_t stream = nullptr;
     bool uploaded = false;
+    // per-kernel timing (magace_profile): 4 events per evaluation
+    bool profiling = false;
+    std::vector<std::array<hipEvent_t, 4>> events;
+    size_t events_used = 0;
     ~Context();
 };
 
@@ -94,10 +100,12 @@ void dfree(T*& p) {
 
 Context::~Context() {
     dfree(t64.lut_a); dfree(t64.lut_m); dfree(t32.lut_a); dfree(t32.lut_m);
-    dfree(tidx); dfree(tw);
+    dfree(crec64); dfree(crec32); dfree(chunks);
     dfree(buf.Z); dfree(buf.W); dfree(buf.ehc); dfree(buf.rho1); dfree(buf.dE); dfree(buf.edf); dfree(buf.err);
     dfree(d_off); dfree(d_dl); dfree(d_el); dfree(d_dm); dfree(d_em); dfree(d_dmi); dfree(d_emi);
     dfree(d_eout); dfree(d_fl); dfree(d_tl); dfree(d_t0);
+    for (auto& e : events)
+        for (auto& x : e) (void)hipEventDestroy(x);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -115,7 +123,7 @@ void fill_params(DevParams<T>& P, const Model& M) {
     P.oA2 = M.z.oA2; P.oA0 = M.z.oA0; P.nz = M.z.nz;
     P.lha = (h.l3a + 1) * (h.l3a + 2) / 2;
     P.lhm = (h.l2m + 1) * (h.l2m + 2) / 2;
-    P.nterms = (int)M.terms.w.size();
+    P.nchunks = (int)(M.terms.chunks.size() / 2);
     P.nfemb = h.nfemb;
     P.p1a = M.pot.p1[0];
     P.p1b = M.pot.p1[1];
@@ -176,19 +184,35 @@ int upload(Context& c) {
     if (upload_table(c.t64.lut_m, M.spl.magnetic)) return MAGACE_ERR_DEVICE;
     if (upload_table(c.t32.lut_a, M.spl.atomic)) return MAGACE_ERR_DEVICE;
     if (upload_table(c.t32.lut_m, M.spl.magnetic)) return MAGACE_ERR_DEVICE;
-    const size_t nt = M.terms.w.size();
-    HIPCHK(hipMalloc((void**)&c.tidx, nt * 4 * sizeof(uint16_t) + 64));
-    HIPCHK(hipMalloc((void**)&c.tw, nt * sizeof(double) + 64));
-    if (nt) {
-        HIPCHK(hipMemcpy(c.tidx, M.terms.idx.data(), nt * 4 * sizeof(uint16_t), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c.tw, M.terms.w.data(), nt * sizeof(double), hipMemcpyHostToDevice));
+    const size_t nc = M.terms.cw.size(), nch = M.terms.chunks.size() / 2;
+    HIPCHK(hipMalloc((void**)&c.crec64, nc * sizeof(uint4) + 64));
+    HIPCHK(hipMalloc((void**)&c.crec32, nc * sizeof(uint4) + 64));
+    HIPCHK(hipMalloc((void**)&c.chunks, nch * sizeof(int2) + 64));
+    if (nc) {
+        std::vector<uint4> r64(nc), r32(nc);
+        for (size_t i = 0; i < nc; ++i) {
+            const uint16_t* q = &M.terms.cidx[4 * i];
+            const double w = M.terms.cw[i];
+            uint64_t wb;
+            std::memcpy(&wb, &w, 8);
+            const float wf = (float)w;
+            uint32_t fb;
+            std::memcpy(&fb, &wf, 4);
+            r64[i] = make_uint4((uint32_t)q[0] | ((uint32_t)q[1] << 16), q[2], (uint32_t)(wb & 0xffffffffu),
+                                (uint32_t)(wb >> 32));
+            r32[i] = make_uint4((uint32_t)q[0] | ((uint32_t)q[1] << 16), q[2], fb, 0u);
+        }
+        HIPCHK(hipMemcpy(c.crec64, r64.data(), nc * sizeof(uint4), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c.crec32, r32.data(), nc * sizeof(uint4), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c.chunks, M.terms.chunks.data(), nch * sizeof(int2), hipMemcpyHostToDevice));
     }
     HIPCHK(hipMalloc((void**)&c.buf.err, sizeof(int)));
     HIPCHK(hipMemset(c.buf.err, 0, sizeof(int)));
     c.P64.lut_a = c.t64.lut_a; c.P64.lut_m = c.t64.lut_m;
     c.P32.lut_a = c.t32.lut_a; c.P32.lut_m = c.t32.lut_m;
-    c.P64.tidx = c.P32.tidx = c.tidx;
-    c.P64.tw = c.P32.tw = c.tw;
+    c.P64.crec = c.crec64;
+    c.P32.crec = c.crec32;
+    c.P64.chunks = c.P32.chunks = c.chunks;
     c.uploaded = true;
     return 0;
 }
@@ -214,13 +238,22 @@ int run_device(Context& c, int natom, const int* off, const double* dlist, const
     if (int rc = upload(c)) return rc;
     if (int rc = ensure_atoms(c, (size_t)natom)) return rc;
     HIPCHK(hipMemsetAsync(c.buf.err, 0, sizeof(int), s));
+    hipEvent_t* ev = nullptr;
+    if (c.profiling) {
+        if (c.events_used == c.events.size()) {
+            std::array<hipEvent_t, 4> q;
+            for (auto& x : q) HIPCHK(hipEventCreate(&x));
+            c.events.push_back(q);
+        }
+        ev = c.events[c.events_used++].data();
+    }
     hipError_t e;
     if (c.precision == MAGACE_FP64)
         e = Launch<double>::run(c.P64, natom, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist, tlist,
-                                tlist0, c.buf, s, c.shape_class);
+                                tlist0, c.buf, s, c.shape_class, ev);
     else
         e = Launch<float>::run(c.P32, natom, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist, tlist,
-                               tlist0, c.buf, s, c.shape_class);
+                               tlist0, c.buf, s, c.shape_class, ev);
     HIPCHK(e);
     return 0;
 }
@@ -434,6 +467,36 @@ void acelib_(int* natom, int* nall, int* nei, int* nlist, int* occ, int* occlist
     if (check_kernel_errors(c, s)) return;
 }
 
+int magace_profile(int enable) {
+    if (!g_ctx) {
+        set_error(MAGACE_ERR_STATE, "library not initialised");
+        return MAGACE_ERR_STATE;
+    }
+    g_ctx->profiling = enable != 0;
+    g_ctx->events_used = 0;
+    return 0;
+}
+
+int magace_profile_read(double* ms, int* count) {
+    if (!g_ctx) {
+        set_error(MAGACE_ERR_STATE, "library not initialised");
+        return MAGACE_ERR_STATE;
+    }
+    Context& c = *g_ctx;
+    ms[0] = ms[1] = ms[2] = 0.0;
+    for (size_t i = 0; i < c.events_used; ++i) {
+        HIPCHK(hipEventSynchronize(c.events[i][3]));
+        for (int k = 0; k < 3; ++k) {
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, c.events[i][k], c.events[i][k + 1]));
+            ms[k] += t;
+        }
+    }
+    *count = (int)c.events_used;
+    c.events_used = 0;
+    return 0;
+}
+
 int magace_basis_counts(int* nvar) {
     if (!g_ctx) {
         set_error(MAGACE_ERR_STATE, "library not initialised");
@@ -493,9 +556,10 @@ int magace_model_info(long long* info, int n) {
         return MAGACE_ERR_STATE;
     }
     const Model& M = g_ctx->model;
-    long long v[8] = {(long long)M.z.nz, (long long)M.terms.w.size(), (long long)M.terms.raw, M.terms.n2,
-                      M.terms.n3, M.terms.n4, g_ctx->shape_class, (long long)M.z.oA2};
-    for (int i = 0; i < n && i < 8; ++i) info[i] = v[i];
+    long long v[10] = {(long long)M.z.nz, (long long)M.terms.w.size(), (long long)M.terms.raw, M.terms.n2,
+                       M.terms.n3, M.terms.n4, g_ctx->shape_class, (long long)M.z.oA2,
+                       (long long)M.terms.cw.size(), (long long)(M.terms.chunks.size() / 2)};
+    for (int i = 0; i < n && i < 10; ++i) info[i] = v[i];
     return 0;
 }
 

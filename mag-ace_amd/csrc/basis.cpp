@@ -312,14 +312,72 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
             folded[key] += kv.second;
         }
     }
+    // device order: slot 0 = the A0 factor (every monomial has exactly one), slots 1..3 = the
+    // other factors ascending, padding last; sorted lexicographically so that a thread walking
+    // a contiguous run sees long runs of equal slot-0 and slot-1 targets
+    std::vector<std::pair<std::array<int, 4>, double>> rows;
+    rows.reserve(folded.size());
     for (const auto& kv : folded) {
         if (kv.second == 0.0) continue;
         const uint64_t key = kv.first;
-        T.idx.push_back((uint16_t)(key >> 48));
-        T.idx.push_back((uint16_t)((key >> 32) & 0xffff));
-        T.idx.push_back((uint16_t)((key >> 16) & 0xffff));
-        T.idx.push_back((uint16_t)(key & 0xffff));
-        T.w.push_back(kv.second);
+        std::array<int, 4> v = {(int)(key >> 48), (int)((key >> 32) & 0xffff), (int)((key >> 16) & 0xffff),
+                                (int)(key & 0xffff)};
+        std::array<int, 4> o;
+        int a0 = -1, k = 1;
+        for (int i = 0; i < 4; ++i) {
+            if (a0 < 0 && v[i] >= z.oA0 && v[i] < z.nz) a0 = v[i];
+            else o[k++] = v[i];
+        }
+        o[0] = a0;
+        std::sort(o.begin() + 1, o.end());
+        rows.emplace_back(o, kv.second);
+    }
+    std::sort(rows.begin(), rows.end());
+    for (const auto& r : rows) {
+        for (int i = 0; i < 4; ++i) T.idx.push_back((uint16_t)r.first[i]);
+        T.w.push_back(r.second);
+    }
+    // transposed gradient: contributions grouped by target
+    std::vector<std::vector<std::pair<std::array<int, 3>, double>>> by(z.nz);
+    for (const auto& r : rows)
+        for (int s = 0; s < 4; ++s) {
+            const int x = r.first[s];
+            if (x == one) continue;
+            std::array<int, 3> o;
+            int k = 0;
+            for (int q = 0; q < 4; ++q)
+                if (q != s) o[k++] = r.first[q];
+            by[x].emplace_back(o, r.second);
+        }
+    // chunks of each row, interleaved across rows (round robin) so that neighbouring lanes
+    // flush to different targets
+    std::vector<std::vector<std::pair<int, int>>> rowchunks(z.nz);  // (start, count)
+    for (int x = 0; x < z.nz; ++x) {
+        auto& v = by[x];
+        std::sort(v.begin(), v.end());
+        for (size_t i = 0; i < v.size(); i += TermList::kChunk) {
+            const int cnt = (int)std::min<size_t>(TermList::kChunk, v.size() - i);
+            const int padded = (cnt + 3) & ~3;  // the kernel consumes 4 records per step
+            rowchunks[x].emplace_back((int)T.cw.size(), padded);
+            for (int k = 0; k < padded; ++k) {
+                const bool real = k < cnt;
+                T.cidx.push_back((uint16_t)(real ? v[i + k].first[0] : one));
+                T.cidx.push_back((uint16_t)(real ? v[i + k].first[1] : one));
+                T.cidx.push_back((uint16_t)(real ? v[i + k].first[2] : one));
+                T.cidx.push_back((uint16_t)one);
+                T.cw.push_back(real ? v[i + k].second : 0.0);
+            }
+        }
+    }
+    for (size_t round = 0;; ++round) {
+        bool any = false;
+        for (int x = 0; x < z.nz; ++x)
+            if (round < rowchunks[x].size()) {
+                any = true;
+                T.chunks.push_back(rowchunks[x][round].first);
+                T.chunks.push_back(x | (rowchunks[x][round].second << 16));
+            }
+        if (!any) break;
     }
     return T;
 }

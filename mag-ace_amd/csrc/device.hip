@@ -419,6 +419,12 @@ size_t density_lds_bytes(const DevParams<double>& Pd, int CH, size_t tsize) {
 }
 
 // ============================================================================ k_adjoint
+template <typename T> __device__ __forceinline__ T rec_weight(uint4 r);
+template <> __device__ __forceinline__ double rec_weight<double>(uint4 r) {
+    return __hiloint2double((int)r.w, (int)r.z);
+}
+template <> __device__ __forceinline__ float rec_weight<float>(uint4 r) { return __uint_as_float(r.z); }
+
 __device__ __forceinline__ void lds_add(double* p, double v) { atomicAdd(p, v); }
 __device__ __forceinline__ void lds_add(float* p, float v) { atomicAdd(p, v); }
 
@@ -478,27 +484,33 @@ __global__ void __launch_bounds__(256) k_adjoint(DevParams<T> P, const cplx<T>* 
         sW[P.nz] = {0, 0};
     }
     __syncthreads();
+    // Gather form of the gradient: a thread sums one chunk of contributions
+    // w * Z[a] Z[b] Z[c] to a single target x in registers (four 16-byte records per step,
+    // loads issued together) and adds it to LDS with one atomic.  rho = Re sum over A0
+    // targets of Z[x] * dRho/dZ[x] (every monomial has exactly one A0 factor), summed in
+    // FP64 once per chunk.
     double rho = 0.0;
-    const uint2* idx = (const uint2*)P.tidx;
-    for (int t = tid; t < P.nterms; t += blockDim.x) {
-        const uint2 pk = idx[t];
-        const int i0 = pk.x & 0xffff, i1 = pk.x >> 16, i2 = pk.y & 0xffff, i3 = pk.y >> 16;
-        const T w = (T)P.tw[t];
-        const cplx<T> z0 = sZ[i0], z1 = sZ[i1], z2 = sZ[i2], z3 = sZ[i3];
-        const cplx<T> p01 = cmul(z0, z1), p23 = cmul(z2, z3);
-        rho += (double)P.tw[t] * (double)re_mul(p01, p23);
-        const cplx<T> wp23 = {w * p23.re, w * p23.im}, wp01 = {w * p01.re, w * p01.im};
-        const cplx<T> d0 = cmul(wp23, z1), d1 = cmul(wp23, z0);
-        lds_add(&sW[i0].re, d0.re); lds_add(&sW[i0].im, d0.im);
-        lds_add(&sW[i1].re, d1.re); lds_add(&sW[i1].im, d1.im);
-        if (i2 != P.nz) {
-            const cplx<T> d2 = cmul(wp01, z3);
-            lds_add(&sW[i2].re, d2.re); lds_add(&sW[i2].im, d2.im);
+    for (int c = tid; c < P.nchunks; c += blockDim.x) {
+        const int2 ch = P.chunks[c];
+        const int x = ch.y & 0xffff, cnt = ch.y >> 16;
+        const uint4* __restrict__ rec = P.crec + ch.x;
+        cplx<T> acc = {0, 0};
+        for (int k = 0; k < cnt; k += 4) {
+            uint4 r[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) r[u] = rec[k + u];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const T w = rec_weight<T>(r[u]);
+                const cplx<T> za = sZ[r[u].x & 0xffff], zb = sZ[r[u].x >> 16], zc = sZ[r[u].y & 0xffff];
+                const cplx<T> pr = cmul(cmul(za, zb), zc);
+                acc.re = fma(w, pr.re, acc.re);
+                acc.im = fma(w, pr.im, acc.im);
+            }
         }
-        if (i3 != P.nz) {
-            const cplx<T> d3 = cmul(wp01, z2);
-            lds_add(&sW[i3].re, d3.re); lds_add(&sW[i3].im, d3.im);
-        }
+        if (x >= P.oA0) rho += (double)re_mul(sZ[x], acc);
+        lds_add(&sW[x].re, acc.re);
+        lds_add(&sW[x].im, acc.im);
     }
     const double tot = block_sum<T>(rho, red);
     if (tid == 0) {
@@ -857,7 +869,7 @@ template <typename T, int L, int CH, int NB, int NM3X, int NB2, int NR2X, int NR
 static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, const double* dlist,
                             const double* elist, const double* dmomi, const double* emomi,
                             const double* dmom, const double* emom, double* eout, double* flist,
-                            double* tlist, double* tlist0, Buffers& B, hipStream_t s) {
+                            double* tlist, double* tlist0, Buffers& B, hipStream_t s, hipEvent_t* ev) {
     DevParams<double> Pd;  // only the dims are used for LDS sizing
     Pd.nr3a = P.nr3a; Pd.l3a = P.l3a; Pd.nr2m = P.nr2m; Pd.l2m = P.l2m; Pd.lha = P.lha;
     Pd.lhm = P.lhm; Pd.nm3 = P.nm3; Pd.nb_a = P.nb_a; Pd.nb_m = P.nb_m;
@@ -865,6 +877,7 @@ static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, co
     auto k1 = k_density<T, L, CH, NB, NM3X, NB2, NR2X>;
     hipError_t e = hipFuncSetAttribute((const void*)k1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1);
     if (e != hipSuccess) return e;
+    if (ev) (void)hipEventRecord(ev[0], s);
     hipLaunchKernelGGL(k1, dim3(natom), dim3(256), lds1, s, P, off, dlist, elist, dmomi, emomi, dmom,
                        emom, (cplx<T>*)B.Z, B.ehc, B.rho1, B.err);
     if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -872,12 +885,15 @@ static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, co
     auto k2 = k_adjoint<T>;
     e = hipFuncSetAttribute((const void*)k2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2);
     if (e != hipSuccess) return e;
+    if (ev) (void)hipEventRecord(ev[1], s);
     hipLaunchKernelGGL(k2, dim3(natom), dim3(256), lds2, s, P, (const cplx<T>*)B.Z, (cplx<T>*)B.W, B.ehc,
                        B.rho1, eout, B.dE, B.edf);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     auto k3 = k_forces<T, L, L, NM3X, 16, NRAX, NR2X>;
+    if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(k3, dim3(natom), dim3(64), 0, s, P, off, dlist, elist, dmomi, emomi, dmom, emom,
                        (const cplx<T>*)B.W, B.dE, B.edf, flist, tlist, tlist0, B.err);
+    if (ev) (void)hipEventRecord(ev[3], s);
     return hipGetLastError();
 }
 
@@ -885,14 +901,15 @@ template <typename T>
 hipError_t Launch<T>::run(const DevParams<T>& P, int natom, const int* off, const double* dlist,
                           const double* elist, const double* dmomi, const double* emomi,
                           const double* dmom, const double* emom, double* eout, double* flist,
-                          double* tlist, double* tlist0, Buffers& B, hipStream_t s, int cls) {
+                          double* tlist, double* tlist0, Buffers& B, hipStream_t s, int cls,
+                          hipEvent_t* ev) {
     if (natom <= 0) return hipSuccess;
     constexpr int CH = sizeof(T) == 8 ? 32 : 64;
     if (cls == 0)
         return run_class<T, 6, CH, 4, 4, 4, 2, 8>(P, natom, off, dlist, elist, dmomi, emomi, dmom, emom, eout,
-                                            flist, tlist, tlist0, B, s);
+                                            flist, tlist, tlist0, B, s, ev);
     return run_class<T, 2, CH, 1, 27, 2, 5, 4>(P, natom, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist,
-                                         tlist, tlist0, B, s);
+                                               tlist, tlist0, B, s, ev);
 }
 
 template struct Launch<double>;

@@ -22,9 +22,11 @@ struct DevParams {
     int nb_a = 0, nb_m = 0, nr2m = 0, l2m = 0, nr3a = 0, nr3m = 0, l3a = 0, l3m = 0;
     int lsa = 0, lsm = 0, lsm3 = 0, nm3 = 0, oA2 = 0, oA0 = 0, nz = 0;
     int lha = 0, lhm = 0;  // (l+1)(l+2)/2 for l3a, l2m
-    const uint16_t* tidx = nullptr;  // 4 per term
-    const double* tw = nullptr;
-    int nterms = 0;
+    // gradient contributions, one 16-byte record each: x = a | b << 16, y = c, (z, w) = weight
+    // (FP64 bits as lo/hi words, or FP32 bits in z)
+    const uint4* crec = nullptr;
+    const int2* chunks = nullptr;    // (start, target | count << 16), count a multiple of 4
+    int nchunks = 0;
     int nfemb = 0;
     double p1a = 1, p1b = 1, ecut = 0, decut = 1;
     double c1[32] = {0};  // coefficients of B1_n = g^m_n(mu_i) - 1
@@ -53,7 +55,8 @@ struct Launch {
     static hipError_t run(const DevParams<T>& P, int natom, const int* offsets, const double* dlist,
                           const double* elist, const double* dmomi, const double* emomi,
                           const double* dmom, const double* emom, double* eout, double* flist,
-                          double* tlist, double* tlist0, Buffers& B, hipStream_t s, int nm3_class);
+                          double* tlist, double* tlist0, Buffers& B, hipStream_t s, int nm3_class,
+                          hipEvent_t* ev);
 };
 
 hipError_t upload_legendre_constants();

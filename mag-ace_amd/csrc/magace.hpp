@@ -59,9 +59,17 @@ struct ZLayout {
 };
 
 // rho = Re sum_t w_t prod_{s<4} Z[i_ts] with Z[nz] == 1 padding lower-degree terms.
+// The gradient is stored transposed ("gather" form): for every target x of Z, the list of
+// contributions w_t * prod_{s' != s} Z[i_ts'] of the monomials containing x, split into
+// chunks of at most kChunk contributions.  rho = Re sum_{x in A0} Z[x] dRho/dZ[x], because
+// every monomial has exactly one A0 factor.
 struct TermList {
-    std::vector<uint16_t> idx;   // 4 per term
+    static constexpr int kChunk = 32;
+    std::vector<uint16_t> idx;   // 4 per term (slot 0 = A0 factor)
     std::vector<double> w;
+    std::vector<uint16_t> cidx;  // 4 per contribution: the three other factors + padding
+    std::vector<double> cw;      // weight per contribution
+    std::vector<int32_t> chunks; // 2 per chunk: start, target | (count << 16)
     int n2 = 0, n3 = 0, n4 = 0;  // raw (before merging) m-combination counts per order
     int raw = 0;
 };
@@ -72,9 +80,9 @@ struct Splines {
     int nfa = 0, nfm = 0;                // functions per atomic / magnetic row
     int atomic_stride = 0, magnetic_stride = 0;  // doubles per row
     double rs_a = 0, rs_m = 0;           // nlut / cutoff
-    // atomic row: [g_1..g_nb_a | R_{n,l} (n major, l minor) | core], each 4 coefficients
+    // atomic row: [g_1..g_nb_a | R_{l,n} (l major, n minor) | core], each 4 coefficients
     std::vector<double> atomic;
-    // magnetic row: [g^m_1..g^m_nb_m | M_{n',l'} (n' major)], each 4 coefficients
+    // magnetic row: [g^m_1..g^m_nb_m | M_{l',n'} (l' major)], each 4 coefficients
     std::vector<double> magnetic;
 };
 

@@ -77,8 +77,10 @@ def test_device_entry_point_matches_acelib(tmp_path):
     capi.compute_device(N, P, off.data_ptr(), *[x.data_ptr() for x in ins], *[x.data_ptr() for x in outs],
                         stream=stream)
     capi.check_errors(stream)
+    # the adjoint accumulates with LDS atomics, so summation order (and the last bits)
+    # may differ between two launches
     for k, o in zip(("eout", "flist", "tlist", "tlist0"), outs):
-        assert np.array_equal(o.cpu().numpy(), host[k]), k
+        assert relerr(o.cpu().numpy(), host[k]) < 1e-12, k
 
 
 def test_pair_compute_matches_python_assembly(tmp_path):
