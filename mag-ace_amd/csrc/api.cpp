@@ -63,6 +63,7 @@ struct Context {
     uint4* crec64 = nullptr;
     uint4* crec32 = nullptr;
     int2* chunks = nullptr;
+    uint32_t* fold = nullptr;
     Buffers buf;
     // staging for acelib_ host arrays
     size_t cap_pairs = 0, cap_atoms = 0;
@@ -100,7 +101,7 @@ void dfree(T*& p) {
 
 Context::~Context() {
     dfree(t64.lut_a); dfree(t64.lut_m); dfree(t32.lut_a); dfree(t32.lut_m);
-    dfree(crec64); dfree(crec32); dfree(chunks);
+    dfree(crec64); dfree(crec32); dfree(chunks); dfree(fold);
     dfree(buf.Z); dfree(buf.W); dfree(buf.ehc); dfree(buf.rho1); dfree(buf.dE); dfree(buf.edf); dfree(buf.err);
     dfree(d_off); dfree(d_dl); dfree(d_el); dfree(d_dm); dfree(d_em); dfree(d_dmi); dfree(d_emi);
     dfree(d_eout); dfree(d_fl); dfree(d_tl); dfree(d_t0);
@@ -188,6 +189,8 @@ int upload(Context& c) {
     HIPCHK(hipMalloc((void**)&c.crec64, nc * sizeof(uint4) + 64));
     HIPCHK(hipMalloc((void**)&c.crec32, nc * sizeof(uint4) + 64));
     HIPCHK(hipMalloc((void**)&c.chunks, nch * sizeof(int2) + 64));
+    HIPCHK(hipMalloc((void**)&c.fold, M.terms.fold.size() * sizeof(uint32_t) + 64));
+    HIPCHK(hipMemcpy(c.fold, M.terms.fold.data(), M.terms.fold.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     if (nc) {
         std::vector<uint4> r64(nc), r32(nc);
         for (size_t i = 0; i < nc; ++i) {
@@ -213,6 +216,7 @@ int upload(Context& c) {
     c.P64.crec = c.crec64;
     c.P32.crec = c.crec32;
     c.P64.chunks = c.P32.chunks = c.chunks;
+    c.P64.fold = c.P32.fold = c.fold;
     c.uploaded = true;
     return 0;
 }

@@ -379,6 +379,28 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
             }
         if (!any) break;
     }
+    // representatives: A entries with m' > 0, or m' = 0 and m >= 0; A2/A0 entries with m' >= 0
+    T.fold.assign(z.nz, 0u);
+    for (int n = 0; n < h.nr3_a; ++n)
+        for (int l = 0; l <= h.l3a; ++l)
+            for (int m = -l; m <= l; ++m)
+                for (int np = 0; np < h.nr3_m; ++np)
+                    for (int lp = 0; lp <= h.l3m; ++lp)
+                        for (int mp = -lp; mp <= lp; ++mp) {
+                            const int x = z.a(n, l, m, np, lp, mp);
+                            const bool rep = mp > 0 || (mp == 0 && m >= 0);
+                            const bool self = m == 0 && mp == 0;
+                            T.fold[x] = (uint32_t)cj.partner[x] | ((cj.sign[x] < 0 ? 1u : 0u) << 16) |
+                                        ((rep ? 1u : 0u) << 17) | ((self ? 1u : 0u) << 18);
+                        }
+    for (int x = z.oA2; x < z.nz; ++x) {
+        const int lm = (x < z.oA0 ? (x - z.oA2) : (x - z.oA0)) % z.lsm;
+        int lp = 0;
+        while ((lp + 1) * (lp + 1) <= lm) ++lp;
+        const int mp = lm - lp * lp - lp;
+        T.fold[x] = (uint32_t)cj.partner[x] | ((cj.sign[x] < 0 ? 1u : 0u) << 16) |
+                    ((mp >= 0 ? 1u : 0u) << 17) | ((mp == 0 ? 1u : 0u) << 18);
+    }
     return T;
 }
 

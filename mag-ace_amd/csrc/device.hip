@@ -532,9 +532,25 @@ __global__ void __launch_bounds__(256) k_adjoint(DevParams<T> P, const cplx<T>* 
         s_scale = fc * dF;
     }
     __syncthreads();
+    // W_eff on representatives of conjugate pairs: since Z_xbar = s conj(Z_x),
+    // Re(W_x dZ_x + W_xbar dZ_xbar) = Re((W_x + s conj(W_xbar)) dZ_x); zero elsewhere
+    // (the reference's half-range loops with factor 2, src/forces.F90:460-614)
     const T sc = (T)s_scale;
     Wout += (size_t)atom * P.nz;
-    for (int x = tid; x < P.nz; x += blockDim.x) Wout[x] = {sc * sW[x].re, sc * sW[x].im};
+    for (int x = tid; x < P.nz; x += blockDim.x) {
+        const uint32_t f = P.fold[x];
+        cplx<T> w = {0, 0};
+        if (f & (1u << 17)) {
+            w = sW[x];
+            if (!(f & (1u << 18))) {
+                const cplx<T> b = sW[f & 0xffff];
+                const T s = (f & (1u << 16)) ? (T)-1 : (T)1;
+                w.re += s * b.re;
+                w.im -= s * b.im;
+            }
+        }
+        Wout[x] = {sc * w.re, sc * w.im};
+    }
 }
 
 // ============================================================================ k_forces
@@ -592,6 +608,8 @@ __device__ __forceinline__ void conj_all(cplx<T>& Y, cplx<T> (&g)[3], int am) {
 
 // lane = pair.  LA/LM: compile-time bounds of l3a/l2m; NM3X: magnetic columns of A;
 // NBAX: atomic radial base bound; NRAX: nr3a bound; NR2X: nr2m bound.
+// W is folded onto representatives of conjugate pairs (A: m' > 0, or m' = 0 and m >= 0;
+// A2, A0: m' >= 0), so only those index pairs are visited.
 template <typename T, int LA, int LM, int NM3X, int NBAX, int NRAX, int NR2X>
 __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __restrict__ off,
                                                const double* __restrict__ dlist,
@@ -610,6 +628,15 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
     const cplx<T>* __restrict__ W = Win + (size_t)atom * P.nz;
     const int nfa = P.nr3a * (P.l3a + 1);
     const double edf = edfin[atom];
+    // magnetic column q of A -> m' (for the representative test)
+    int qmp[NM3X];
+#pragma unroll
+    for (int q = 0; q < NM3X; ++q) {
+        const int lq = q % P.lsm3;
+        int lp = 0;
+        while ((lp + 1) * (lp + 1) <= lq) ++lp;
+        qmp[q] = lq - lp * lp - lp;
+    }
 
     for (int c0 = 0; c0 < nn; c0 += 64) {
         const int j = c0 + lane;
@@ -619,20 +646,15 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
         T wa = 0;
         const T* rowa = active ? atomic_row(P, dlist[p], wa, err) : nullptr;
         if (rowa) {
-            const T r = (T)dlist[p], invr = (T)1 / r;
+            const T invr = (T)(1.0 / dlist[p]);
             const T ex = (T)elist[3 * p], ey = (T)elist[3 * p + 1], ez = (T)elist[3 * p + 2];
             const double mu_d = dmom[p];
-            const T invmu = (T)(1.0 / mu_d);
             const T sx = (T)emom[3 * p], sy = (T)emom[3 * p + 1], sz = (T)emom[3 * p + 2];
             T wm = 0;
             bool unit = false;
             const T* rowm = magnetic_row(P, mu_d, wm, unit, err);
             const T rsa = (T)P.rs_a, rsm = (T)P.rs_m;
-            cplx<T> phm[LM + 1];
-            phm[0] = {1, 0};
-#pragma unroll
-            for (int m = 1; m <= LM; ++m) phm[m] = cmul(phm[m - 1], cplx<T>{sx, sy});
-            // H_q = M_{l',n'}(mu_j) Y_{l'm'}(s_j) for the magnetic columns of A
+            // H_q = M_{l',n'}(mu_j) Y_{l'm'}(s_j), m' >= 0, for the magnetic columns of A
             cplx<T> H[NM3X], U[NM3X];
 #pragma unroll
             for (int q = 0; q < NM3X; ++q) {
@@ -640,27 +662,24 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                 U[q] = {0, 0};
             }
             {
+                cplx<T> ph = {1, 0};
                 LegRows<T, LM> Lr;
                 for (int lp = 0; lp <= P.l3m; ++lp) {
                     Lr.step(lp, sz);
+                    ph = {1, 0};
 #pragma unroll
                     for (int am = 0; am <= LM; ++am) {
                         if (am > lp) break;
-                        const cplx<T> y0 = {Lr.P[am] * phm[am].re, Lr.P[am] * phm[am].im};
+                        const cplx<T> y = {Lr.P[am] * ph.re, Lr.P[am] * ph.im};
+                        for (int np = 0; np < P.nr3m; ++np) {
+                            T v = unit ? (T)1 : (T)0, dv = 0;
+                            if (rowm) horner(rowm + 4 * (P.nb_m + lp * P.nr2m + np), wm, rsm, v, dv);
+                            const int q = np * P.lsm3 + lp * lp + lp + am;
 #pragma unroll
-                        for (int sg = 0; sg < 2; ++sg) {
-                            if (sg == 1 && am == 0) break;
-                            const int mp = sg ? -am : am;
-                            const cplx<T> y = sg ? cconj_sign(y0, am) : y0;
-                            for (int np = 0; np < P.nr3m; ++np) {
-                                T v = unit ? (T)1 : (T)0, dv = 0;
-                                if (rowm) horner(rowm + 4 * (P.nb_m + lp * P.nr2m + np), wm, rsm, v, dv);
-                                const int q = np * P.lsm3 + lp * lp + lp + mp;
-#pragma unroll
-                                for (int qq = 0; qq < NM3X; ++qq)
-                                    if (qq == q) H[qq] = {v * y.re, v * y.im};
-                            }
+                            for (int qq = 0; qq < NM3X; ++qq)
+                                if (qq == q) H[qq] = {v * y.re, v * y.im};
                         }
+                        ph = cmul(ph, cplx<T>{sx, sy});
                     }
                 }
             }
@@ -700,9 +719,11 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
 #pragma unroll
                                 for (int q = 0; q < NM3X; ++q) {
                                     if (q >= P.nm3) break;
-                                    const cplx<T> wq = Wn[q];
-                                    cfma(V, wq, H[q]);
-                                    cfma(U[q], wq, ry);
+                                    if (qmp[q] > 0 || (qmp[q] == 0 && sg == 0)) {
+                                        const cplx<T> wq = Wn[q];
+                                        cfma(V, wq, H[q]);
+                                        cfma(U[q], wq, ry);
+                                    }
                                 }
                                 // Re[V (R' Y e + R/r grad Y)]
                                 const T a = dR[n] * re_mul(V, Y), b = R[n] * invr;
@@ -714,8 +735,10 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                     }
                 }
             }
-            // ---- magnetic loop: A2 part of F and T, A part of T (src/forces.F90:460-491, 793-833)
+            // ---- magnetic loop over m' >= 0: A2 part of F and T, A part of T
+            // (src/forces.F90:460-491, 793-957)
             {
+                const T invmu = (T)(1.0 / mu_d);
                 T ga[NBAX], dga[NBAX];
 #pragma unroll
                 for (int n = 0; n < NBAX; ++n) {
@@ -723,9 +746,11 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                     dga[n] = 0;
                     if (n < P.nb_a) horner(rowa + 4 * n, wa, rsa, ga[n], dga[n]);
                 }
-                cplx<T> V2[NBAX];
+                T sf = 0;  // sum_n g'_n Re(W2 H2)
+                cplx<T> phm[LM + 1];
+                phm[0] = {1, 0};
 #pragma unroll
-                for (int n = 0; n < NBAX; ++n) V2[n] = {0, 0};
+                for (int m = 1; m <= LM; ++m) phm[m] = cmul(phm[m - 1], cplx<T>{sx, sy});
                 LegRows<T, LM> Lr;
                 for (int lp = 0; lp <= P.l2m; ++lp) {
                     Lr.step(lp, sz);
@@ -739,54 +764,44 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
 #pragma unroll
                     for (int am = 0; am <= LM; ++am) {
                         if (am > lp) break;
-                        cplx<T> Y0, g0[3];
-                        ylm_grad<T>(am, Lr.P[am], Lr.dP[am], phm[am], phm[am > 0 ? am - 1 : 0], sx, sy, sz, Y0, g0);
+                        cplx<T> Y, g[3];
+                        ylm_grad<T>(am, Lr.P[am], Lr.dP[am], phm[am], phm[am > 0 ? am - 1 : 0], sx, sy, sz, Y, g);
+                        const int lmp = lp * lp + lp + am;
 #pragma unroll
-                        for (int sg = 0; sg < 2; ++sg) {
-                            if (sg == 1 && am == 0) break;
-                            cplx<T> Y = Y0, g[3] = {g0[0], g0[1], g0[2]};
-                            if (sg) conj_all(Y, g, am);
-                            const int lmp = lp * lp + lp + (sg ? -am : am);
+                        for (int np = 0; np < NR2X; ++np) {
+                            if (np >= P.nr2m) break;
+                            const cplx<T> H2 = {Mv[np] * Y.re, Mv[np] * Y.im};
+                            cplx<T> U2 = {0, 0};
+                            const cplx<T>* W2 = W + P.oA2 + np * P.lsm + lmp;
 #pragma unroll
-                            for (int np = 0; np < NR2X; ++np) {
-                                if (np >= P.nr2m) break;
-                                const cplx<T> H2 = {Mv[np] * Y.re, Mv[np] * Y.im};
-                                cplx<T> U2 = {0, 0};
-                                const cplx<T>* W2 = W + P.oA2 + np * P.lsm + lmp;
-#pragma unroll
-                                for (int n = 0; n < NBAX; ++n) {
-                                    if (n >= P.nb_a) break;
-                                    const cplx<T> wv = W2[(size_t)n * P.nr2m * P.lsm];
-                                    cfma(V2[n], wv, H2);
-                                    U2.re = fma(wv.re, ga[n], U2.re);
-                                    U2.im = fma(wv.im, ga[n], U2.im);
-                                }
-                                if (lp <= P.l3m && np < P.nr3m) {
-                                    const int q = np * P.lsm3 + lmp;
-#pragma unroll
-                                    for (int qq = 0; qq < NM3X; ++qq)
-                                        if (qq == q) {
-                                            U2.re += U[qq].re;
-                                            U2.im += U[qq].im;
-                                        }
-                                }
-                                // d(M Y)/dm = M' s Y + M/mu grad Y
-                                const T a = dMv[np] * re_mul(U2, Y), b = Mv[np] * invmu;
-                                Tx += fma(a, sx, b * re_mul(U2, g[0]));
-                                Ty += fma(a, sy, b * re_mul(U2, g[1]));
-                                Tz += fma(a, sz, b * re_mul(U2, g[2]));
+                            for (int n = 0; n < NBAX; ++n) {
+                                if (n >= P.nb_a) break;
+                                const cplx<T> wv = W2[(size_t)n * P.nr2m * P.lsm];
+                                sf = fma(dga[n], re_mul(wv, H2), sf);
+                                U2.re = fma(wv.re, ga[n], U2.re);
+                                U2.im = fma(wv.im, ga[n], U2.im);
                             }
+                            if (lp <= P.l3m && np < P.nr3m) {
+                                const int q = np * P.lsm3 + lmp;
+#pragma unroll
+                                for (int qq = 0; qq < NM3X; ++qq)
+                                    if (qq == q) {
+                                        U2.re += U[qq].re;
+                                        U2.im += U[qq].im;
+                                    }
+                            }
+                            // d(M Y)/dm = M' s Y + M/mu grad Y
+                            const T a = dMv[np] * re_mul(U2, Y), b = Mv[np] * invmu;
+                            Tx += fma(a, sx, b * re_mul(U2, g[0]));
+                            Ty += fma(a, sy, b * re_mul(U2, g[1]));
+                            Tz += fma(a, sz, b * re_mul(U2, g[2]));
                         }
                     }
                 }
                 // F from A2 (g'_n e) and the core term (src/forces.F90:464-494)
-                T s = 0;
-#pragma unroll
-                for (int n = 0; n < NBAX; ++n)
-                    if (n < P.nb_a) s = fma(dga[n], V2[n].re, s);
                 T cr, dcr;
                 horner(rowa + 4 * (P.nb_a + nfa), wa, rsa, cr, dcr);
-                s += (T)edf * dcr;
+                const T s = sf + (T)edf * dcr;
                 Fx = fma(s, ex, Fx);
                 Fy = fma(s, ey, Fy);
                 Fz = fma(s, ez, Fz);
@@ -801,7 +816,7 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
             tlist[3 * (size_t)p + 2] = (double)Tz;
         }
     }
-    // ---- self term T0 = dE_i/dm_i (src/forces.F90:712-749)
+    // ---- self term T0 = dE_i/dm_i over m' >= 0 (src/forces.F90:712-749)
     if (lane == 0) {
         const double mu_d = dmomi[atom];
         const T invmu = (T)(1.0 / mu_d);
@@ -829,23 +844,16 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
 #pragma unroll
             for (int am = 0; am <= LM; ++am) {
                 if (am > lp) break;
-                cplx<T> Y0, g0[3];
-                ylm_grad<T>(am, Lr.P[am], Lr.dP[am], phm[am], phm[am > 0 ? am - 1 : 0], sx, sy, sz, Y0, g0);
-#pragma unroll
-                for (int sg = 0; sg < 2; ++sg) {
-                    if (sg == 1 && am == 0) break;
-                    cplx<T> Y = Y0, g[3] = {g0[0], g0[1], g0[2]};
-                    if (sg) conj_all(Y, g, am);
-                    const int mp = sg ? -am : am;
-                    for (int np = 0; np < P.nr2m; ++np) {
-                        T v = unit ? (T)1 : (T)0, dv = 0;
-                        if (rowm) horner(rowm + 4 * (P.nb_m + lp * P.nr2m + np), wm, rsm, v, dv);
-                        const cplx<T> w0 = W[P.oA0 + np * P.lsm + lp * lp + lp + mp];
-                        const T a = dv * re_mul(w0, Y), b = v * invmu;
-                        Tx += fma(a, sx, b * re_mul(w0, g[0]));
-                        Ty += fma(a, sy, b * re_mul(w0, g[1]));
-                        Tz += fma(a, sz, b * re_mul(w0, g[2]));
-                    }
+                cplx<T> Y, g[3];
+                ylm_grad<T>(am, Lr.P[am], Lr.dP[am], phm[am], phm[am > 0 ? am - 1 : 0], sx, sy, sz, Y, g);
+                for (int np = 0; np < P.nr2m; ++np) {
+                    T v = unit ? (T)1 : (T)0, dv = 0;
+                    if (rowm) horner(rowm + 4 * (P.nb_m + lp * P.nr2m + np), wm, rsm, v, dv);
+                    const cplx<T> w0 = W[P.oA0 + np * P.lsm + lp * lp + lp + am];
+                    const T a = dv * re_mul(w0, Y), b = v * invmu;
+                    Tx += fma(a, sx, b * re_mul(w0, g[0]));
+                    Ty += fma(a, sy, b * re_mul(w0, g[1]));
+                    Tz += fma(a, sz, b * re_mul(w0, g[2]));
                 }
             }
         }
@@ -856,16 +864,16 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
 }
 
 // ============================================================================ launch
-// shape classes: 0 = small magnetic A block (nm3 <= 4, nr2m <= 2, nb_a <= 16, nr3a <= 8)
-//                1 = wide magnetic A block  (nm3 <= 27, nr2m <= 5, nb_a <= 16, nr3a <= 8)
+// shape classes: 0 = small magnetic A block (nm3 <= 4, nr2m <= 2, nb_a <= 10, nr3a <= 8, l <= 6)
+//                1 = wide magnetic A block  (nm3 <= 27, nr2m <= 5, nb_a <= 16, nr3a <= 4, l <= 2)
 int supported_nm3_class(int nm3, int nr2m, int nb_a, int l3a, int l2m, int nr3a) {
     if (nb_a > 16) return -1;
-    if (nm3 <= 4 && nr2m <= 2 && l3a <= 6 && l2m <= 6 && nr3a <= 8) return 0;
+    if (nm3 <= 4 && nr2m <= 2 && l3a <= 6 && l2m <= 6 && nr3a <= 8 && nb_a <= 10) return 0;
     if (nm3 <= 27 && nr2m <= 5 && l3a <= 2 && l2m <= 2 && nr3a <= 4) return 1;
     return -1;
 }
 
-template <typename T, int L, int CH, int NB, int NM3X, int NB2, int NR2X, int NRAX>
+template <typename T, int L, int CH, int NB, int NM3X, int NB2, int NR2X, int NRAX, int NBAX>
 static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, const double* dlist,
                             const double* elist, const double* dmomi, const double* emomi,
                             const double* dmom, const double* emom, double* eout, double* flist,
@@ -889,7 +897,7 @@ static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, co
     hipLaunchKernelGGL(k2, dim3(natom), dim3(256), lds2, s, P, (const cplx<T>*)B.Z, (cplx<T>*)B.W, B.ehc,
                        B.rho1, eout, B.dE, B.edf);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    auto k3 = k_forces<T, L, L, NM3X, 16, NRAX, NR2X>;
+    auto k3 = k_forces<T, L, L, NM3X, NBAX, NRAX, NR2X>;
     if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(k3, dim3(natom), dim3(64), 0, s, P, off, dlist, elist, dmomi, emomi, dmom, emom,
                        (const cplx<T>*)B.W, B.dE, B.edf, flist, tlist, tlist0, B.err);
@@ -906,9 +914,9 @@ hipError_t Launch<T>::run(const DevParams<T>& P, int natom, const int* off, cons
     if (natom <= 0) return hipSuccess;
     constexpr int CH = sizeof(T) == 8 ? 32 : 64;
     if (cls == 0)
-        return run_class<T, 6, CH, 4, 4, 4, 2, 8>(P, natom, off, dlist, elist, dmomi, emomi, dmom, emom, eout,
+        return run_class<T, 6, CH, 4, 4, 4, 2, 8, 10>(P, natom, off, dlist, elist, dmomi, emomi, dmom, emom, eout,
                                             flist, tlist, tlist0, B, s, ev);
-    return run_class<T, 2, CH, 1, 27, 2, 5, 4>(P, natom, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist,
+    return run_class<T, 2, CH, 1, 27, 2, 5, 4, 16>(P, natom, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist,
                                                tlist, tlist0, B, s, ev);
 }
 

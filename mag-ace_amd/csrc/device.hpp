@@ -26,6 +26,7 @@ struct DevParams {
     // (FP64 bits as lo/hi words, or FP32 bits in z)
     const uint4* crec = nullptr;
     const int2* chunks = nullptr;    // (start, target | count << 16), count a multiple of 4
+    const uint32_t* fold = nullptr;  // conjugate-pair folding of W (see TermList::fold)
     int nchunks = 0;
     int nfemb = 0;
     double p1a = 1, p1b = 1, ecut = 0, decut = 1;

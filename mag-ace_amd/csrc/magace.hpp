@@ -70,6 +70,9 @@ struct TermList {
     std::vector<uint16_t> cidx;  // 4 per contribution: the three other factors + padding
     std::vector<double> cw;      // weight per contribution
     std::vector<int32_t> chunks; // 2 per chunk: start, target | (count << 16)
+    // folding of the gradient onto representatives of conjugate pairs (x, xbar):
+    // partner | sign << 16 (1 = odd m+m') | representative << 17 | self-conjugate << 18
+    std::vector<uint32_t> fold;
     int n2 = 0, n3 = 0, n4 = 0;  // raw (before merging) m-combination counts per order
     int raw = 0;
 };
