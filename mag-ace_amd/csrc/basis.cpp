@@ -349,35 +349,56 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
                 if (q != s) o[k++] = r.first[q];
             by[x].emplace_back(o, r.second);
         }
-    // chunks of each row, interleaved across rows (round robin) so that neighbouring lanes
-    // flush to different targets
-    std::vector<std::vector<std::pair<int, int>>> rowchunks(z.nz);  // (start, count)
+    // Chunks of <= kChunk contributions per row, ordered round robin across rows so that
+    // neighbouring lanes flush to different targets.  A wave processes 64 consecutive
+    // chunks at once; their records are stored interleaved (record k of lane L of group G
+    // at G*64*len + k*64 + L), so each step of the 64 lanes reads 64 consecutive 16-byte
+    // records.  Chunks are padded (factor index nz, weight 0) to a multiple of 4 and to the
+    // longest chunk of their group.
+    struct Ch { int x, off, cnt; };
+    std::vector<std::vector<Ch>> rowchunks(z.nz);
     for (int x = 0; x < z.nz; ++x) {
         auto& v = by[x];
         std::sort(v.begin(), v.end());
-        for (size_t i = 0; i < v.size(); i += TermList::kChunk) {
-            const int cnt = (int)std::min<size_t>(TermList::kChunk, v.size() - i);
-            const int padded = (cnt + 3) & ~3;  // the kernel consumes 4 records per step
-            rowchunks[x].emplace_back((int)T.cw.size(), padded);
-            for (int k = 0; k < padded; ++k) {
-                const bool real = k < cnt;
-                T.cidx.push_back((uint16_t)(real ? v[i + k].first[0] : one));
-                T.cidx.push_back((uint16_t)(real ? v[i + k].first[1] : one));
-                T.cidx.push_back((uint16_t)(real ? v[i + k].first[2] : one));
-                T.cidx.push_back((uint16_t)one);
-                T.cw.push_back(real ? v[i + k].second : 0.0);
-            }
-        }
+        for (size_t i = 0; i < v.size(); i += TermList::kChunk)
+            rowchunks[x].push_back({x, (int)i, (int)std::min<size_t>(TermList::kChunk, v.size() - i)});
     }
+    // full chunks round robin, then each row's partial tail chunk by decreasing length, so
+    // that the chunks of a wave have (nearly) equal lengths and distinct targets
+    std::vector<Ch> order, tails;
     for (size_t round = 0;; ++round) {
         bool any = false;
         for (int x = 0; x < z.nz; ++x)
             if (round < rowchunks[x].size()) {
                 any = true;
-                T.chunks.push_back(rowchunks[x][round].first);
-                T.chunks.push_back(x | (rowchunks[x][round].second << 16));
+                const Ch& ch = rowchunks[x][round];
+                (ch.cnt == TermList::kChunk ? order : tails).push_back(ch);
             }
         if (!any) break;
+    }
+    std::stable_sort(tails.begin(), tails.end(), [](const Ch& a, const Ch& b) { return a.cnt > b.cnt; });
+    order.insert(order.end(), tails.begin(), tails.end());
+    for (size_t g0 = 0; g0 < order.size(); g0 += 64) {
+        const size_t g1 = std::min(order.size(), g0 + 64);
+        int len = 0;
+        for (size_t c = g0; c < g1; ++c) len = std::max(len, (order[c].cnt + 3) & ~3);
+        const size_t base = T.cw.size();
+        T.cidx.resize((base + (size_t)len * 64) * 4, (uint16_t)one);
+        T.cw.resize(base + (size_t)len * 64, 0.0);
+        for (size_t c = g0; c < g1; ++c) {
+            const Ch& ch = order[c];
+            const int lane = (int)(c - g0);
+            for (int k = 0; k < ch.cnt; ++k) {
+                const auto& e = by[ch.x][ch.off + k];
+                const size_t r = base + (size_t)k * 64 + lane;
+                T.cidx[4 * r] = (uint16_t)e.first[0];
+                T.cidx[4 * r + 1] = (uint16_t)e.first[1];
+                T.cidx[4 * r + 2] = (uint16_t)e.first[2];
+                T.cw[r] = e.second;
+            }
+            T.chunks.push_back((int32_t)(base + lane));
+            T.chunks.push_back(ch.x | (((ch.cnt + 3) & ~3) << 16));
+        }
     }
     // representatives: A entries with m' > 0, or m' = 0 and m >= 0; A2/A0 entries with m' >= 0
     T.fold.assign(z.nz, 0u);

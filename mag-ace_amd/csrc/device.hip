@@ -486,19 +486,21 @@ __global__ void __launch_bounds__(256) k_adjoint(DevParams<T> P, const cplx<T>* 
     __syncthreads();
     // Gather form of the gradient: a thread sums one chunk of contributions
     // w * Z[a] Z[b] Z[c] to a single target x in registers (four 16-byte records per step,
-    // loads issued together) and adds it to LDS with one atomic.  rho = Re sum over A0
+    // loads issued together; the 64 lanes of a wave read consecutive records) and adds it
+    // to LDS with one atomic.  The chunk -> thread mapping (chunk c on thread c mod 256)
+    // must match the record layout built by build_terms.  rho = Re sum over A0
     // targets of Z[x] * dRho/dZ[x] (every monomial has exactly one A0 factor), summed in
     // FP64 once per chunk.
     double rho = 0.0;
     for (int c = tid; c < P.nchunks; c += blockDim.x) {
         const int2 ch = P.chunks[c];
         const int x = ch.y & 0xffff, cnt = ch.y >> 16;
-        const uint4* __restrict__ rec = P.crec + ch.x;
+        const uint4* __restrict__ rec = P.crec + ch.x;  // records of this lane, stride 64
         cplx<T> acc = {0, 0};
         for (int k = 0; k < cnt; k += 4) {
             uint4 r[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) r[u] = rec[k + u];
+            for (int u = 0; u < 4; ++u) r[u] = rec[(size_t)(k + u) * 64];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const T w = rec_weight<T>(r[u]);

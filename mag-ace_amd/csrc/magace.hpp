@@ -69,7 +69,8 @@ struct TermList {
     std::vector<double> w;
     std::vector<uint16_t> cidx;  // 4 per contribution: the three other factors + padding
     std::vector<double> cw;      // weight per contribution
-    std::vector<int32_t> chunks; // 2 per chunk: start, target | (count << 16)
+    std::vector<int32_t> chunks; // 2 per chunk: first record, target | (count << 16); records of
+                                 // a chunk are 64 apart (wave-interleaved layout)
     // folding of the gradient onto representatives of conjugate pairs (x, xbar):
     // partner | sign << 16 (1 = odd m+m') | representative << 17 | self-conjugate << 18
     std::vector<uint32_t> fold;
