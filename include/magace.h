@@ -19,6 +19,7 @@ extern "C" {
 #define MAGACE_ERR_INPUT 1  /* bad file, header, sizes, geometry (e.g. very small distance) */
 #define MAGACE_ERR_STATE 2  /* called before initialisation */
 #define MAGACE_ERR_DEVICE 3 /* HIP runtime error */
+#define MAGACE_ERR_CAPACITY 4 /* output capacity too small; the needed size is returned */
 
 /* Precision modes */
 #define MAGACE_FP64 0  /* FP64 everywhere; agrees with libace to ~1e-14 relative */
@@ -87,6 +88,40 @@ int magace_pair_compute(int nlocal, int nghost, const double *x, const double *s
  * milliseconds per kernel in ms[3] and the number of evaluations in *count. */
 int magace_profile(int enable);
 int magace_profile_read(double *ms, int *count);
+
+/* Neighbour lists on the device.  Replaces the reference neighbour list
+ * (neilist/neilist.F90:148-796: periodic images, grid boxes of edge rcut, 27-box stencil,
+ * fastdist) run per structure and concatenated by packstructures (src/neiutil.F90:5-264),
+ * with the same pair set, pair order and FP64 values.
+ * Host inputs: nstruc structures of natoms[s] atoms, periodic[s] (0 = open cluster),
+ * cells[9*s + 3*k + i] = component i of lattice vector k, rcut (1e-6 <= rcut <= 100).
+ * Device inputs: pos[3*ntot], mag[3*ntot] (magnetic moment vectors, muB), structures one
+ * after another.  Device outputs in the acelib_ conventions, ready for
+ * magace_compute_device: offsets[ntot+1] (0-based global CSR), nlist[p] = global 0-based
+ * atom index of the neighbour (the periodic image's origin), dlist[p] = r,
+ * elist[3p] = (x_i - x_j)/r (the negated reference eist), dlist_mom/elist_mom of the
+ * neighbour, dlist_momi/elist_momi of the central atom.  Host outputs: *npairs, natombox[s]
+ * (atoms of the image cluster; may be NULL).  nlist == NULL counts only (offsets filled).
+ * Synchronises `stream` once to learn the pair count; returns MAGACE_ERR_CAPACITY (with
+ * *npairs set, offsets filled) when capacity < *npairs.  Errors as the reference's stops:
+ * rcut out of range, extreme cell shape, identical positions. */
+int magace_neighbors_device(int nstruc, const int *natoms, const int *periodic, const double *cells,
+                            double rcut, const double *pos, const double *mag, long long capacity,
+                            int *offsets, int *nlist, double *dlist, double *elist, double *dlist_mom,
+                            double *elist_mom, double *dlist_momi, double *elist_momi, int *npairs,
+                            int *natombox, void *stream);
+/* packstructures (src/neiutil.F90:5-264) on host arrays, computed on the device, returned in
+ * the reference's conventions: strucaddress[2*s] = 1-based first atom, [2*s+1] = 1-based
+ * first pair; n2startpack/n2stoppack[ntot] 1-based ranges local to the structure;
+ * n2listpack[p] = origin(nlist) (1-based, local); d2listpack, e2listpack (x_j - x_i)/d,
+ * d2listpack_mom, e2listpack_mom, d2listpack_momi, e2listpack_momi; natombox[s] (may be
+ * NULL); *npairs.  MAGACE_ERR_CAPACITY when capacity < *npairs. */
+int magace_packstructures(int nstruc, const int *natoms, const int *periodic, const double *cells,
+                          double rcut, const double *atompos, const double *magmom, long long capacity,
+                          int *strucaddress, int *n2startpack, int *n2stoppack, int *n2listpack,
+                          double *d2listpack, double *e2listpack, double *d2listpack_mom,
+                          double *e2listpack_mom, double *d2listpack_momi, double *e2listpack_momi,
+                          int *natombox, int *npairs);
 
 /* Basis introspection: counts of B functions of orders 1..4 of the loaded potential
  * (the numbers src/util.F90:1831,1929,2235,2596 print). */

@@ -7,8 +7,9 @@ The package directory is ``mag-ace_amd/`` (not an importable identifier); it is 
 * ``capi``          -- ctypes binding of that C-ABI
 * ``systems``       -- synthetic bcc-Fe workloads and the pair_spin_ace array packing
 * ``potgen``        -- synthetic potential files in the reference text format
-* ``pair``          -- Python mirror of the LAMMPS PairSpinACE caller
+* ``neighbors``     -- GPU neighbour lists with the reference neilist's pair sets, single
+                       structures or batches packed as packstructures packs them
 """
-from . import capi, potgen, systems  # noqa: F401
+from . import capi, neighbors, potgen, systems  # noqa: F401
 
-__all__ = ["capi", "potgen", "systems"]
+__all__ = ["capi", "neighbors", "potgen", "systems"]

@@ -19,7 +19,9 @@ LIB = os.path.join(HERE, "libmagace.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MAGACE_ARCH", "gfx950")
 
-SOURCES = ["device.hip", "api.cpp", "basis.cpp", "potential.cpp", "splines.cpp", "pair_host.cpp"]
+SOURCES = ["device.hip", "neighbors.hip", "api.cpp", "basis.cpp", "potential.cpp", "splines.cpp", "pair_host.cpp"]
+# neighbour lists reproduce the reference's FP64 values bit for bit: no contraction
+EXTRA = {"neighbors.hip": ["-ffp-contract=off"]}
 # packed FP32 instructions are disabled on purpose (the compiler would otherwise form
 # v_pk_* from ordinary float code)
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Xclang", "-target-feature",
@@ -29,7 +31,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Xclang", "-t
 
 def _compile(src: str) -> str:
     obj = os.path.join(OUT, src + ".o")
-    cmd = [HIPCC] + COMMON + ["-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC] + COMMON + EXTRA.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), p.stdout, p.stderr))

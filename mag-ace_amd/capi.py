@@ -24,7 +24,10 @@ _DP = C.POINTER(C.c_double)
 EXPORTS = ("acelibini_", "acelib_", "magace_last_error", "magace_error_string", "magace_init_file",
            "magace_finalize", "magace_set_precision", "magace_get_precision", "magace_compute_device",
            "magace_check_errors", "magace_basis_counts", "magace_enumerate", "magace_model_info",
-           "magace_pair_compute", "magace_model_terms", "magace_profile", "magace_profile_read")
+           "magace_pair_compute", "magace_model_terms", "magace_profile", "magace_profile_read",
+           "magace_neighbors_device", "magace_packstructures")
+
+ERR_CAPACITY = 4
 
 
 class MagaceError(RuntimeError):
@@ -54,6 +57,10 @@ def lib():
     L.magace_basis_counts.argtypes = [_I]
     L.magace_enumerate.argtypes = [_I, C.c_int, C.c_void_p, C.c_int]
     L.magace_model_info.argtypes = [np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), C.c_int]
+    L.magace_neighbors_device.argtypes = [C.c_int, _I, _I, _D, C.c_double, C.c_void_p, C.c_void_p,
+                                          C.c_longlong] + [C.c_void_p] * 8 + [_IP, C.c_void_p, C.c_void_p]
+    L.magace_packstructures.argtypes = [C.c_int, _I, _I, _D, C.c_double, _D, _D, C.c_longlong,
+                                        _I, _I, _I, _I, _D, _D, _D, _D, _D, _D, _I, _IP]
     _lib = L
     return L
 
@@ -208,3 +215,68 @@ def profile_read():
     n = C.c_int(0)
     _check(L.magace_profile_read(ms, C.byref(n)))
     return ms, n.value
+
+
+def _structs(natoms, periodic, cells):
+    natoms = np.ascontiguousarray(natoms, dtype=np.int32).reshape(-1)
+    periodic = np.ascontiguousarray(periodic, dtype=np.int32).reshape(-1)
+    cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1)
+    if not (len(periodic) == len(natoms) and len(cells) == 9 * len(natoms)):
+        raise MagaceError("natoms, periodic and cells (9 per structure) disagree in length")
+    return natoms, periodic, cells
+
+
+def neighbors_device(natoms, periodic, cells, rcut, pos, mag, capacity, offsets, nlist, dlist, elist,
+                     dlist_mom, elist_mom, dlist_momi, elist_momi, stream=0):
+    """magace_neighbors_device: device pointers (ints) for pos/mag and the outputs; nlist = 0
+    counts only.  cells: (nstruc, 3, 3) with ROWS = lattice vectors.  Returns (rc, npairs,
+    natombox) where rc is 0 or ERR_CAPACITY; other errors raise."""
+    natoms, periodic, cells = _structs(natoms, periodic, cells)
+    npairs = C.c_int(0)
+    nbox = np.zeros(max(1, len(natoms)), np.int32)
+    rc = lib().magace_neighbors_device(len(natoms), natoms, periodic, cells, float(rcut), pos, mag,
+                                       int(capacity), offsets, nlist or None, dlist, elist, dlist_mom,
+                                       elist_mom, dlist_momi, elist_momi, C.byref(npairs),
+                                       nbox.ctypes.data, stream or None)
+    if rc not in (0, ERR_CAPACITY):
+        _check(rc)
+    return rc, npairs.value, nbox[:len(natoms)]
+
+
+def packstructures(natoms, periodic, cells, rcut, pos, mag, capacity=None):
+    """magace_packstructures on host arrays; returns the packed arrays with the reference's
+    names (src/neiutil.F90:5-264).  Grows the capacity and retries once if needed."""
+    natoms, periodic, cells = _structs(natoms, periodic, cells)
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+    mag = np.ascontiguousarray(mag, dtype=np.float64).reshape(-1)
+    ntot = int(natoms.sum())
+    if len(pos) != 3 * ntot or len(mag) != 3 * ntot:
+        raise MagaceError("pos and mag need 3 values per atom")
+    cap = int(capacity if capacity is not None else 96 * max(ntot, 1))
+    S = len(natoms)
+    for _ in range(2):
+        o = dict(strucaddress=np.zeros(2 * max(S, 1), np.int32), n2startpack=np.zeros(max(ntot, 1), np.int32),
+                 n2stoppack=np.zeros(max(ntot, 1), np.int32), n2listpack=np.zeros(max(cap, 1), np.int32),
+                 d2listpack=np.zeros(max(cap, 1)), e2listpack=np.zeros(3 * max(cap, 1)),
+                 d2listpack_mom=np.zeros(max(cap, 1)), e2listpack_mom=np.zeros(3 * max(cap, 1)),
+                 d2listpack_momi=np.zeros(max(ntot, 1)), e2listpack_momi=np.zeros(3 * max(ntot, 1)),
+                 natombox=np.zeros(max(S, 1), np.int32))
+        npairs = C.c_int(0)
+        rc = lib().magace_packstructures(S, natoms, periodic, cells, float(rcut), pos, mag, cap,
+                                         o["strucaddress"], o["n2startpack"], o["n2stoppack"], o["n2listpack"],
+                                         o["d2listpack"], o["e2listpack"], o["d2listpack_mom"],
+                                         o["e2listpack_mom"], o["d2listpack_momi"], o["e2listpack_momi"],
+                                         o["natombox"], C.byref(npairs))
+        if rc == ERR_CAPACITY:
+            cap = npairs.value
+            continue
+        _check(rc)
+        P = npairs.value
+        sizes = dict(strucaddress=2 * S, n2startpack=ntot, n2stoppack=ntot, n2listpack=P, d2listpack=P,
+                     e2listpack=3 * P, d2listpack_mom=P, e2listpack_mom=3 * P, d2listpack_momi=ntot,
+                     e2listpack_momi=3 * ntot, natombox=S)
+        out = {k: v[:sizes[k]].copy() for k, v in o.items()}
+        out["strucaddress"] = out["strucaddress"].reshape(S, 2)
+        out["npairs"] = P
+        return out
+    raise MagaceError("packstructures: capacity retry failed")

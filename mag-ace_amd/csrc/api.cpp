@@ -37,6 +37,14 @@ void clear_error() {
     g_err.clear();
 }
 
+}  // namespace
+
+// shared with the other translation units of the library (neighbour build)
+void report_error(int code, const std::string& msg) { set_error(code, msg); }
+void reset_error() { clear_error(); }
+
+namespace {
+
 const char* kElements[] = {
     "X",  "H",  "He", "Li", "Be", "B",  "C",  "N",  "O",  "F",  "Ne", "Na", "Mg", "Al", "Si", "P",  "S",
     "Cl", "Ar", "K",  "Ca", "Sc", "Ti", "V",  "Cr", "Mn", "Fe", "Co", "Ni", "Cu", "Zn", "Ga", "Ge", "As",

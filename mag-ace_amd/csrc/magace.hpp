@@ -97,6 +97,9 @@ struct Model {
     Splines spl;
 };
 
+// api.cpp: the library's last-error state (magace_last_error / magace_error_string)
+void report_error(int code, const std::string& msg);
+void reset_error();
 // potential.cpp
 bool parse_potential(const std::string& path, const std::string& element, int xc, double smear,
                      Potential& pot, std::string& err);

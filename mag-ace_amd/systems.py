@@ -27,6 +27,39 @@ def bcc(ncell: int, a: float = A_FE):
     return pos, cell
 
 
+def bcc_box(n1: int, n2: int, n3: int, a: float = A_FE):
+    """bcc lattice of n1 x n2 x n3 conventional cells in an orthorhombic periodic box."""
+    g = np.stack(np.meshgrid(np.arange(n1), np.arange(n2), np.arange(n3), indexing="ij"), -1)
+    corners = g.reshape(-1, 3).astype(np.float64)
+    pos = np.stack([corners, corners + 0.5], axis=1).reshape(-1, 3) * a
+    return pos, np.diag([n1 * a, n2 * a, n3 * a])
+
+
+def config_batch(nconf: int = 256, seed: int = 2029, lo: int = 16, hi: int = 1024, a: float = A_FE):
+    """Batched-fit workload: nconf periodic bcc Fe configurations with sizes log-uniform in
+    [lo, hi] atoms (rounded to 2*n1*n2*n3), isotropic strain U[-3%, +3%], 0-2 random
+    vacancies, jitter N(0, 0.05 A), directions uniform on the sphere with magnitudes
+    N(2.2, 0.1) muB.  Returns a list of dicts: pos, cell (rows = lattice vectors), sdir,
+    smag, mag = sdir * smag, periodic."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nconf):
+        target = np.exp(rng.uniform(np.log(lo), np.log(hi)))
+        nc = max(1, int(round(target / 2)))
+        n1 = max(1, int(round(nc ** (1.0 / 3.0))))
+        n2 = max(1, int(round(np.sqrt(nc / n1))))
+        n3 = max(1, int(round(nc / (n1 * n2))))
+        strain = rng.uniform(-0.03, 0.03)
+        pos, cell = bcc_box(n1, n2, n3, a * (1.0 + strain))
+        nvac = int(rng.integers(0, 3))
+        if nvac and len(pos) > nvac + 1:
+            pos = np.delete(pos, rng.choice(len(pos), nvac, replace=False), axis=0)
+        pos = jitter(pos, rng, 0.05)
+        sdir, smag = spins_uniform(len(pos), rng)
+        out.append(dict(pos=pos, cell=cell, sdir=sdir, smag=smag, mag=sdir * smag[:, None], periodic=True))
+    return out
+
+
 def jitter(pos, rng, sigma=0.05):
     return pos + rng.normal(0.0, sigma, size=pos.shape)
 

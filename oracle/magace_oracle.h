@@ -67,6 +67,19 @@ void orc_radcore(double r, double pre, double lambda, double cut, double *cr, do
 int orc_spline_eval(const orc_pot *p, int which, double r, double *val, double *dval);
 double orc_cutoff(const orc_pot *p, int which); /* 0 atomic global cutoff, 1 magnetic */
 
+/* Reference neighbour list of one structure (neilist/neilist.F90:148-796).
+ * cell[i + 3k] = component i of lattice vector k; posin/magin [3*natom].
+ * Outputs (reference conventions, 1-based): nstart/nstop[natom], nlist[nei] (cluster
+ * index), origin[natombox], nshift[3*natombox], dist/dist_mom[nei], eist/eist_mom[3*nei]
+ * with eist = (x_j - x_i)/d, dist_momi[natom], eist_momi[3*natom], posbox[3*natombox].
+ * counts = {natombox, nei}.  Returns 0, 1 if a capacity is too small (counts hold the
+ * sizes), or < 0 on an input the reference stops on. */
+int orc_neilist(int natom, int periodic, double rcut, const double *cell, const double *posin,
+                const double *magin, int cap_box, int cap_pair, int *counts, int *nstart,
+                int *nstop, int *nlist, int *origin, int *nshift, double *dist, double *eist,
+                double *dist_mom, double *eist_mom, double *dist_momi, double *eist_momi,
+                double *posbox);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DRIVER = os.path.join(HERE, "_ref", "ace_ref_driver")
+REF_NEILIST = os.path.join(HERE, "_ref", "neilist_ref")
 LIBORACLE = os.path.join(HERE, "liboracle.so")
 
 _D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -76,6 +77,8 @@ def lib():
         L.orc_spline_eval.argtypes = [C.c_void_p, C.c_int, C.c_double, _D, _D]
         L.orc_cutoff.restype = C.c_double
         L.orc_cutoff.argtypes = [C.c_void_p, C.c_int]
+        L.orc_neilist.argtypes = [C.c_int, C.c_int, C.c_double, _D, _D, _D, C.c_int, C.c_int, _I,
+                                  _I, _I, _I, _I, _I, _D, _D, _D, _D, _D, _D, _D]
         _lib = L
     return _lib
 
@@ -233,3 +236,121 @@ class RefDriver:
             o += n
         out["stdout"] = p.stdout
         return out
+
+
+_NL_INT = ("nstart", "nstop", "nlist", "origin", "nshift")
+
+
+def _nl_unpack(natom, nbox, nei, ints, dbls):
+    out = {"natom": natom, "natombox": nbox, "nei": nei}
+    o = 0
+    for k, n in (("nstart", natom), ("nstop", natom), ("nlist", nei), ("origin", nbox),
+                 ("nshift", 3 * nbox)):
+        out[k] = ints[o:o + n].copy()
+        o += n
+    o = 0
+    for k, n in (("dist", nei), ("eist", 3 * nei), ("dist_mom", nei), ("eist_mom", 3 * nei),
+                 ("dist_momi", natom), ("eist_momi", 3 * natom), ("pos", 3 * nbox)):
+        out[k] = dbls[o:o + n].copy()
+        o += n
+    return out
+
+
+def neilist(pos, mag, cell, rcut, periodic=True):
+    """C restatement of the reference neighbour list (neilist/neilist.F90:148-796).
+
+    pos, mag: (natom, 3); cell: (3, 3) with ROWS = lattice vectors.  Returns the reference's
+    module arrays (1-based indices, eist = (x_j - x_i)/d) as a dict of flat arrays."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+    mag = np.ascontiguousarray(mag, dtype=np.float64).reshape(-1)
+    natom = len(pos) // 3
+    cellc = np.ascontiguousarray(np.asarray(cell, dtype=np.float64).reshape(3, 3)).reshape(-1)
+    L = lib()
+    capb, capp = max(64, 64 * natom), max(64, 80 * natom)
+    for _ in range(3):
+        counts = np.zeros(2, np.int32)
+        I = {k: np.zeros(max(1, n), np.int32) for k, n in (("nstart", natom), ("nstop", natom),
+             ("nlist", capp), ("origin", capb), ("nshift", 3 * capb))}
+        D = {k: np.zeros(max(1, n)) for k, n in (("dist", capp), ("eist", 3 * capp), ("dist_mom", capp),
+             ("eist_mom", 3 * capp), ("dist_momi", natom), ("eist_momi", 3 * natom), ("pos", 3 * capb))}
+        rc = L.orc_neilist(natom, int(bool(periodic)), float(rcut), cellc, pos, mag, capb, capp, counts,
+                           I["nstart"], I["nstop"], I["nlist"], I["origin"], I["nshift"], D["dist"],
+                           D["eist"], D["dist_mom"], D["eist_mom"], D["dist_momi"], D["eist_momi"], D["pos"])
+        if rc < 0:
+            raise ValueError("orc_neilist: input rejected (%d)" % rc)
+        if rc == 0:
+            break
+        capb, capp = int(counts[0]), int(counts[1])
+    nbox, nei = int(counts[0]), int(counts[1])
+    out = {"natom": natom, "natombox": nbox, "nei": nei}
+    for k, n in (("nstart", natom), ("nstop", natom), ("nlist", nei), ("origin", nbox), ("nshift", 3 * nbox)):
+        out[k] = I[k][:n].copy()
+    for k, n in (("dist", nei), ("eist", 3 * nei), ("dist_mom", nei), ("eist_mom", 3 * nei),
+                 ("dist_momi", natom), ("eist_momi", 3 * natom), ("pos", 3 * nbox)):
+        out[k] = D[k][:n].copy()
+    return out
+
+
+def ref_neilist(pos, mag, cell, rcut, periodic=True, timeout=600.0):
+    """Runs the reference neilist itself (oracle/_ref/neilist_ref, built from
+    neilist/neilist.F90 by oracle/Makefile).  Same inputs and outputs as neilist()."""
+    if not os.path.exists(REF_NEILIST):
+        raise RuntimeError("reference neighbour list not built (oracle/_ref/neilist_ref)")
+    pos = np.ascontiguousarray(pos, dtype="<f8").reshape(-1)
+    mag = np.ascontiguousarray(mag, dtype="<f8").reshape(-1)
+    natom = len(pos) // 3
+    cellc = np.ascontiguousarray(np.asarray(cell, dtype="<f8").reshape(3, 3)).reshape(-1)
+    with tempfile.TemporaryDirectory(prefix="magace_nl_") as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(struct.pack("<2id", natom, int(bool(periodic)), float(rcut)))
+            f.write(cellc.tobytes() + pos.tobytes() + mag.tobytes())
+        p = subprocess.run([REF_NEILIST, fin, fout], capture_output=True, text=True, timeout=timeout)
+        if p.returncode != 0 or not os.path.exists(fout):
+            raise RuntimeError("reference neilist failed (%d): %s %s" % (p.returncode, p.stdout[-2000:],
+                                                                         p.stderr[-2000:]))
+        raw = open(fout, "rb").read()
+    natom, nbox, nei = struct.unpack("<3i", raw[:12])
+    ni = 2 * natom + nei + 4 * nbox
+    ints = np.frombuffer(raw[12:12 + 4 * ni], dtype="<i4")
+    dbls = np.frombuffer(raw[12 + 4 * ni:], dtype="<f8")
+    return _nl_unpack(natom, nbox, nei, ints, dbls)
+
+
+def packstructures(structures, rcut, builder=neilist):
+    """Restatement of packstructures (src/neiutil.F90:5-264): runs the neighbour list per
+    structure and concatenates.  structures: list of dicts with pos (n,3), mag (n,3), cell
+    (3,3, rows = lattice vectors), periodic.  Returns the packed arrays with the reference's
+    names: n2startpack/n2stoppack (per-structure local, 1-based), n2listpack = origin(nlist)
+    (local, 1-based), d2listpack, e2listpack, d2listpack_mom, e2listpack_mom,
+    d2listpack_momi, e2listpack_momi, strucaddress (nstruc, 2) 1-based atom and pair starts,
+    originpack, natomboxpackstart/stop.  (The reference's strucaddress has 4 columns of which
+    it fills 2, neiutil.F90:73,146,178.)"""
+    keys = ("n2startpack", "n2stoppack", "n2listpack", "d2listpack", "e2listpack", "d2listpack_mom",
+            "e2listpack_mom", "d2listpack_momi", "e2listpack_momi", "originpack")
+    acc = {k: [] for k in keys}
+    strucaddress, boxstart, boxstop = [], [], []
+    natompack = nei2pack = natomboxpack = 0
+    for s in structures:
+        nl = builder(s["pos"], s["mag"], s["cell"], rcut, s.get("periodic", True))
+        natom, nei = nl["natom"], nl["nei"]
+        boxstart.append(natomboxpack + 1)
+        boxstop.append(natomboxpack + nl["natombox"])
+        acc["originpack"].append(nl["origin"])
+        acc["d2listpack_momi"].append(nl["dist_momi"])
+        acc["e2listpack_momi"].append(nl["eist_momi"])
+        strucaddress.append((natompack + 1, nei2pack + 1))
+        acc["n2startpack"].append(nl["nstart"])
+        acc["n2stoppack"].append(nl["nstop"])
+        acc["n2listpack"].append(nl["origin"][nl["nlist"] - 1] if nei else np.zeros(0, np.int32))
+        for k, src in (("d2listpack", "dist"), ("e2listpack", "eist"), ("d2listpack_mom", "dist_mom"),
+                       ("e2listpack_mom", "eist_mom")):
+            acc[k].append(nl[src])
+        natompack += natom
+        nei2pack += nei
+        natomboxpack += nl["natombox"]
+    out = {k: (np.concatenate(v) if v else np.zeros(0)) for k, v in acc.items()}
+    out["strucaddress"] = np.array(strucaddress, np.int32).reshape(-1, 2)
+    out["natomboxpackstart"] = np.array(boxstart, np.int32)
+    out["natomboxpackstop"] = np.array(boxstop, np.int32)
+    return out
