@@ -35,6 +35,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel on this workload, from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes summarised in profiles/fe16k_lmax6_traffic.txt (scaled by the plain-copy
+    calibration recorded there).  None if the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "fe16k_lmax6_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(kernel + " "):
+            return int(dict(kv.split("=") for kv in line.split()[1:])["bytes"])
+    return None
+
+
 def cpu_baseline(pkg, pot, a, natom_sample, use_reference=True):
     """Reference libace (oracle/_ref, serial Fortran) on the first natom_sample atoms of the
     same workload; falls back to the oracle's C restatement (OpenMP) if the reference
@@ -71,6 +84,8 @@ def main():
     ap.add_argument("--ncell", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=400)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one plain 512 MiB device copy (the reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build evaluates on one GPU only; multi-GPU runs are not part of it "
@@ -84,6 +99,14 @@ def main():
     capi, potgen, systems = pkg.capi, pkg.potgen, pkg.systems
     import importlib
     flops = importlib.import_module("magace_amd.flops")
+
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: %d bytes read, %d bytes written" % (src.numel() * 8, dst.numel() * 8))
+        del src, dst
 
     H = potgen.H2000
     td = tempfile.mkdtemp(prefix="magace_bench_")
@@ -139,6 +162,23 @@ def main():
         log("%s: %.3f ms/step, %.3e atom-steps/s, kernels [density, adjoint, forces] ms %s" %
             (prec, res[prec]["ms_per_step"], res[prec]["value"], np.round(kms / max(cnt, 1), 4).tolist()))
 
+    # neighbour rebuild on the device (same pair set and order as the reference neilist)
+    neighbors = importlib.import_module("magace_amd.neighbors")
+    pos_t = T(s["pos"])
+    mag_t = T(s["sdir"] * s["smag"][:, None])
+    dn = neighbors.DeviceNeighbors([N], [1], [s["cell"]], 5.0)
+    for _ in range(2):
+        dn.build(pos_t, mag_t)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nb_reps = 10
+    for _ in range(nb_reps):
+        dn.build(pos_t, mag_t)
+    torch.cuda.synchronize()
+    nb_ms = 1e3 * (time.perf_counter() - t0) / nb_reps
+    assert dn.npairs == P, (dn.npairs, P)
+    log("neighbour build: %.3f ms (%d pairs)" % (nb_ms, dn.npairs))
+
     o64, o32 = res["fp64"]["out"], res["mixed"]["out"]
     _, f64, m64, _ = systems.assemble(a, o64["eout"], o64["flist"], o64["tlist"], o64["tlist0"])
     _, f32, m32, _ = systems.assemble(a, o32["eout"], o32["flist"], o32["tlist"], o32["tlist0"])
@@ -188,8 +228,12 @@ def main():
         "flops_per_atom_step": fm,
         "achieved_tflops_total": total_tf,
         "roofline": {"bound": "valu", "kernel": kernels[dom], "achieved": achieved, "peak": PEAK["mixed"],
-                     "unit": "TFLOP/s", "frac": achieved / PEAK["mixed"], "traffic": None},
+                     "unit": "TFLOP/s", "frac": achieved / PEAK["mixed"],
+                     "traffic": measured_traffic("k_%s<float>" % kernels[dom]) if args.ncell == 20 else None,
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"},
         "cpu_baseline": cpu,
+        "neighbor_build_ms": nb_ms,
+        "value_with_rebuild_every_10": N / (1e-3 * (res["mixed"]["ms_per_step"] + nb_ms / 10.0)),
     }
     if cpu:
         line["gpu_over_cpu"] = res["mixed"]["value"] / cpu["value"]
