@@ -627,7 +627,16 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                                                double* __restrict__ tlist0, int* err) {
     const int atom = blockIdx.x, lane = threadIdx.x;
     const int p0 = off[atom], nn = off[atom + 1] - p0;
-    const cplx<T>* __restrict__ W = Win + (size_t)atom * P.nz;
+    // W of this atom staged in LDS with coalesced loads: every lane reads the same entries
+    // (broadcast), which as scalar loads would thrash the scalar cache with 20-40 KB per atom
+    extern __shared__ __align__(16) unsigned char smem_f[];
+    cplx<T>* sW = reinterpret_cast<cplx<T>*>(smem_f);
+    {
+        const cplx<T>* __restrict__ Wg = Win + (size_t)atom * P.nz;
+        for (int i = lane; i < P.nz; i += 64) sW[i] = Wg[i];
+        __syncthreads();
+    }
+    const cplx<T>* W = sW;
     const int nfa = P.nr3a * (P.l3a + 1);
     const double edf = edfin[atom];
     // magnetic column q of A -> m' (for the representative test)
@@ -900,8 +909,11 @@ static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, co
                        B.rho1, eout, B.dE, B.edf);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     auto k3 = k_forces<T, L, L, NM3X, NBAX, NRAX, NR2X>;
+    const size_t lds3 = (size_t)P.nz * sizeof(cplx<T>);
+    e = hipFuncSetAttribute((const void*)k3, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds3);
+    if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[2], s);
-    hipLaunchKernelGGL(k3, dim3(natom), dim3(64), 0, s, P, off, dlist, elist, dmomi, emomi, dmom, emom,
+    hipLaunchKernelGGL(k3, dim3(natom), dim3(64), lds3, s, P, off, dlist, elist, dmomi, emomi, dmom, emom,
                        (const cplx<T>*)B.W, B.dE, B.edf, flist, tlist, tlist0, B.err);
     if (ev) (void)hipEventRecord(ev[3], s);
     return hipGetLastError();
