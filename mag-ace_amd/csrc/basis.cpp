@@ -8,8 +8,10 @@
 // merged and complex-conjugate partners folded, so the device evaluates
 // rho = sum_k c_k B_k and its gradient with one pass over a flat table.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <unordered_map>
 
@@ -378,6 +380,20 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
     }
     std::stable_sort(tails.begin(), tails.end(), [](const Ch& a, const Ch& b) { return a.cnt > b.cnt; });
     order.insert(order.end(), tails.begin(), tails.end());
+    // Inside a chunk the records can be summed in any order.  At step k the 64 lanes read
+    // Z[a], Z[b], Z[c] of their k-th records from LDS (three reads, random addresses), and
+    // LDS serves a wave in lane groups whose accesses must fall in distinct banks.  The
+    // order inside each chunk is therefore chosen greedily, step by step, so that the
+    // factor indices of the lanes of a group spread over the bank classes: x mod 8, mod 16
+    // and mod 32 over groups of 8, 16 and 32 lanes (16-byte FP64 and 8-byte FP32 entries
+    // with 32 or 64 banks).  Padding (index nz) is a broadcast read.
+    struct Spread {
+        int load[3][64];  // t = 0, 1, 2: groups of 8 << t lanes, classes x mod (8 << t)
+        static int slot(int lane, int v, int t) {
+            const int sh = 3 + t;
+            return (lane >> sh) * (1 << sh) + (v & ((1 << sh) - 1));
+        }
+    };
     for (size_t g0 = 0; g0 < order.size(); g0 += 64) {
         const size_t g1 = std::min(order.size(), g0 + 64);
         int len = 0;
@@ -385,18 +401,42 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
         const size_t base = T.cw.size();
         T.cidx.resize((base + (size_t)len * 64) * 4, (uint16_t)one);
         T.cw.resize(base + (size_t)len * 64, 0.0);
-        for (size_t c = g0; c < g1; ++c) {
-            const Ch& ch = order[c];
-            const int lane = (int)(c - g0);
-            for (int k = 0; k < ch.cnt; ++k) {
-                const auto& e = by[ch.x][ch.off + k];
-                const size_t r = base + (size_t)k * 64 + lane;
+        const int nl = (int)(g1 - g0);
+        std::vector<std::vector<int>> left(nl);
+        for (int L = 0; L < nl; ++L)
+            for (int k = 0; k < order[g0 + L].cnt; ++k) left[L].push_back(k);
+        for (int k = 0; k < len; ++k) {
+            Spread sp[3];
+            std::memset(sp, 0, sizeof sp);
+            for (int L = 0; L < nl; ++L) {
+                auto& rem = left[L];
+                if (rem.empty()) continue;
+                const Ch& ch = order[g0 + L];
+                int best = 0, bestc = INT_MAX;
+                for (int i = 0; i < (int)rem.size(); ++i) {
+                    const auto& e = by[ch.x][ch.off + rem[i]];
+                    int cost = 0;
+                    for (int f = 0; f < 3; ++f) {
+                        if (e.first[f] == one) continue;
+                        for (int t = 0; t < 3; ++t) cost += sp[f].load[t][Spread::slot(L, e.first[f], t)];
+                    }
+                    if (cost < bestc) { bestc = cost; best = i; }
+                }
+                const auto& e = by[ch.x][ch.off + rem[best]];
+                for (int f = 0; f < 3; ++f)
+                    if (e.first[f] != one)
+                        for (int t = 0; t < 3; ++t) ++sp[f].load[t][Spread::slot(L, e.first[f], t)];
+                const size_t r = base + (size_t)k * 64 + L;
                 T.cidx[4 * r] = (uint16_t)e.first[0];
                 T.cidx[4 * r + 1] = (uint16_t)e.first[1];
                 T.cidx[4 * r + 2] = (uint16_t)e.first[2];
                 T.cw[r] = e.second;
+                rem.erase(rem.begin() + best);
             }
-            T.chunks.push_back((int32_t)(base + lane));
+        }
+        for (int L = 0; L < nl; ++L) {
+            const Ch& ch = order[g0 + L];
+            T.chunks.push_back((int32_t)(base + L));
             T.chunks.push_back(ch.x | (((ch.cnt + 3) & ~3) << 16));
         }
     }
