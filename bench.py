@@ -48,6 +48,41 @@ def measured_traffic(kernel):
     return None
 
 
+def md_leg(pkg, td, steps=100, warmup=20):
+    """Secondary measurement: device-resident spin-lattice MD, 128000 bcc Fe atoms (40^3
+    cells), lmax-4 header "8 3 2 4 / 4 1 4 1 / 2 1 2 1" with c ~ N(0, 1e-3) (seed 2030),
+    displacements N(0, 0.063 A) (1000 K, omega = 61.5 rad/ps, Debye 470 K), Maxwell velocities
+    at 1000 K, uniform spins, dt = 0.1 fs, list cutoff 5.5 A rebuilt every 10 steps on the
+    device, steps replayed from a captured HIP graph, mixed precision."""
+    capi, potgen, systems, md = pkg.capi, pkg.potgen, pkg.systems, pkg.md
+    h = dict(potgen.H500)
+    pot = os.path.join(td, "Fe_md.pbe.in")
+    with open(pot, "w") as f:
+        f.write(potgen.render(h, capi.enumerate_basis(h), 2030, c_scale=1e-3))
+    capi.init_file(pot)
+    capi.set_precision(capi.MIXED)
+    rng = np.random.default_rng(2030)
+    pos, cell = systems.bcc(40)
+    pos = systems.jitter(pos, rng, 0.063)
+    sdir, smag = systems.spins_uniform(len(pos), rng)
+    vel = md.maxwell_velocities(len(pos), 1000.0, rng)
+    m = md.SpinLatticeMD(pos, cell, sdir, smag, vel=vel, dt=1e-4, rcut=5.0, skin=0.5, rebuild_every=10)
+    st0 = m.state()
+    m.run(warmup)
+    m.state()
+    t0 = time.perf_counter()
+    m.run(steps)
+    st = m.state()
+    dt = time.perf_counter() - t0
+    m.close()
+    n = len(pos)
+    return {"workload": "spin-lattice MD, bcc Fe 40x40x40 (%d atoms), lmax-4 header (489 B functions), "
+                        "1000 K, dt 0.1 fs, neighbour rebuild every 10 steps on the device, HIP graph" % n,
+            "atoms": n, "steps": steps, "ms_per_step": 1e3 * dt / steps, "atom_steps_per_s": n * steps / dt,
+            "pairs": st["pairs"], "rebuilds": st["rebuilds"] - st0["rebuilds"], "graph_captures": st["captures"],
+            "energy_drift_per_atom": (st["epot"] + st["ekin"] - st0["epot"] - st0["ekin"]) / n}
+
+
 def cpu_baseline(pkg, pot, a, natom_sample, use_reference=True):
     """Reference libace (oracle/_ref, serial Fortran) on the first natom_sample atoms of the
     same workload; falls back to the oracle's C restatement (OpenMP) if the reference
@@ -84,6 +119,7 @@ def main():
     ap.add_argument("--ncell", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=400)
+    ap.add_argument("--no-md", action="store_true", help="skip the 128000-atom spin-lattice MD leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain 512 MiB device copy (the reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -195,6 +231,11 @@ def main():
     achieved = fm[kernels[dom]] * N / (km[dom] * 1e-3) / 1e12
     total_tf = fm["total"] * res["mixed"]["value"] / 1e12
 
+    mdres = None
+    if not args.no_md:
+        mdres = md_leg(pkg, td)
+        log("md (128000 atoms):", {k: v for k, v in mdres.items() if k != "workload"})
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -233,6 +274,7 @@ def main():
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"},
         "cpu_baseline": cpu,
         "neighbor_build_ms": nb_ms,
+        "spin_lattice_md": mdres,
         "value_with_rebuild_every_10": N / (1e-3 * (res["mixed"]["ms_per_step"] + nb_ms / 10.0)),
     }
     if cpu:

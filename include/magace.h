@@ -100,7 +100,10 @@ int magace_profile_read(double *ms, int *count);
  * magace_compute_device: offsets[ntot+1] (0-based global CSR), nlist[p] = global 0-based
  * atom index of the neighbour (the periodic image's origin), dlist[p] = r,
  * elist[3p] = (x_i - x_j)/r (the negated reference eist), dlist_mom/elist_mom of the
- * neighbour, dlist_momi/elist_momi of the central atom.  Host outputs: *npairs, natombox[s]
+ * neighbour, dlist_momi/elist_momi of the central atom, and (optional, may be NULL)
+ * shifts[3p] = integer lattice shift n of the neighbour's image, i.e. the image sits at
+ * x_j + sum_k n_k a_k for the positions as passed (not wrapped), so pair vectors can be
+ * recomputed from moved positions between rebuilds.  Host outputs: *npairs, natombox[s]
  * (atoms of the image cluster; may be NULL).  nlist == NULL counts only (offsets filled).
  * Synchronises `stream` once to learn the pair count; returns MAGACE_ERR_CAPACITY (with
  * *npairs set, offsets filled) when capacity < *npairs.  Errors as the reference's stops:
@@ -108,8 +111,8 @@ int magace_profile_read(double *ms, int *count);
 int magace_neighbors_device(int nstruc, const int *natoms, const int *periodic, const double *cells,
                             double rcut, const double *pos, const double *mag, long long capacity,
                             int *offsets, int *nlist, double *dlist, double *elist, double *dlist_mom,
-                            double *elist_mom, double *dlist_momi, double *elist_momi, int *npairs,
-                            int *natombox, void *stream);
+                            double *elist_mom, double *dlist_momi, double *elist_momi, int *shifts,
+                            int *npairs, int *natombox, void *stream);
 /* packstructures (src/neiutil.F90:5-264) on host arrays, computed on the device, returned in
  * the reference's conventions: strucaddress[2*s] = 1-based first atom, [2*s+1] = 1-based
  * first pair; n2startpack/n2stoppack[ntot] 1-based ranges local to the structure;
@@ -122,6 +125,28 @@ int magace_packstructures(int nstruc, const int *natoms, const int *periodic, co
                           double *d2listpack, double *e2listpack, double *d2listpack_mom,
                           double *e2listpack_mom, double *d2listpack_momi, double *e2listpack_momi,
                           int *natombox, int *npairs);
+
+/* Device-resident spin-lattice dynamics around the evaluator (replaces the LAMMPS-side loop
+ * around pair_spin_ace: PairSpinACE::compute's assembly, lammps/pair_spin_ace.cpp:571-641, and
+ * the per-spin acelib_ sweep of compute_single_pair, :677-986).  Velocity Verlet for the
+ * lattice (metal units: A, ps, amu, eV) and rotations of the spins about
+ * Omega_i = -(2/hbar) dE/dm_i with the field extrapolated to mid-step (second order), one
+ * evaluation per step, neighbour lists (cutoff rlist =
+ * r_c + skin) rebuilt every rebuild_every steps, the steps in between captured once into a
+ * HIP graph when use_graph != 0.  Periodic cell[3k + i] = component i of lattice vector k;
+ * x, v [3*natom]; sdir [3*natom] spin directions (normalised here), smag [natom] moment
+ * magnitudes (muB, constant).  Needs the potential loaded (acelibini_ / magace_init_file);
+ * uses the current precision mode. */
+int magace_md_create(int natom, const double *cell, double rlist, double mass, double dt,
+                     int rebuild_every, const double *x, const double *v, const double *sdir,
+                     const double *smag, void **md);
+int magace_md_run(void *md, int nsteps, int use_graph);
+/* Copies the state back (any output may be NULL): x, v, sdir, f (forces), dEdm [3*natom],
+ * eatom [natom]; epot = sum eatom and ekin (eV); counters = {steps, rebuilds, graph
+ * captures, pairs}. */
+int magace_md_read(void *md, double *x, double *v, double *sdir, double *f, double *dEdm,
+                   double *eatom, double *epot, double *ekin, long long *counters);
+int magace_md_destroy(void *md);
 
 /* Basis introspection: counts of B functions of orders 1..4 of the loaded potential
  * (the numbers src/util.F90:1831,1929,2235,2596 print). */

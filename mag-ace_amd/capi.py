@@ -25,7 +25,8 @@ EXPORTS = ("acelibini_", "acelib_", "magace_last_error", "magace_error_string", 
            "magace_finalize", "magace_set_precision", "magace_get_precision", "magace_compute_device",
            "magace_check_errors", "magace_basis_counts", "magace_enumerate", "magace_model_info",
            "magace_pair_compute", "magace_model_terms", "magace_profile", "magace_profile_read",
-           "magace_neighbors_device", "magace_packstructures")
+           "magace_neighbors_device", "magace_packstructures", "magace_md_create", "magace_md_run",
+           "magace_md_read", "magace_md_destroy")
 
 ERR_CAPACITY = 4
 
@@ -58,7 +59,12 @@ def lib():
     L.magace_enumerate.argtypes = [_I, C.c_int, C.c_void_p, C.c_int]
     L.magace_model_info.argtypes = [np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), C.c_int]
     L.magace_neighbors_device.argtypes = [C.c_int, _I, _I, _D, C.c_double, C.c_void_p, C.c_void_p,
-                                          C.c_longlong] + [C.c_void_p] * 8 + [_IP, C.c_void_p, C.c_void_p]
+                                          C.c_longlong] + [C.c_void_p] * 9 + [_IP, C.c_void_p, C.c_void_p]
+    L.magace_md_create.argtypes = [C.c_int, _D, C.c_double, C.c_double, C.c_double, C.c_int, _D, _D, _D, _D,
+                                   C.POINTER(C.c_void_p)]
+    L.magace_md_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.magace_md_read.argtypes = [C.c_void_p] + [C.c_void_p] * 8 + [C.c_void_p]
+    L.magace_md_destroy.argtypes = [C.c_void_p]
     L.magace_packstructures.argtypes = [C.c_int, _I, _I, _D, C.c_double, _D, _D, C.c_longlong,
                                         _I, _I, _I, _I, _D, _D, _D, _D, _D, _D, _I, _IP]
     _lib = L
@@ -227,7 +233,7 @@ def _structs(natoms, periodic, cells):
 
 
 def neighbors_device(natoms, periodic, cells, rcut, pos, mag, capacity, offsets, nlist, dlist, elist,
-                     dlist_mom, elist_mom, dlist_momi, elist_momi, stream=0):
+                     dlist_mom, elist_mom, dlist_momi, elist_momi, stream=0, shifts=0):
     """magace_neighbors_device: device pointers (ints) for pos/mag and the outputs; nlist = 0
     counts only.  cells: (nstruc, 3, 3) with ROWS = lattice vectors.  Returns (rc, npairs,
     natombox) where rc is 0 or ERR_CAPACITY; other errors raise."""
@@ -236,7 +242,7 @@ def neighbors_device(natoms, periodic, cells, rcut, pos, mag, capacity, offsets,
     nbox = np.zeros(max(1, len(natoms)), np.int32)
     rc = lib().magace_neighbors_device(len(natoms), natoms, periodic, cells, float(rcut), pos, mag,
                                        int(capacity), offsets, nlist or None, dlist, elist, dlist_mom,
-                                       elist_mom, dlist_momi, elist_momi, C.byref(npairs),
+                                       elist_mom, dlist_momi, elist_momi, shifts or None, C.byref(npairs),
                                        nbox.ctypes.data, stream or None)
     if rc not in (0, ERR_CAPACITY):
         _check(rc)

@@ -249,7 +249,7 @@ int run_device(Context& c, int natom, const int* off, const double* dlist, const
                double* eout, double* flist, double* tlist, double* tlist0, hipStream_t s) {
     if (int rc = upload(c)) return rc;
     if (int rc = ensure_atoms(c, (size_t)natom)) return rc;
-    HIPCHK(hipMemsetAsync(c.buf.err, 0, sizeof(int), s));
+    HIPCHK(clear_flags(c.buf.err, s));
     hipEvent_t* ev = nullptr;
     if (c.profiling) {
         if (c.events_used == c.events.size()) {

@@ -875,6 +875,14 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
 }
 
 // ============================================================================ launch
+__global__ void k_clear_flags(int* p) { *p = 0; }
+
+hipError_t clear_flags(int* p, hipStream_t s) {
+    // a kernel rather than hipMemsetAsync, so that the evaluation can be captured in a graph
+    hipLaunchKernelGGL(k_clear_flags, dim3(1), dim3(1), 0, s, p);
+    return hipGetLastError();
+}
+
 // shape classes: 0 = small magnetic A block (nm3 <= 4, nr2m <= 2, nb_a <= 10, nr3a <= 8, l <= 6)
 //                1 = wide magnetic A block  (nm3 <= 27, nr2m <= 5, nb_a <= 16, nr3a <= 4, l <= 2)
 int supported_nm3_class(int nm3, int nr2m, int nb_a, int l3a, int l2m, int nr3a) {
@@ -894,24 +902,40 @@ static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, co
     Pd.lhm = P.lhm; Pd.nm3 = P.nm3; Pd.nb_a = P.nb_a; Pd.nb_m = P.nb_m;
     const size_t lds1 = density_lds_bytes(Pd, CH, sizeof(T));
     auto k1 = k_density<T, L, CH, NB, NM3X, NB2, NR2X>;
-    hipError_t e = hipFuncSetAttribute((const void*)k1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1);
-    if (e != hipSuccess) return e;
+    auto k2 = k_adjoint<T>;
+    auto k3 = k_forces<T, L, L, NM3X, NBAX, NRAX, NR2X>;
+    hipError_t e = hipSuccess;
+    // the dynamic-LDS limit is raised once per kernel instantiation (to the workgroup
+    // maximum), so that evaluations can be captured into graphs without further API calls
+    static bool attributes_set = false;
+    static size_t limit[3] = {0, 0, 0};
+    if (!attributes_set) {
+        int dev = 0, maxsh = 0;
+        if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+        if ((e = hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev)) != hipSuccess)
+            return e;
+        const void* ks[3] = {(const void*)k1, (const void*)k2, (const void*)k3};
+        for (int i = 0; i < 3; ++i) {
+            hipFuncAttributes fa;
+            if ((e = hipFuncGetAttributes(&fa, ks[i])) != hipSuccess) return e;
+            limit[i] = (size_t)maxsh > fa.sharedSizeBytes ? (size_t)maxsh - fa.sharedSizeBytes : 0;
+            if ((e = hipFuncSetAttribute(ks[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)limit[i])) !=
+                hipSuccess)
+                return e;
+        }
+        attributes_set = true;
+    }
+    const size_t lds2 = 2 * (size_t)(P.nz + 1) * sizeof(cplx<T>) + 8 * sizeof(double);
+    const size_t lds3 = (size_t)P.nz * sizeof(cplx<T>);
+    if (lds1 > limit[0] || lds2 > limit[1] || lds3 > limit[2]) return hipErrorInvalidValue;
     if (ev) (void)hipEventRecord(ev[0], s);
     hipLaunchKernelGGL(k1, dim3(natom), dim3(256), lds1, s, P, off, dlist, elist, dmomi, emomi, dmom,
                        emom, (cplx<T>*)B.Z, B.ehc, B.rho1, B.err);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    const size_t lds2 = 2 * (size_t)(P.nz + 1) * sizeof(cplx<T>) + 8 * sizeof(double);
-    auto k2 = k_adjoint<T>;
-    e = hipFuncSetAttribute((const void*)k2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2);
-    if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[1], s);
     hipLaunchKernelGGL(k2, dim3(natom), dim3(256), lds2, s, P, (const cplx<T>*)B.Z, (cplx<T>*)B.W, B.ehc,
                        B.rho1, eout, B.dE, B.edf);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    auto k3 = k_forces<T, L, L, NM3X, NBAX, NRAX, NR2X>;
-    const size_t lds3 = (size_t)P.nz * sizeof(cplx<T>);
-    e = hipFuncSetAttribute((const void*)k3, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds3);
-    if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(k3, dim3(natom), dim3(64), lds3, s, P, off, dlist, elist, dmomi, emomi, dmom, emom,
                        (const cplx<T>*)B.W, B.dE, B.edf, flist, tlist, tlist0, B.err);

@@ -61,6 +61,7 @@ struct Launch {
 };
 
 hipError_t upload_legendre_constants();
+hipError_t clear_flags(int* p, hipStream_t s);
 int supported_nm3_class(int nm3, int nr2m, int nb_a, int l3a, int l2m, int nr3a);
 
 }  // namespace magace

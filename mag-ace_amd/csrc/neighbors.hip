@@ -184,8 +184,9 @@ __global__ __launch_bounds__(256) void k_nb_bbox(const NbGeom* __restrict__ G, c
 }
 
 // fractional coordinates wrapped into [0,1) (periodic) -- neilist.F90:356-367
+// afl = the integer part removed by the wrap (image bookkeeping for pair shifts)
 __global__ void k_nb_frac(const NbGeom* __restrict__ G, const int* __restrict__ atom_off, int nstruc, int ntot,
-                          const double* __restrict__ pos, double* __restrict__ frac) {
+                          const double* __restrict__ pos, double* __restrict__ frac, int* __restrict__ afl) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= ntot) return;
     const NbGeom& g = G[find_struct(atom_off, nstruc, a)];
@@ -193,9 +194,12 @@ __global__ void k_nb_frac(const NbGeom* __restrict__ G, const int* __restrict__ 
     for (int k = 0; k < 3; ++k) {
         if (g.periodic) {
             const double v = g.ci[k] * p0 + g.ci[k + 3] * p1 + g.ci[k + 6] * p2;
-            frac[3 * (size_t)a + k] = v - floor(v + kEps);
+            const double fl = floor(v + kEps);
+            frac[3 * (size_t)a + k] = v - fl;
+            afl[3 * (size_t)a + k] = (int)fmax(fmin(fl, 1.0e9), -1.0e9);
         } else {
             frac[3 * (size_t)a + k] = k == 0 ? p0 : (k == 1 ? p1 : p2);
+            afl[3 * (size_t)a + k] = 0;
         }
     }
 }
@@ -253,8 +257,8 @@ __global__ void k_nb_flag(const NbGeom* __restrict__ G, const long long* __restr
 __global__ void k_nb_cluster(const NbGeom* __restrict__ G, const long long* __restrict__ cand_off, int nstruc,
                              long long ncand, const double* __restrict__ frac, const int* __restrict__ flag,
                              const int* __restrict__ cidx, double invrc, double* __restrict__ cpos,
-                             int* __restrict__ corg, int* __restrict__ cbox, int* __restrict__ bcount,
-                             int* __restrict__ err) {
+                             int* __restrict__ corg, int* __restrict__ cbox, int* __restrict__ cshift,
+                             int* __restrict__ bcount, int* __restrict__ err) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ncand || !flag[i]) return;
     const Cand c = decode(G, cand_off, nstruc, i);
@@ -286,6 +290,9 @@ __global__ void k_nb_cluster(const NbGeom* __restrict__ G, const long long* __re
     else
         atomicOr(err, kNbErrBox);
     for (int n = 0; n < 3; ++n) cpos[3 * (size_t)ci + n] = r2[n];
+    cshift[3 * (size_t)ci] = c.k0;
+    cshift[3 * (size_t)ci + 1] = c.k1;
+    cshift[3 * (size_t)ci + 2] = c.k2;
     corg[ci] = c.a;
     cbox[ci] = box;
     if (box >= 0) atomicAdd(&bcount[box], 1);
@@ -321,6 +328,7 @@ struct PairOut {
     const int* offsets;
     int* nlist;
     double *dl, *el, *dm, *em, *dmi, *emi;
+    int* shifts;  // optional: lattice shift n_p of the neighbour image (x_j + cell n_p), 3 per pair
 };
 
 // One wavefront per central atom.  Stencil lane t (< 27): t = 0 the atom's own box, t >= 1
@@ -333,6 +341,7 @@ __global__ __launch_bounds__(256) void k_nb_pairs(const NbGeom* __restrict__ G, 
                                                   const double* __restrict__ cpos, const int* __restrict__ corg,
                                                   const int* __restrict__ cbox, const int* __restrict__ bstart,
                                                   const int* __restrict__ batoms, const double* __restrict__ mag,
+                                                  const int* __restrict__ cshift, const int* __restrict__ afl,
                                                   double rcut, int* __restrict__ counts, PairOut out,
                                                   int* __restrict__ err) {
     const int lane = threadIdx.x & 63;
@@ -405,6 +414,10 @@ __global__ __launch_bounds__(256) void k_nb_pairs(const NbGeom* __restrict__ G, 
                 out.el[3 * (size_t)p] = -(v0 / d);
                 out.el[3 * (size_t)p + 1] = -(v1 / d);
                 out.el[3 * (size_t)p + 2] = -(v2 / d);
+                if (out.shifts)
+                    for (int n = 0; n < 3; ++n)
+                        out.shifts[3 * (size_t)p + n] = cshift[3 * (size_t)c2 + n] - afl[3 * (size_t)j + n] +
+                                                        afl[3 * (size_t)a + n];
                 const double mx = mag[3 * (size_t)j], my = mag[3 * (size_t)j + 1], mz = mag[3 * (size_t)j + 2];
                 const double mm = sqrt(mx * mx + my * my + mz * mz);
                 out.dm[p] = mm;
@@ -466,6 +479,8 @@ struct NbWork {
     double* cpos = nullptr; size_t c_cpos = 0;
     int* corg = nullptr; size_t c_corg = 0;
     int* cbox = nullptr; size_t c_cbox = 0;
+    int* cshift = nullptr; size_t c_cshift = 0;
+    int* afl = nullptr; size_t c_afl = 0;
     int* bcount = nullptr; size_t c_bcount = 0;
     int* bstart = nullptr; size_t c_bstart = 0;
     int* bcur = nullptr; size_t c_bcur = 0;
@@ -711,6 +726,8 @@ int build(const Request& R) {
     NBGROW(cpos, 3 * (size_t)ncand + 1);
     NBGROW(corg, ncand + 1);
     NBGROW(cbox, ncand + 1);
+    NBGROW(cshift, 3 * (size_t)ncand + 1);
+    NBGROW(afl, 3 * (size_t)ntot + 1);
     NBGROW(bcount, nbox + 1);
     NBGROW(bstart, nbox + 1);
     NBGROW(bcur, nbox + 1);
@@ -726,7 +743,8 @@ int build(const Request& R) {
         k_fill_int<<<nblocks(nbox, 256), 256, 0, s>>>(g_nb.bcur, nbox, 0);
     }
     if (ntot > 0)
-        k_nb_frac<<<nblocks(ntot, 256), 256, 0, s>>>(g_nb.geom, g_nb.atom_off, S, (int)ntot, R.pos, g_nb.frac);
+        k_nb_frac<<<nblocks(ntot, 256), 256, 0, s>>>(g_nb.geom, g_nb.atom_off, S, (int)ntot, R.pos, g_nb.frac,
+                                                      g_nb.afl);
     if (ncand > 0)
         k_nb_flag<<<nblocks(ncand, 256), 256, 0, s>>>(g_nb.geom, g_nb.cand_off, S, ncand, g_nb.frac, g_nb.flag);
     NBCHK(hipGetLastError());
@@ -735,7 +753,7 @@ int build(const Request& R) {
     if (ncand > 0)
         k_nb_cluster<<<nblocks(ncand, 256), 256, 0, s>>>(g_nb.geom, g_nb.cand_off, S, ncand, g_nb.frac, g_nb.flag,
                                                          g_nb.cidx, invrc, g_nb.cpos, g_nb.corg, g_nb.cbox,
-                                                         g_nb.bcount, g_nb.err);
+                                                         g_nb.cshift, g_nb.bcount, g_nb.err);
     NBCHK(hipGetLastError());
     if (int rc = scan(g_nb.bcount, nbox, g_nb.bstart, s)) return rc;
     if (ncand > 0)
@@ -746,7 +764,8 @@ int build(const Request& R) {
     if (ntot > 0)
         k_nb_pairs<false><<<nblocks(ntot, 4), 256, 0, s>>>(g_nb.geom, g_nb.atom_off, S, (int)ntot, g_nb.cidx,
                                                             g_nb.cpos, g_nb.corg, g_nb.cbox, g_nb.bstart,
-                                                            g_nb.batoms, R.mag, R.rcut, g_nb.counts, none, g_nb.err);
+                                                            g_nb.batoms, R.mag, g_nb.cshift, g_nb.afl, R.rcut,
+                                                            g_nb.counts, none, g_nb.err);
     NBCHK(hipGetLastError());
     if (int rc = scan(g_nb.counts, ntot, R.offsets, s)) return rc;
     k_nb_gather_clusters<<<nblocks(S + 1, 64), 64, 0, s>>>(S, g_nb.geom, g_nb.cidx, ncand, g_nb.clus);
@@ -779,7 +798,8 @@ int build(const Request& R) {
         o.offsets = R.offsets;
         k_nb_pairs<true><<<nblocks(ntot, 4), 256, 0, s>>>(g_nb.geom, g_nb.atom_off, S, (int)ntot, g_nb.cidx,
                                                            g_nb.cpos, g_nb.corg, g_nb.cbox, g_nb.bstart,
-                                                           g_nb.batoms, R.mag, R.rcut, g_nb.counts, o, g_nb.err);
+                                                           g_nb.batoms, R.mag, g_nb.cshift, g_nb.afl, R.rcut,
+                                                           g_nb.counts, o, g_nb.err);
         NBCHK(hipGetLastError());
     }
     return 0;
@@ -795,11 +815,11 @@ extern "C" {
 int magace_neighbors_device(int nstruc, const int* natoms, const int* periodic, const double* cells,
                             double rcut, const double* pos, const double* mag, long long capacity,
                             int* offsets, int* nlist, double* dlist, double* elist, double* dlist_mom,
-                            double* elist_mom, double* dlist_momi, double* elist_momi, int* npairs,
-                            int* natombox, void* stream) {
+                            double* elist_mom, double* dlist_momi, double* elist_momi, int* shifts,
+                            int* npairs, int* natombox, void* stream) {
     reset_error();
     Request R{nstruc, natoms, periodic, cells, rcut, pos, mag, capacity, offsets,
-              PairOut{offsets, nlist, dlist, elist, dlist_mom, elist_mom, dlist_momi, elist_momi},
+              PairOut{offsets, nlist, dlist, elist, dlist_mom, elist_mom, dlist_momi, elist_momi, shifts},
               npairs, natombox, (hipStream_t)stream};
     if (nlist && (!dlist || !elist || !dlist_mom || !elist_mom || !dlist_momi || !elist_momi)) {
         report_error(MAGACE_ERR_INPUT, "neighbour build: missing output arrays");
@@ -841,7 +861,7 @@ int magace_packstructures(int nstruc, const int* natoms, const int* periodic, co
     NBGROW(dmi, ntot + 1);
     NBGROW(emi, 3 * ntot + 1);
     Request R{nstruc, natoms, periodic, cells, rcut, g_nb.pos, g_nb.mag, cap, g_nb.off,
-              PairOut{g_nb.off, g_nb.nl, g_nb.dl, g_nb.el, g_nb.dm, g_nb.em, g_nb.dmi, g_nb.emi},
+              PairOut{g_nb.off, g_nb.nl, g_nb.dl, g_nb.el, g_nb.dm, g_nb.em, g_nb.dmi, g_nb.emi, nullptr},
               npairs, natombox, s};
     if (int rc = build(R)) return rc;
     NBCHK(hipStreamSynchronize(s));
