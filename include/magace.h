@@ -62,8 +62,8 @@ int magace_set_precision(int mode);
 int magace_get_precision(void);
 
 /* acelib_ on device-resident arrays (same meaning as acelib_, FP64), with 0-based CSR
- * offsets[natom+1] instead of nstart/nstop.  Enqueued on `stream` (a hipStream_t, or NULL for
- * the library's stream); no host synchronisation.  magace_check_errors() synchronises the
+ * offsets[natom+1] instead of nstart/nstop.  Enqueued on `stream` (a hipStream_t; NULL is the
+ * default stream, as in HIP); no host synchronisation.  magace_check_errors() synchronises the
  * stream and reports geometry errors raised by the kernels. */
 int magace_compute_device(int natom, int nei, const int *offsets, const double *dlist,
                           const double *elist, const double *dlist_momi, const double *elist_momi,
@@ -147,6 +147,33 @@ int magace_md_run(void *md, int nsteps, int use_graph);
 int magace_md_read(void *md, double *x, double *v, double *sdir, double *f, double *dEdm,
                    double *eatom, double *epot, double *ekin, long long *counters);
 int magace_md_destroy(void *md);
+
+/* Fitting (FP64).  Coefficients are ordered as the potential file's B functions: c1 (order
+ * 1, nradbase_magnetic of them), then orders 2, 3, 4 in enumeration order; the reference's
+ * c0..c4 arrays (src/modules.F90) and its writer (src/util.F90:1110-1362) use the same order.
+ * magace_fit_set_coefficients recomputes the merged monomial weights (the contraction of
+ * src/forces.F90:93-141) and takes effect for every later evaluation.
+ * magace_fit_forward: an FP64 evaluation on device arrays (acelib_ meaning, 0-based CSR
+ * offsets, nlist[p] = global 0-based neighbour atom) that also assembles per-atom forces
+ * forces[3*natom] and per-structure energies estruct[nstruc] (atom_struct[natom] = structure
+ * of each atom; may be NULL when nstruc = 0).
+ * magace_fit_backward: right after magace_fit_forward on the same arrays, the gradient
+ * grad[ncoef] = d/dc (sum_s gE[s] E_s + sum_j gF[3j..] . F_j) for cotangents gE (may be NULL)
+ * and gF (device arrays), by forward-mode differentiation of the density along gF. */
+int magace_fit_ncoef(void);
+int magace_fit_get_coefficients(double *c, int n);
+int magace_fit_set_coefficients(const double *c, int n);
+int magace_fit_forward(int natom, int nei, const int *offsets, const int *nlist, const double *dlist,
+                       const double *elist, const double *dlist_momi, const double *elist_momi,
+                       const double *dlist_mom, const double *elist_mom, const int *atom_struct,
+                       int nstruc, double *estruct, double *forces, void *stream);
+int magace_fit_backward(int natom, int nei, const int *offsets, const int *nlist, const double *dlist,
+                        const double *elist, const double *dlist_momi, const double *dlist_mom,
+                        const double *elist_mom, const int *atom_struct, const double *gE,
+                        const double *gF, double *grad, void *stream);
+/* Writes the loaded potential with its current coefficients (same header; records par,
+ * core, rad_atomic, rad_magnetic, clu as src/util.F90:1110-1362 writes them). */
+int magace_write_potential(const char *path);
 
 /* Basis introspection: counts of B functions of orders 1..4 of the loaded potential
  * (the numbers src/util.F90:1831,1929,2235,2596 print). */

@@ -26,7 +26,8 @@ EXPORTS = ("acelibini_", "acelib_", "magace_last_error", "magace_error_string", 
            "magace_check_errors", "magace_basis_counts", "magace_enumerate", "magace_model_info",
            "magace_pair_compute", "magace_model_terms", "magace_profile", "magace_profile_read",
            "magace_neighbors_device", "magace_packstructures", "magace_md_create", "magace_md_run",
-           "magace_md_read", "magace_md_destroy")
+           "magace_md_read", "magace_md_destroy", "magace_fit_ncoef", "magace_fit_get_coefficients",
+           "magace_fit_set_coefficients", "magace_fit_forward", "magace_fit_backward", "magace_write_potential")
 
 ERR_CAPACITY = 4
 
@@ -60,6 +61,12 @@ def lib():
     L.magace_model_info.argtypes = [np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), C.c_int]
     L.magace_neighbors_device.argtypes = [C.c_int, _I, _I, _D, C.c_double, C.c_void_p, C.c_void_p,
                                           C.c_longlong] + [C.c_void_p] * 9 + [_IP, C.c_void_p, C.c_void_p]
+    L.magace_fit_get_coefficients.argtypes = [_D, C.c_int]
+    L.magace_fit_set_coefficients.argtypes = [_D, C.c_int]
+    L.magace_fit_forward.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 9 + [C.c_int, C.c_void_p, C.c_void_p,
+                                                                              C.c_void_p]
+    L.magace_fit_backward.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 12
+    L.magace_write_potential.argtypes = [C.c_char_p]
     L.magace_md_create.argtypes = [C.c_int, _D, C.c_double, C.c_double, C.c_double, C.c_int, _D, _D, _D, _D,
                                    C.POINTER(C.c_void_p)]
     L.magace_md_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -286,3 +293,39 @@ def packstructures(natoms, periodic, cells, rcut, pos, mag, capacity=None):
         out["npairs"] = P
         return out
     raise MagaceError("packstructures: capacity retry failed")
+
+
+def fit_ncoef():
+    n = lib().magace_fit_ncoef()
+    if n < 0:
+        _check(-n)
+    return n
+
+
+def fit_get_coefficients():
+    c = np.zeros(fit_ncoef())
+    _check(lib().magace_fit_get_coefficients(c, len(c)))
+    return c
+
+
+def fit_set_coefficients(c):
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    _check(lib().magace_fit_set_coefficients(c, len(c)))
+
+
+def fit_forward(natom, nei, offsets, nlist, dlist, elist, dmomi, emomi, dmom, emom, atom_struct, nstruc,
+                estruct, forces, stream=0):
+    """magace_fit_forward on device pointers."""
+    _check(lib().magace_fit_forward(natom, nei, offsets, nlist, dlist, elist, dmomi, emomi, dmom, emom,
+                                    atom_struct or None, nstruc, estruct or None, forces, stream or None))
+
+
+def fit_backward(natom, nei, offsets, nlist, dlist, elist, dmomi, dmom, emom, atom_struct, gE, gF, grad,
+                 stream=0):
+    """magace_fit_backward on device pointers (gE may be 0)."""
+    _check(lib().magace_fit_backward(natom, nei, offsets, nlist, dlist, elist, dmomi, dmom, emom,
+                                     atom_struct or None, gE or None, gF, grad, stream or None))
+
+
+def write_potential(path):
+    _check(lib().magace_write_potential(os.fsencode(path)))

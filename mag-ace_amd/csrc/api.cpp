@@ -82,6 +82,12 @@ struct Context {
     std::vector<int> h_off;
     hipStream_t stream = nullptr;
     bool uploaded = false;
+    // fitting: adjoint buffers, pair outputs of the fit forward, device coefficient structure
+    FitBuffers fit;
+    size_t fit_cap_pairs = 0;
+    double *f_fl = nullptr, *f_tl = nullptr, *f_t0 = nullptr, *f_eout = nullptr;
+    size_t fit_cap_atoms = 0;
+    int fit_natom = -1;
     // per-kernel timing (magace_profile): 4 events per evaluation
     bool profiling = false;
     std::vector<std::array<hipEvent_t, 4>> events;
@@ -110,7 +116,11 @@ void dfree(T*& p) {
 Context::~Context() {
     dfree(t64.lut_a); dfree(t64.lut_m); dfree(t32.lut_a); dfree(t32.lut_m);
     dfree(crec64); dfree(crec32); dfree(chunks); dfree(fold);
-    dfree(buf.Z); dfree(buf.W); dfree(buf.ehc); dfree(buf.rho1); dfree(buf.dE); dfree(buf.edf); dfree(buf.err);
+    dfree(buf.Z); dfree(buf.W); dfree(buf.ehc); dfree(buf.rho1); dfree(buf.dE); dfree(buf.edf); dfree(buf.rho);
+    dfree(buf.err);
+    dfree(fit.Zdot); dfree(fit.ehc_dot); dfree(fit.alpha); dfree(fit.beta); dfree(fit.M); dfree(fit.idx);
+    dfree(fit.ub_ptr); dfree(fit.ub_t); dfree(fit.ub_u);
+    dfree(f_fl); dfree(f_tl); dfree(f_t0); dfree(f_eout);
     dfree(d_off); dfree(d_dl); dfree(d_el); dfree(d_dm); dfree(d_em); dfree(d_dmi); dfree(d_emi);
     dfree(d_eout); dfree(d_fl); dfree(d_tl); dfree(d_t0);
     for (auto& e : events)
@@ -181,6 +191,51 @@ int build_context(Potential&& pot, std::unique_ptr<Context>& out) {
     return 0;
 }
 
+// 16-byte gradient records (factor indices a | b << 16, c; weight as FP64 bits or FP32 bits)
+int upload_records(Context& c) {
+    const TermList& T = c.model.terms;
+    const size_t nc = T.cw.size();
+    std::vector<uint4> r64(nc), r32(nc);
+    for (size_t i = 0; i < nc; ++i) {
+        const uint16_t* q = &T.cidx[4 * i];
+        const double w = T.cw[i];
+        uint64_t wb;
+        std::memcpy(&wb, &w, 8);
+        const float wf = (float)w;
+        uint32_t fb;
+        std::memcpy(&fb, &wf, 4);
+        r64[i] = make_uint4((uint32_t)q[0] | ((uint32_t)q[1] << 16), q[2], (uint32_t)(wb & 0xffffffffu),
+                            (uint32_t)(wb >> 32));
+        r32[i] = make_uint4((uint32_t)q[0] | ((uint32_t)q[1] << 16), q[2], fb, 0u);
+    }
+    HIPCHK(hipMemcpy(c.crec64, r64.data(), nc * sizeof(uint4), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c.crec32, r32.data(), nc * sizeof(uint4), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// merged weights w_t = sum_k c_k u_{k,t} and record weights after a coefficient change
+int refresh_weights(Context& c) {
+    Potential& pot = c.model.pot;
+    TermList& T = c.model.terms;
+    std::vector<double> cv;
+    cv.insert(cv.end(), pot.c2.begin(), pot.c2.end());
+    cv.insert(cv.end(), pot.c3.begin(), pot.c3.end());
+    cv.insert(cv.end(), pot.c4.begin(), pot.c4.end());
+    for (size_t t = 0; t + 1 < T.ut_ptr.size(); ++t) {
+        double w = 0.0;
+        for (int e = T.ut_ptr[t]; e < T.ut_ptr[t + 1]; ++e) w += cv[T.ut_k[e]] * T.ut_u[e];
+        T.w[t] = w;
+    }
+    for (size_t r = 0; r < T.cw.size(); ++r) T.cw[r] = T.rec_t[r] >= 0 ? T.w[T.rec_t[r]] : 0.0;
+    for (int n = 0; n < 32; ++n) {
+        const double v = n < (int)pot.c1.size() ? pot.c1[n] : 0.0;
+        c.P64.c1[n] = v;
+        c.P32.c1[n] = v;
+    }
+    if (c.uploaded && !T.cw.empty()) return upload_records(c);
+    return 0;
+}
+
 // device tables are uploaded on first use so the host model can be built without a GPU
 int upload(Context& c) {
     if (c.uploaded) return 0;
@@ -200,21 +255,7 @@ int upload(Context& c) {
     HIPCHK(hipMalloc((void**)&c.fold, M.terms.fold.size() * sizeof(uint32_t) + 64));
     HIPCHK(hipMemcpy(c.fold, M.terms.fold.data(), M.terms.fold.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     if (nc) {
-        std::vector<uint4> r64(nc), r32(nc);
-        for (size_t i = 0; i < nc; ++i) {
-            const uint16_t* q = &M.terms.cidx[4 * i];
-            const double w = M.terms.cw[i];
-            uint64_t wb;
-            std::memcpy(&wb, &w, 8);
-            const float wf = (float)w;
-            uint32_t fb;
-            std::memcpy(&fb, &wf, 4);
-            r64[i] = make_uint4((uint32_t)q[0] | ((uint32_t)q[1] << 16), q[2], (uint32_t)(wb & 0xffffffffu),
-                                (uint32_t)(wb >> 32));
-            r32[i] = make_uint4((uint32_t)q[0] | ((uint32_t)q[1] << 16), q[2], fb, 0u);
-        }
-        HIPCHK(hipMemcpy(c.crec64, r64.data(), nc * sizeof(uint4), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c.crec32, r32.data(), nc * sizeof(uint4), hipMemcpyHostToDevice));
+        if (int rc = upload_records(c)) return rc;
         HIPCHK(hipMemcpy(c.chunks, M.terms.chunks.data(), nch * sizeof(int2), hipMemcpyHostToDevice));
     }
     HIPCHK(hipMalloc((void**)&c.buf.err, sizeof(int)));
@@ -240,6 +281,8 @@ int ensure_atoms(Context& c, size_t natom) {
     HIPCHK(hipMalloc((void**)&c.buf.rho1, cap * sizeof(double)));
     HIPCHK(hipMalloc((void**)&c.buf.dE, cap * sizeof(double)));
     HIPCHK(hipMalloc((void**)&c.buf.edf, cap * sizeof(double)));
+    dfree(c.buf.rho);
+    HIPCHK(hipMalloc((void**)&c.buf.rho, cap * sizeof(double)));
     c.buf.cap_atoms = cap;
     return 0;
 }
@@ -375,7 +418,7 @@ int magace_compute_device(int natom, int nei, const int* offsets, const double* 
         return MAGACE_ERR_INPUT;
     }
     if (int rc = upload(*g_ctx)) return rc;
-    hipStream_t s = stream ? (hipStream_t)stream : g_ctx->stream;
+    hipStream_t s = (hipStream_t)stream;  // NULL: the default (null) stream
     if (int rc = run_device(*g_ctx, natom, offsets, dlist, elist, dlist_momi, elist_momi, dlist_mom, elist_mom,
                             eout, flist, tlist, tlist0, s))
         return rc;
@@ -388,7 +431,7 @@ int magace_check_errors(void* stream) {
         return MAGACE_ERR_STATE;
     }
     if (int rc = upload(*g_ctx)) return rc;
-    return check_kernel_errors(*g_ctx, stream ? (hipStream_t)stream : g_ctx->stream);
+    return check_kernel_errors(*g_ctx, (hipStream_t)stream);
 }
 
 void acelib_(int* natom, int* nall, int* nei, int* nlist, int* occ, int* occlist, int* nstart,
@@ -560,6 +603,149 @@ int magace_model_terms(uint16_t* idx, double* w, int cap) {
         std::memcpy(w, T.w.data(), (size_t)n * sizeof(double));
     }
     return n;
+}
+
+int magace_fit_ncoef(void) {
+    if (!g_ctx) {
+        set_error(MAGACE_ERR_STATE, "library not initialised");
+        return -MAGACE_ERR_STATE;
+    }
+    const Potential& p = g_ctx->model.pot;
+    return (int)(p.c1.size() + p.c2.size() + p.c3.size() + p.c4.size());
+}
+
+int magace_fit_get_coefficients(double* c, int n) {
+    clear_error();
+    const int nc = magace_fit_ncoef();
+    if (nc < 0) return -nc;
+    if (!c || n != nc) {
+        set_error(MAGACE_ERR_INPUT, "coefficient count mismatch (magace_fit_ncoef)");
+        return MAGACE_ERR_INPUT;
+    }
+    const Potential& p = g_ctx->model.pot;
+    size_t o = 0;
+    for (const auto* v : {&p.c1, &p.c2, &p.c3, &p.c4})
+        for (double x : *v) c[o++] = x;
+    return 0;
+}
+
+int magace_fit_set_coefficients(const double* c, int n) {
+    clear_error();
+    const int nc = magace_fit_ncoef();
+    if (nc < 0) return -nc;
+    if (!c || n != nc) {
+        set_error(MAGACE_ERR_INPUT, "coefficient count mismatch (magace_fit_ncoef)");
+        return MAGACE_ERR_INPUT;
+    }
+    Potential& p = g_ctx->model.pot;
+    size_t o = 0;
+    for (auto* v : {&p.c1, &p.c2, &p.c3, &p.c4})
+        for (double& x : *v) x = c[o++];
+    g_ctx->fit_natom = -1;  // Z and W of the last forward no longer match
+    return refresh_weights(*g_ctx);
+}
+
+int magace_write_potential(const char* path) {
+    clear_error();
+    if (!g_ctx) {
+        set_error(MAGACE_ERR_STATE, "library not initialised");
+        return MAGACE_ERR_STATE;
+    }
+    std::string err;
+    if (!path || !write_potential(path, g_ctx->model.pot, err)) {
+        set_error(MAGACE_ERR_INPUT, path ? err : "null path");
+        return MAGACE_ERR_INPUT;
+    }
+    return 0;
+}
+
+int magace_fit_forward(int natom, int nei, const int* offsets, const int* nlist, const double* dlist,
+                       const double* elist, const double* dlist_momi, const double* elist_momi,
+                       const double* dlist_mom, const double* elist_mom, const int* atom_struct, int nstruc,
+                       double* estruct, double* forces, void* stream) {
+    clear_error();
+    if (!g_ctx) {
+        set_error(MAGACE_ERR_STATE, "library not initialised");
+        return MAGACE_ERR_STATE;
+    }
+    Context& c = *g_ctx;
+    if (natom < 0 || nei < 0 || nstruc < 0 || ((nstruc > 0 || estruct) && !atom_struct) || !forces) {
+        set_error(MAGACE_ERR_INPUT, "magace_fit_forward: invalid sizes");
+        return MAGACE_ERR_INPUT;
+    }
+    if (int rc = upload(c)) return rc;
+    hipStream_t s = (hipStream_t)stream;  // NULL: the default (null) stream
+    if ((size_t)nei > c.fit_cap_pairs) {
+        dfree(c.f_fl); dfree(c.f_tl);
+        const size_t cap = (size_t)nei + nei / 4 + 64;
+        HIPCHK(hipMalloc((void**)&c.f_fl, 3 * cap * sizeof(double)));
+        HIPCHK(hipMalloc((void**)&c.f_tl, 3 * cap * sizeof(double)));
+        c.fit_cap_pairs = cap;
+    }
+    if ((size_t)natom > c.fit_cap_atoms) {
+        dfree(c.f_t0); dfree(c.f_eout);
+        const size_t cap = (size_t)natom + natom / 4 + 16;
+        HIPCHK(hipMalloc((void**)&c.f_t0, 3 * cap * sizeof(double)));
+        HIPCHK(hipMalloc((void**)&c.f_eout, cap * sizeof(double)));
+        c.fit_cap_atoms = cap;
+    }
+    const int saved = c.precision;
+    c.precision = MAGACE_FP64;  // the adjoint runs on the FP64 Z and W of this evaluation
+    const int rc = run_device(c, natom, offsets, dlist, elist, dlist_momi, elist_momi, dlist_mom, elist_mom,
+                              c.f_eout, c.f_fl, c.f_tl, c.f_t0, s);
+    c.precision = saved;
+    if (rc) return rc;
+    HIPCHK(fit_assemble(natom, offsets, nlist, c.f_fl, c.f_eout, atom_struct, nstruc, forces, estruct, s));
+    c.fit_natom = natom;
+    return 0;
+}
+
+int magace_fit_backward(int natom, int nei, const int* offsets, const int* nlist, const double* dlist,
+                        const double* elist, const double* dlist_momi, const double* dlist_mom,
+                        const double* elist_mom, const int* atom_struct, const double* gE, const double* gF,
+                        double* grad, void* stream) {
+    clear_error();
+    if (!g_ctx) {
+        set_error(MAGACE_ERR_STATE, "library not initialised");
+        return MAGACE_ERR_STATE;
+    }
+    Context& c = *g_ctx;
+    if (natom != c.fit_natom) {
+        set_error(MAGACE_ERR_STATE, "magace_fit_backward must follow magace_fit_forward on the same atoms "
+                                    "and coefficients");
+        return MAGACE_ERR_STATE;
+    }
+    if (nei < 0 || !gF || !grad || (gE && !atom_struct)) {
+        set_error(MAGACE_ERR_INPUT, "magace_fit_backward: invalid arguments");
+        return MAGACE_ERR_INPUT;
+    }
+    hipStream_t s = (hipStream_t)stream;  // NULL: the default (null) stream
+    const TermList& T = c.model.terms;
+    FitBuffers& F = c.fit;
+    if (!F.idx) {
+        HIPCHK(hipMalloc((void**)&F.idx, T.idx.size() * sizeof(uint16_t) + 64));
+        HIPCHK(hipMemcpy(F.idx, T.idx.data(), T.idx.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc((void**)&F.ub_ptr, T.ub_ptr.size() * sizeof(int32_t) + 64));
+        HIPCHK(hipMalloc((void**)&F.ub_t, T.ub_t.size() * sizeof(int32_t) + 64));
+        HIPCHK(hipMalloc((void**)&F.ub_u, T.ub_u.size() * sizeof(double) + 64));
+        HIPCHK(hipMemcpy(F.ub_ptr, T.ub_ptr.data(), T.ub_ptr.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(F.ub_t, T.ub_t.data(), T.ub_t.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(F.ub_u, T.ub_u.data(), T.ub_u.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc((void**)&F.M, 2 * T.w.size() * sizeof(double) + 64));
+    }
+    if ((size_t)natom > F.cap_atoms) {
+        dfree(F.Zdot); dfree(F.ehc_dot); dfree(F.alpha); dfree(F.beta);
+        const size_t cap = (size_t)natom + natom / 4 + 16;
+        HIPCHK(hipMalloc(&F.Zdot, cap * c.model.z.nz * 2 * sizeof(double)));
+        HIPCHK(hipMalloc((void**)&F.ehc_dot, cap * sizeof(double)));
+        HIPCHK(hipMalloc((void**)&F.alpha, cap * sizeof(double)));
+        HIPCHK(hipMalloc((void**)&F.beta, cap * sizeof(double)));
+        F.cap_atoms = cap;
+    }
+    const int nb234 = (int)T.ub_ptr.size() - 1;
+    HIPCHK(fit_backward(c.P64, c.shape_class, natom, offsets, nlist, gF, dlist, elist, dlist_momi, dlist_mom,
+                        elist_mom, gE ? atom_struct : nullptr, gE, c.buf, F, (int)T.w.size(), nb234, grad, s));
+    return 0;
 }
 
 int magace_model_info(long long* info, int n) {

@@ -224,28 +224,54 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
     const Header& h = pot.h;
     const Basis& B = pot.basis;
     const int one = z.nz;  // padding factor Z[nz] = 1
-    std::unordered_map<uint64_t, double> acc;
-    acc.reserve(1 << 16);
     TermList T;
-    auto emit = [&](double w, int a, int b, int c, int d) {
-        if (w == 0.0) return;
-        acc[pack4({a, b, c, d})] += w;
+    // Structure first, coefficients second: every B function k of order >= 2 is expanded
+    // into m-combinations u * Z[a]Z[b]Z[c]Z[d] (CG products, no coefficient), so that the
+    // monomial set does not depend on c and the merged weights w_t = sum_k c_k u_{k,t} can be
+    // recomputed for new coefficients (fitting).
+    const Conj cj = conj_map(h, z);
+    std::unordered_map<uint64_t, double> one_b;  // canonical key -> u of the current B function
+    std::vector<std::vector<std::pair<uint64_t, double>>> per_b;
+    auto emit = [&](double u, int a, int b, int c, int d) {
+        if (u == 0.0) return;
         ++T.raw;
+        // fold complex-conjugate partners: Re(u P) + Re(u' s conj P) = (u + s u') Re P
+        const std::array<int, 4> v = {a, b, c, d};
+        std::array<int, 4> vb;
+        int sg = 1;
+        for (int i = 0; i < 4; ++i) {
+            vb[i] = cj.partner[v[i]];
+            sg *= cj.sign[v[i]];
+        }
+        const uint64_t key = pack4(v), kb = pack4(vb);
+        if (kb == key) {
+            // a self-conjugate monomial with s = -1 is purely imaginary: Re P == 0
+            if (sg == 1) one_b[key] += u;
+        } else if (kb < key) {
+            one_b[kb] += sg * u;
+        } else {
+            one_b[key] += u;
+        }
+    };
+    auto close_b = [&]() {
+        std::vector<std::pair<uint64_t, double>> v(one_b.begin(), one_b.end());
+        std::sort(v.begin(), v.end());
+        per_b.push_back(std::move(v));
+        one_b.clear();
     };
     // order 2: B2 = Re sum_m C(l l 0; m -m) A0[n0'][l m] A2[n1][n1'][l -m]  (b_basis.F90:95-124)
     for (size_t k = 0; k < B.b2.size(); ++k) {
         const auto& b = B.b2[k];
-        const double c = pot.c2[k];
         const int n0 = b[0] - 1, n1p = b[1] - 1, n1 = b[2] - 1, l = b[3];
         for (int m = -l; m <= l; ++m) {
             ++T.n2;
-            emit(c * clebsch_gordan(l, l, 0, m, -m, 0), z.a0(n0, l, m), z.a2(n1, n1p, l, -m), one, one);
+            emit(clebsch_gordan(l, l, 0, m, -m, 0), z.a0(n0, l, m), z.a2(n1, n1p, l, -m), one, one);
         }
+        close_b();
     }
     // order 3 (b_basis.F90:146-205, couplings util.F90:4292-4333)
     for (size_t k = 0; k < B.b3.size(); ++k) {
         const auto& b = B.b3[k];
-        const double c = pot.c3[k];
         const int n1 = b[0] - 1, n2 = b[1] - 1, n0p = b[2] - 1, n1p = b[3] - 1, n2p = b[4] - 1;
         const int l1 = b[5], l0p = b[6], l1p = b[7], l2p = b[8];
         for (int m1 = -l1; m1 <= l1; ++m1) {
@@ -257,15 +283,15 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
                     ++T.n3;
                     const double cm = clebsch_gordan(l0p, l1p, l2p, q0, q1, -q2) *
                                       clebsch_gordan(l2p, l2p, 0, q2, -q2, 0);
-                    emit(c * ca * cm, z.a0(n0p, l0p, q0), z.a(n1, l1, m1, n1p, l1p, q1),
+                    emit(ca * cm, z.a0(n0p, l0p, q0), z.a(n1, l1, m1, n1p, l1p, q1),
                          z.a(n2, l1, -m1, n2p, l2p, q2), one);
                 }
         }
+        close_b();
     }
     // order 4 (b_basis.F90:229-301, couplings util.F90:4343-4388)
     for (size_t k = 0; k < B.b4.size(); ++k) {
         const auto& b = B.b4[k];
-        const double c = pot.c4[k];
         const int n1 = b[0] - 1, n2 = b[1] - 1, n3 = b[2] - 1;
         const int p0 = b[3] - 1, p1 = b[4] - 1, p2 = b[5] - 1, p3 = b[6] - 1;
         const int l1 = b[7], l2 = b[8], l3 = b[9], k0 = b[10], k1 = b[11], k2 = b[12], k3 = b[13], L = b[14];
@@ -285,42 +311,23 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
                             const double cm = clebsch_gordan(k0, k1, L, q0, q1, M) *
                                               clebsch_gordan(k2, k3, L, q2, q3, -M) *
                                               clebsch_gordan(L, L, 0, M, -M, 0);
-                            emit(c * ca * cm, z.a0(p0, k0, q0), z.a(n1, l1, m1, p1, k1, q1),
+                            emit(ca * cm, z.a0(p0, k0, q0), z.a(n1, l1, m1, p1, k1, q1),
                                  z.a(n2, l2, m2, p2, k2, q2), z.a(n3, l3, m3, p3, k3, q3));
                         }
                     }
             }
+        close_b();
     }
-    // fold complex-conjugate partners: Re(w P) + Re(w' s conj P) = (w + s w') Re P
-    const Conj cj = conj_map(h, z);
-    std::map<uint64_t, double> folded;
-    for (const auto& kv : acc) {
-        const uint64_t key = kv.first;
-        std::array<int, 4> v = {(int)(key >> 48), (int)((key >> 32) & 0xffff), (int)((key >> 16) & 0xffff),
-                                (int)(key & 0xffff)};
-        int s = 1;
-        std::array<int, 4> vb;
-        for (int i = 0; i < 4; ++i) {
-            vb[i] = cj.partner[v[i]];
-            s *= cj.sign[v[i]];
-        }
-        const uint64_t kb = pack4(vb);
-        if (kb == key) {
-            // a self-conjugate monomial with s = -1 is purely imaginary: Re P == 0
-            if (s == 1) folded[key] += kv.second;
-        } else if (kb < key) {
-            folded[kb] += s * kv.second;
-        } else {
-            folded[key] += kv.second;
-        }
-    }
+    // the monomial set: every key with a nonzero coefficient in some B function
+    std::map<uint64_t, int> keys;
+    for (const auto& v : per_b)
+        for (const auto& e : v)
+            if (e.second != 0.0) keys.emplace(e.first, 0);
     // device order: slot 0 = the A0 factor (every monomial has exactly one), slots 1..3 = the
-    // other factors ascending, padding last; sorted lexicographically so that a thread walking
-    // a contiguous run sees long runs of equal slot-0 and slot-1 targets
-    std::vector<std::pair<std::array<int, 4>, double>> rows;
-    rows.reserve(folded.size());
-    for (const auto& kv : folded) {
-        if (kv.second == 0.0) continue;
+    // other factors ascending, padding last; sorted lexicographically
+    std::vector<std::pair<std::array<int, 4>, uint64_t>> rows;
+    rows.reserve(keys.size());
+    for (const auto& kv : keys) {
         const uint64_t key = kv.first;
         std::array<int, 4> v = {(int)(key >> 48), (int)((key >> 32) & 0xffff), (int)((key >> 16) & 0xffff),
                                 (int)(key & 0xffff)};
@@ -332,24 +339,53 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
         }
         o[0] = a0;
         std::sort(o.begin() + 1, o.end());
-        rows.emplace_back(o, kv.second);
+        rows.emplace_back(o, key);
     }
     std::sort(rows.begin(), rows.end());
-    for (const auto& r : rows) {
-        for (int i = 0; i < 4; ++i) T.idx.push_back((uint16_t)r.first[i]);
-        T.w.push_back(r.second);
+    for (size_t t = 0; t < rows.size(); ++t) keys[rows[t].second] = (int)t;
+    // U (by B function and transposed by monomial), then w_t = sum_k c_k u_{k,t}
+    const size_t nt = rows.size();
+    T.ub_ptr.assign(1, 0);
+    std::vector<std::vector<std::pair<int, double>>> byt(nt);
+    std::vector<double> cvec;
+    for (double c : pot.c2) cvec.push_back(c);
+    for (double c : pot.c3) cvec.push_back(c);
+    for (double c : pot.c4) cvec.push_back(c);
+    for (size_t k = 0; k < per_b.size(); ++k) {
+        for (const auto& e : per_b[k]) {
+            if (e.second == 0.0) continue;
+            const int t = keys[e.first];
+            T.ub_t.push_back(t);
+            T.ub_u.push_back(e.second);
+            byt[t].emplace_back((int)k, e.second);
+        }
+        T.ub_ptr.push_back((int32_t)T.ub_t.size());
+    }
+    T.ut_ptr.assign(1, 0);
+    T.w.assign(nt, 0.0);
+    for (size_t t = 0; t < nt; ++t) {
+        double w = 0.0;
+        for (const auto& e : byt[t]) {
+            T.ut_k.push_back(e.first);
+            T.ut_u.push_back(e.second);
+            w += cvec[e.first] * e.second;
+        }
+        T.ut_ptr.push_back((int32_t)T.ut_k.size());
+        T.w[t] = w;
+        for (int i = 0; i < 4; ++i) T.idx.push_back((uint16_t)rows[t].first[i]);
     }
     // transposed gradient: contributions grouped by target
-    std::vector<std::vector<std::pair<std::array<int, 3>, double>>> by(z.nz);
-    for (const auto& r : rows)
+    // (entries: the three other factors, the monomial index)
+    std::vector<std::vector<std::pair<std::array<int, 3>, int>>> by(z.nz);
+    for (size_t t = 0; t < nt; ++t)
         for (int s = 0; s < 4; ++s) {
-            const int x = r.first[s];
+            const int x = rows[t].first[s];
             if (x == one) continue;
             std::array<int, 3> o;
             int k = 0;
             for (int q = 0; q < 4; ++q)
-                if (q != s) o[k++] = r.first[q];
-            by[x].emplace_back(o, r.second);
+                if (q != s) o[k++] = rows[t].first[q];
+            by[x].emplace_back(o, (int)t);
         }
     // Chunks of <= kChunk contributions per row, ordered round robin across rows so that
     // neighbouring lanes flush to different targets.  A wave processes 64 consecutive
@@ -401,6 +437,7 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
         const size_t base = T.cw.size();
         T.cidx.resize((base + (size_t)len * 64) * 4, (uint16_t)one);
         T.cw.resize(base + (size_t)len * 64, 0.0);
+        T.rec_t.resize(base + (size_t)len * 64, -1);
         const int nl = (int)(g1 - g0);
         std::vector<std::vector<int>> left(nl);
         for (int L = 0; L < nl; ++L)
@@ -430,7 +467,8 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
                 T.cidx[4 * r] = (uint16_t)e.first[0];
                 T.cidx[4 * r + 1] = (uint16_t)e.first[1];
                 T.cidx[4 * r + 2] = (uint16_t)e.first[2];
-                T.cw[r] = e.second;
+                T.cw[r] = T.w[e.second];
+                T.rec_t[r] = e.second;
                 rem.erase(rem.begin() + best);
             }
         }

@@ -467,7 +467,7 @@ __global__ void __launch_bounds__(256) k_adjoint(DevParams<T> P, const cplx<T>* 
                                                 const double* __restrict__ ehc,
                                                 const double* __restrict__ rho1,
                                                 double* __restrict__ eout, double* __restrict__ dEout,
-                                                double* __restrict__ edfout) {
+                                                double* __restrict__ edfout, double* __restrict__ rho_out) {
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<T>* sZ = (cplx<T>*)smem;           // [nz+1], sZ[nz] = 1
     cplx<T>* sW = sZ + (P.nz + 1);          // [nz+1]
@@ -518,6 +518,7 @@ __global__ void __launch_bounds__(256) k_adjoint(DevParams<T> P, const cplx<T>* 
     if (tid == 0) {
         // E_i = f_in(e_hc) F(rho) + e_hc (src/forces.F90:1104-1128)
         const double r = tot + rho1[atom];
+        rho_out[atom] = r;
         double F, dF;
         embed(P.nfemb, r, P.p1a, P.p1b, F, dF);
         const double e = ehc[atom], elow = P.ecut - P.decut;
@@ -874,6 +875,451 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
     }
 }
 
+// ============================================================================ fitting
+// Gradient of L = sum_s gE_s E_s + sum_j gF_j . F_j with respect to the expansion
+// coefficients (the contraction of src/forces.F90:93-141 run backwards): with
+// E_i = fc(e_hc) F(rho_i) + e_hc and rho_i = sum_k c_k B_k,i,
+//   dL/dc_k = sum_i [alpha_i B_k,i + beta_i D_u B_k,i],  u = gF (a displacement field),
+//   alpha_i = gE_s(i) fc F' - fc F'' D_u rho_i - F' D_u fc,   beta_i = -fc F'
+// (F_j = -dE/dx_j, so sum_j gF_j . dF_j/dc = -D_u dE/dc).  B_k = Re sum_t u_{k,t} P_t with
+// P_t = prod Z[idx_t]; D_u P_t follows from Zdot = D_u Z, the forward-mode derivative of the
+// density along u.  FP64 only.
+
+// Y_lm (m >= 0) and D_u Y_lm = (grad Y . du) / r for a bond of unit vector (x, y, z)
+template <int L>
+__device__ __forceinline__ void ylm_half_jvp(double x, double y, double z, int lmax, double dux, double duy,
+                                             double duz, double invr, cplx<double>* out, cplx<double>* dout) {
+    double P[(L + 1) * (L + 2) / 2], dP[(L + 1) * (L + 2) / 2];
+    legendre<double, L>(z, P, dP);
+    cplx<double> ph[L + 1];
+    ph[0] = {1, 0};
+#pragma unroll
+    for (int m = 1; m <= L; ++m) ph[m] = cmul(ph[m - 1], cplx<double>{x, y});
+#pragma unroll
+    for (int l = 0; l <= L; ++l)
+#pragma unroll
+        for (int m = 0; m <= l; ++m) {
+            if (l > lmax) continue;
+            const int k = l * (l + 1) / 2 + m;
+            cplx<double> Y, g[3];
+            ylm_grad<double>(m, P[k], dP[k], ph[m], ph[m > 0 ? m - 1 : 0], x, y, z, Y, g);
+            out[k] = Y;
+            dout[k] = {(g[0].re * dux + g[1].re * duy + g[2].re * duz) * invr,
+                       (g[0].im * dux + g[1].im * duy + g[2].im * duz) * invr};
+        }
+}
+
+// Zdot = D_u Z and D_u e_hc, the pair vector of pair p = (i -> j) moving by u_i - u_j.
+// Same task layout as k_density; A0 does not depend on positions.
+template <int L, int CH, int NB, int NM3X, int NB2, int NR2X>
+__global__ void __launch_bounds__(256) k_density_jvp(DevParams<double> P, const int* __restrict__ off,
+                                                    const int* __restrict__ nlist, const double* __restrict__ du,
+                                                    const double* __restrict__ dlist,
+                                                    const double* __restrict__ elist,
+                                                    const double* __restrict__ dmom,
+                                                    const double* __restrict__ emom,
+                                                    cplx<double>* __restrict__ Zdot,
+                                                    double* __restrict__ ehc_dot, int* err) {
+    using T = double;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int atom = blockIdx.x;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int nfa = P.nr3a * (P.l3a + 1), nfm = P.nr2m * (P.l2m + 1);
+    cplx<T>* sY = (cplx<T>*)smem;        // [CH][lha]
+    cplx<T>* sYd = sY + CH * P.lha;      // [CH][lha]
+    cplx<T>* sYm = sYd + CH * P.lha;     // [CH][lhm]
+    cplx<T>* sH = sYm + CH * P.lhm;      // [CH][nm3]
+    T* sR = (T*)(sH + CH * P.nm3);       // [CH][nfa]
+    T* sRd = sR + CH * nfa;              // [CH][nfa]
+    T* sGd = sRd + CH * nfa;             // [CH][nb_a]
+    T* sM = sGd + CH * P.nb_a;           // [CH][nfm]
+    T* sCrd = sM + CH * nfm;             // [CH]
+    double* sE = sCrd + CH;              // [1]
+
+    const int p0 = off[atom], nn = off[atom + 1] - p0;
+    Zdot += (size_t)atom * P.nz;
+    const double uix = du[3 * (size_t)atom], uiy = du[3 * (size_t)atom + 1], uiz = du[3 * (size_t)atom + 2];
+    for (int t = tid; t < P.nr2m * P.lsm; t += nthr) Zdot[P.oA0 + t] = {0, 0};
+    if (tid == 0) *sE = 0.0;
+
+    const int nblkA = (P.nr3a + NB - 1) / NB, nblkA2 = (P.nb_a + NB2 - 1) / NB2;
+    const int nTA = P.lsa * nblkA, nTA2 = P.lsm * nblkA2;
+    for (int round = 0; round * nthr < nTA + nTA2; ++round) {
+        const int task = round * nthr + tid;
+        cplx<T> acc[NB > NB2 ? NB : NB2][NM3X > NR2X ? NM3X : NR2X];
+#pragma unroll
+        for (int a = 0; a < (NB > NB2 ? NB : NB2); ++a)
+#pragma unroll
+            for (int b = 0; b < (NM3X > NR2X ? NM3X : NR2X); ++b) acc[a][b] = {0, 0};
+        const bool isA = task < nTA, isA2 = !isA && task < nTA + nTA2;
+        int lm = 0, l = 0, m = 0, n0 = 0;
+        if (isA) { lm = task % P.lsa; n0 = (task / P.lsa) * NB; }
+        if (isA2) { lm = (task - nTA) % P.lsm; n0 = ((task - nTA) / P.lsm) * NB2; }
+        while ((l + 1) * (l + 1) <= lm) ++l;
+        m = lm - l * l - l;
+        for (int c0 = 0; c0 < nn; c0 += CH) {
+            const int nj = nn - c0 < CH ? nn - c0 : CH;
+            __syncthreads();
+            if (tid < CH) {  // atomic side: values and their derivatives along u
+                const int j = tid;
+                if (j < nj) {
+                    const int p = p0 + c0 + j;
+                    const int jn = nlist[p];
+                    const double dux = uix - du[3 * (size_t)jn], duy = uiy - du[3 * (size_t)jn + 1],
+                                 duz = uiz - du[3 * (size_t)jn + 2];
+                    const double ex = elist[3 * p], ey = elist[3 * p + 1], ez = elist[3 * p + 2];
+                    const double rdot = ex * dux + ey * duy + ez * duz;
+                    T w;
+                    const T* row = atomic_row(P, dlist[p], w, err);
+                    for (int k = 0; k < P.nb_a; ++k) {
+                        T v = 0, dv = 0;
+                        if (row) horner(row + 4 * k, w, (T)P.rs_a, v, dv);
+                        sGd[j * P.nb_a + k] = dv * rdot;
+                    }
+                    for (int k = 0; k < nfa; ++k) {
+                        T v = 0, dv = 0;
+                        if (row) horner(row + 4 * (P.nb_a + k), w, (T)P.rs_a, v, dv);
+                        sR[j * nfa + k] = v;
+                        sRd[j * nfa + k] = dv * rdot;
+                    }
+                    T cr = 0, dcr = 0;
+                    if (row) horner(row + 4 * (P.nb_a + nfa), w, (T)P.rs_a, cr, dcr);
+                    sCrd[j] = dcr * rdot;
+                    ylm_half_jvp<L>(ex, ey, ez, P.l3a, dux, duy, duz, 1.0 / dlist[p], sY + j * P.lha,
+                                    sYd + j * P.lha);
+                } else {
+                    sCrd[j] = 0;
+                }
+            } else if (tid < 2 * CH) {  // magnetic side (does not move with positions)
+                const int j = tid - CH;
+                if (j < nj) {
+                    const int p = p0 + c0 + j;
+                    T w;
+                    bool unit;
+                    const T* row = magnetic_row(P, dmom[p], w, unit, err);
+                    for (int k = 0; k < nfm; ++k) {
+                        T v = unit ? (T)1 : (T)0, dv;
+                        if (row) horner(row + 4 * (P.nb_m + k), w, (T)P.rs_m, v, dv);
+                        sM[j * nfm + k] = v;
+                    }
+                    cplx<T>* yh = sYm + j * P.lhm;
+                    ylm_half<T, L>(emom[3 * p], emom[3 * p + 1], emom[3 * p + 2], P.l2m, yh);
+                    for (int q = 0; q < P.nm3; ++q) {
+                        const int np = q / P.lsm3, lmp = q % P.lsm3;
+                        int lp = 0;
+                        while ((lp + 1) * (lp + 1) <= lmp) ++lp;
+                        const int mp = lmp - lp * lp - lp;
+                        const cplx<T> y = yfull(yh, lp, mp);
+                        const T mv = sM[j * nfm + lp * P.nr2m + np];
+                        sH[j * P.nm3 + q] = {mv * y.re, mv * y.im};
+                    }
+                }
+            }
+            __syncthreads();
+            if (round == 0 && tid == 0) {
+                double sum = 0.0;
+                for (int j = 0; j < nj; ++j) sum += sCrd[j];
+                *sE += sum;
+            }
+            if (isA) {
+                for (int j = 0; j < nj; ++j) {
+                    const cplx<T> y = yfull(sY + j * P.lha, l, m), yd = yfull(sYd + j * P.lha, l, m);
+                    cplx<T> h[NM3X];
+#pragma unroll
+                    for (int q = 0; q < NM3X; ++q) h[q] = q < P.nm3 ? sH[j * P.nm3 + q] : cplx<T>{0, 0};
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) {
+                        const int n = n0 + b;
+                        const T r = n < P.nr3a ? sR[j * nfa + l * P.nr3a + n] : (T)0;
+                        const T rd = n < P.nr3a ? sRd[j * nfa + l * P.nr3a + n] : (T)0;
+                        const cplx<T> ry = {rd * y.re + r * yd.re, rd * y.im + r * yd.im};
+#pragma unroll
+                        for (int q = 0; q < NM3X; ++q) cfma(acc[b][q], ry, h[q]);
+                    }
+                }
+            } else if (isA2) {
+                for (int j = 0; j < nj; ++j) {
+                    const cplx<T> y = yfull(sYm + j * P.lhm, l, m);
+                    cplx<T> my[NR2X];
+#pragma unroll
+                    for (int q = 0; q < NR2X; ++q) {
+                        const T mv = q < P.nr2m ? sM[j * nfm + l * P.nr2m + q] : (T)0;
+                        my[q] = {mv * y.re, mv * y.im};
+                    }
+#pragma unroll
+                    for (int b = 0; b < NB2; ++b) {
+                        const int n = n0 + b;
+                        const T gd = n < P.nb_a ? sGd[j * P.nb_a + n] : (T)0;
+#pragma unroll
+                        for (int q = 0; q < NR2X; ++q) {
+                            acc[b][q].re = fma(gd, my[q].re, acc[b][q].re);
+                            acc[b][q].im = fma(gd, my[q].im, acc[b][q].im);
+                        }
+                    }
+                }
+            }
+        }
+        if (isA) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int n = n0 + b;
+                if (n < P.nr3a)
+#pragma unroll
+                    for (int q = 0; q < NM3X; ++q)
+                        if (q < P.nm3) Zdot[(n * P.lsa + lm) * P.nm3 + q] = acc[b][q];
+            }
+        } else if (isA2) {
+#pragma unroll
+            for (int b = 0; b < NB2; ++b) {
+                const int n = n0 + b;
+                if (n < P.nb_a)
+#pragma unroll
+                    for (int q = 0; q < NR2X; ++q)
+                        if (q < P.nr2m) Zdot[P.oA2 + (n * P.nr2m + q) * P.lsm + lm] = acc[b][q];
+            }
+        }
+    }
+    __syncthreads();
+    if (tid == 0) ehc_dot[atom] = *sE;
+}
+
+size_t density_jvp_lds_bytes(const DevParams<double>& Pd, int CH) {
+    const int nfa = Pd.nr3a * (Pd.l3a + 1), nfm = Pd.nr2m * (Pd.l2m + 1);
+    const size_t n = 16 * ((size_t)CH * 2 * Pd.lha + (size_t)CH * Pd.lhm + (size_t)CH * Pd.nm3) +
+                     8 * ((size_t)CH * 2 * nfa + (size_t)CH * Pd.nb_a + (size_t)CH * nfm + CH + 1);
+    return ((n + 15) & ~(size_t)15) + 16;
+}
+
+// F, F' and F'' of the embedding (src/forces.F90:1134-1233)
+__device__ inline void embed2(int nfemb, double rho, double wpre, double mexp, double& F, double& dF,
+                              double& d2F) {
+    embed(nfemb, rho, wpre, mexp, F, dF);
+    d2F = 0.0;
+    if (nfemb == 0) return;
+    const double w = 10.0, eps = 1e-10;
+    const double s = nfemb == 2 ? 1.0 + rho : rho;
+    if (!(fabs(s) > eps)) return;
+    const double a = fabs(s), sg = s >= 0.0 ? 1.0 : -1.0;
+    const double y0 = w * s * s;
+    const double g = y0 > 30.0 ? 0.0 : exp(-y0);
+    const double g1 = -2.0 * w * s * g, g2 = -2.0 * w * g + 4.0 * w * w * s * s * g;
+    const double q = (1.0 - g) * a, q1 = -g1 * a + (1.0 - g) * sg, q2 = -g2 * a - 2.0 * g1 * sg;
+    double y1pp = 0.0;
+    if (q > 0.0) y1pp = mexp * (mexp - 1.0) * pow(q, mexp - 2.0) * q1 * q1 + mexp * pow(q, mexp - 1.0) * q2;
+    const double y2pp = mexp * (g2 * a + 2.0 * g1 * sg);
+    d2F = wpre * sg * (y1pp + y2pp);
+}
+
+__device__ inline void inner_cutoff(double e, double ecut, double decut, double& fc, double& dfc) {
+    const double elow = ecut - decut;
+    fc = 1.0;
+    dfc = 0.0;
+    if (e >= ecut) {
+        fc = 0.0;
+    } else if (e > elow) {
+        const double pi = 3.14159265358979323846;
+        fc = 0.5 * (1.0 + cos(pi * (e - elow) / decut));
+        dfc = -0.5 * sin(pi * (e - elow) / decut) * pi / decut;
+    }
+}
+
+// alpha_i, beta_i (one block per atom; D_u rho = Re sum_x W_x Zdot_x / (fc F'), W = fc F' drho/dZ
+// folded onto representatives, which the conjugate symmetry of Zdot allows)
+__global__ void __launch_bounds__(256) k_fit_atom(DevParams<double> P, const cplx<double>* __restrict__ W,
+                                                 const cplx<double>* __restrict__ Zdot,
+                                                 const double* __restrict__ rho, const double* __restrict__ ehc,
+                                                 const double* __restrict__ ehc_dot,
+                                                 const int* __restrict__ atom_struct,
+                                                 const double* __restrict__ gE, double* __restrict__ alpha,
+                                                 double* __restrict__ beta) {
+    __shared__ double red[8];
+    const int atom = blockIdx.x;
+    double s = 0.0;
+    const cplx<double>* w = W + (size_t)atom * P.nz;
+    const cplx<double>* zd = Zdot + (size_t)atom * P.nz;
+    for (int x = threadIdx.x; x < P.nz; x += blockDim.x) s += re_mul(w[x], zd[x]);
+    const double tot = block_sum<double>(s, red);
+    if (threadIdx.x == 0) {
+        double F, dF, d2F, fc, dfc;
+        embed2(P.nfemb, rho[atom], P.p1a, P.p1b, F, dF, d2F);
+        inner_cutoff(ehc[atom], P.ecut, P.decut, fc, dfc);
+        const double sc = fc * dF;
+        const double rho_dot = sc != 0.0 ? tot / sc : 0.0;
+        const double g = atom_struct ? gE[atom_struct[atom]] : 0.0;
+        alpha[atom] = g * fc * dF - fc * d2F * rho_dot - dF * dfc * ehc_dot[atom];
+        beta[atom] = -fc * dF;
+    }
+}
+
+// M_t += sum_i alpha_i P_t,i + beta_i D_u P_t,i over a chunk of atoms (Z, Zdot of one atom in
+// LDS at a time); grid = (monomial tiles, atom chunks)
+__global__ void __launch_bounds__(256) k_fit_accum(int nterms, int nz, const ushort4* __restrict__ idx,
+                                                  const cplx<double>* __restrict__ Z,
+                                                  const cplx<double>* __restrict__ Zdot,
+                                                  const double* __restrict__ alpha, const double* __restrict__ beta,
+                                                  int natom, int chunk, double* __restrict__ M) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<double>* sZ = (cplx<double>*)smem;  // [nz + 1]
+    cplx<double>* sD = sZ + (nz + 1);          // [nz + 1]
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    ushort4 ix = {0, 0, 0, 0};
+    if (t < nterms) ix = idx[t];
+    const int a0 = blockIdx.y * chunk, a1 = min(natom, a0 + chunk);
+    double accr = 0.0, acci = 0.0;
+    for (int a = a0; a < a1; ++a) {
+        __syncthreads();
+        for (int x = threadIdx.x; x < nz; x += blockDim.x) {
+            sZ[x] = Z[(size_t)a * nz + x];
+            sD[x] = Zdot[(size_t)a * nz + x];
+        }
+        if (threadIdx.x == 0) {
+            sZ[nz] = {1, 0};
+            sD[nz] = {0, 0};
+        }
+        __syncthreads();
+        if (t < nterms) {
+            const cplx<double> z0 = sZ[ix.x], z1 = sZ[ix.y], z2 = sZ[ix.z], z3 = sZ[ix.w];
+            const cplx<double> d0 = sD[ix.x], d1 = sD[ix.y], d2 = sD[ix.z], d3 = sD[ix.w];
+            const cplx<double> z01 = cmul(z0, z1), z23 = cmul(z2, z3);
+            const cplx<double> Pt = cmul(z01, z23);
+            // product rule: d(z0 z1 z2 z3) = (d0 z1 + z0 d1) z23 + z01 (d2 z3 + z2 d3)
+            cplx<double> l = cmul(d0, z1), r = cmul(d2, z3);
+            cfma(l, z0, d1);
+            cfma(r, z2, d3);
+            cplx<double> Pd = cmul(l, z23);
+            cfma(Pd, z01, r);
+            const double al = alpha[a], be = beta[a];
+            accr += al * Pt.re + be * Pd.re;
+            acci += al * Pt.im + be * Pd.im;
+        }
+    }
+    if (t < nterms) {
+        atomicAdd(&M[2 * (size_t)t], accr);
+        atomicAdd(&M[2 * (size_t)t + 1], acci);
+    }
+}
+
+// gradient of the B1 coefficients: sum_i alpha_i (g^m_n(mu_i) - 1)
+__global__ void k_fit_b1(DevParams<double> P, int natom, const double* __restrict__ dmomi,
+                         const double* __restrict__ alpha, double* __restrict__ grad, int* err) {
+    const int atom = blockIdx.x * blockDim.x + threadIdx.x;
+    if (atom >= natom) return;
+    double w;
+    bool unit;
+    const double* row = magnetic_row(P, dmomi[atom], w, unit, err);
+    for (int n = 0; n < P.nb_m; ++n) {
+        double v = unit ? 1.0 : 0.0, dv;
+        if (row) horner(row + 4 * n, w, P.rs_m, v, dv);
+        atomicAdd(&grad[n], alpha[atom] * (v - 1.0));
+    }
+}
+
+// grad[off + k] = Re sum_t u_{k,t} M_t
+__global__ void k_fit_grad(int nb, const int* __restrict__ ptr, const int* __restrict__ tt,
+                           const double* __restrict__ uu, const double* __restrict__ M, double* __restrict__ grad) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nb) return;
+    double s = 0.0;
+    for (int e = ptr[k]; e < ptr[k + 1]; ++e) s += uu[e] * M[2 * (size_t)tt[e]];
+    grad[k] = s;
+}
+
+// per-atom forces f_i = -sum_p F_p (own pairs) + sum_{p: j(p) = i} F_p, and per-structure
+// energies (atomics; outputs zeroed by the caller)
+__global__ void k_fit_assemble(int natom, const int* __restrict__ off, const int* __restrict__ nlist,
+                               const double* __restrict__ flist, const double* __restrict__ eout,
+                               const int* __restrict__ atom_struct, double* __restrict__ f,
+                               double* __restrict__ estruct) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natom) return;
+    double fx = 0, fy = 0, fz = 0;
+    for (int p = off[i]; p < off[i + 1]; ++p) {
+        const double a = flist[3 * (size_t)p], b = flist[3 * (size_t)p + 1], c = flist[3 * (size_t)p + 2];
+        fx -= a; fy -= b; fz -= c;
+        const int j = nlist[p];
+        atomicAdd(&f[3 * (size_t)j], a);
+        atomicAdd(&f[3 * (size_t)j + 1], b);
+        atomicAdd(&f[3 * (size_t)j + 2], c);
+    }
+    atomicAdd(&f[3 * (size_t)i], fx);
+    atomicAdd(&f[3 * (size_t)i + 1], fy);
+    atomicAdd(&f[3 * (size_t)i + 2], fz);
+    if (estruct) atomicAdd(&estruct[atom_struct[i]], eout[i]);
+}
+
+__global__ void k_fill_double(double* __restrict__ p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+template <int L, int CH, int NB, int NM3X, int NB2, int NR2X>
+static hipError_t run_jvp_class(const DevParams<double>& P, int natom, const int* off, const int* nlist,
+                                const double* du, const double* dlist, const double* elist, const double* dmom,
+                                const double* emom, cplx<double>* Zdot, double* ehc_dot, int* err, hipStream_t s) {
+    auto k = k_density_jvp<L, CH, NB, NM3X, NB2, NR2X>;
+    const size_t lds = density_jvp_lds_bytes(P, CH);
+    static bool set = false;
+    static size_t limit = 0;
+    hipError_t e;
+    if (!set) {
+        int dev = 0, maxsh = 0;
+        if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+        if ((e = hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev)) != hipSuccess)
+            return e;
+        hipFuncAttributes fa;
+        if ((e = hipFuncGetAttributes(&fa, (const void*)k)) != hipSuccess) return e;
+        limit = (size_t)maxsh - fa.sharedSizeBytes;
+        if ((e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)limit)) !=
+            hipSuccess)
+            return e;
+        set = true;
+    }
+    if (lds > limit) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k, dim3(natom), dim3(256), lds, s, P, off, nlist, du, dlist, elist, dmom, emom, Zdot,
+                       ehc_dot, err);
+    return hipGetLastError();
+}
+
+hipError_t fit_backward(const DevParams<double>& P, int cls, int natom, const int* off, const int* nlist,
+                        const double* du, const double* dlist, const double* elist, const double* dmomi,
+                        const double* dmom, const double* emom, const int* atom_struct, const double* gE,
+                        Buffers& B, FitBuffers& F, int nterms, int nb234, double* grad, hipStream_t s) {
+    if (natom <= 0) return hipSuccess;
+    hipError_t e = cls == 0 ? run_jvp_class<6, 16, 4, 4, 4, 2>(P, natom, off, nlist, du, dlist, elist, dmom, emom,
+                                                               (cplx<double>*)F.Zdot, F.ehc_dot, B.err, s)
+                            : run_jvp_class<2, 16, 1, 27, 2, 5>(P, natom, off, nlist, du, dlist, elist, dmom, emom,
+                                                                (cplx<double>*)F.Zdot, F.ehc_dot, B.err, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fit_atom, dim3(natom), dim3(256), 0, s, P, (const cplx<double>*)B.W,
+                       (const cplx<double>*)F.Zdot, B.rho, B.ehc, F.ehc_dot, atom_struct, gE, F.alpha, F.beta);
+    const size_t nM = 2 * (size_t)nterms;
+    hipLaunchKernelGGL(k_fill_double, dim3((unsigned)((nM + 255) / 256)), dim3(256), 0, s, F.M, nM);
+    const size_t ng = (size_t)P.nb_m + nb234;
+    hipLaunchKernelGGL(k_fill_double, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s, grad, ng);
+    const int chunk = 64;
+    const size_t lds = 2 * (size_t)(P.nz + 1) * sizeof(cplx<double>);
+    dim3 grid((unsigned)((nterms + 255) / 256), (unsigned)((natom + chunk - 1) / chunk));
+    hipLaunchKernelGGL(k_fit_accum, grid, dim3(256), lds, s, nterms, P.nz, (const ushort4*)F.idx,
+                       (const cplx<double>*)B.Z, (const cplx<double>*)F.Zdot, F.alpha, F.beta, natom, chunk, F.M);
+    hipLaunchKernelGGL(k_fit_b1, dim3((natom + 255) / 256), dim3(256), 0, s, P, natom, dmomi, F.alpha, grad, B.err);
+    hipLaunchKernelGGL(k_fit_grad, dim3((nb234 + 255) / 256), dim3(256), 0, s, nb234, F.ub_ptr, F.ub_t, F.ub_u,
+                       F.M, grad + P.nb_m);
+    return hipGetLastError();
+}
+
+hipError_t fit_assemble(int natom, const int* off, const int* nlist, const double* flist, const double* eout,
+                        const int* atom_struct, int nstruc, double* f, double* estruct, hipStream_t s) {
+    const size_t nf = 3 * (size_t)natom;
+    if (nf) hipLaunchKernelGGL(k_fill_double, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, f, nf);
+    if (estruct && nstruc > 0)
+        hipLaunchKernelGGL(k_fill_double, dim3((unsigned)((nstruc + 255) / 256)), dim3(256), 0, s, estruct,
+                           (size_t)nstruc);
+    if (natom > 0)
+        hipLaunchKernelGGL(k_fit_assemble, dim3((natom + 255) / 256), dim3(256), 0, s, natom, off, nlist, flist,
+                           eout, atom_struct, f, estruct);
+    return hipGetLastError();
+}
+
 // ============================================================================ launch
 __global__ void k_clear_flags(int* p) { *p = 0; }
 
@@ -934,7 +1380,7 @@ static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, co
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[1], s);
     hipLaunchKernelGGL(k2, dim3(natom), dim3(256), lds2, s, P, (const cplx<T>*)B.Z, (cplx<T>*)B.W, B.ehc,
-                       B.rho1, eout, B.dE, B.edf);
+                       B.rho1, eout, B.dE, B.edf, B.rho);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(k3, dim3(natom), dim3(64), lds3, s, P, off, dlist, elist, dmomi, emomi, dmom, emom,

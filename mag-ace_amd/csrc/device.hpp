@@ -47,6 +47,7 @@ struct Buffers {
     double* rho1 = nullptr;  // B1 part of rho
     double* dE = nullptr;    // fcut * F'(rho)
     double* edf = nullptr;   // d f_in/d e_hc * F + 1
+    double* rho = nullptr;   // rho_i (including the B1 part)
     int* err = nullptr;
     size_t cap_atoms = 0;
 };
@@ -59,6 +60,25 @@ struct Launch {
                           double* tlist, double* tlist0, Buffers& B, hipStream_t s, int nm3_class,
                           hipEvent_t* ev);
 };
+
+// fitting (FP64): forward-mode density, per-atom adjoint scalars, monomial accumulation
+struct FitBuffers {
+    void* Zdot = nullptr;        // [natom][nz] complex
+    double* ehc_dot = nullptr;   // [natom]
+    double* alpha = nullptr;     // [natom]
+    double* beta = nullptr;      // [natom]
+    double* M = nullptr;         // [nterms] complex
+    unsigned short* idx = nullptr;  // [nterms][4] monomial factors
+    int *ub_ptr = nullptr, *ub_t = nullptr;
+    double* ub_u = nullptr;
+    size_t cap_atoms = 0;
+};
+hipError_t fit_backward(const DevParams<double>& P, int cls, int natom, const int* off, const int* nlist,
+                        const double* du, const double* dlist, const double* elist, const double* dmomi,
+                        const double* dmom, const double* emom, const int* atom_struct, const double* gE,
+                        Buffers& B, FitBuffers& F, int nterms, int nb234, double* grad, hipStream_t s);
+hipError_t fit_assemble(int natom, const int* off, const int* nlist, const double* flist, const double* eout,
+                        const int* atom_struct, int nstruc, double* f, double* estruct, hipStream_t s);
 
 hipError_t upload_legendre_constants();
 hipError_t clear_flags(int* p, hipStream_t s);

@@ -43,6 +43,9 @@ struct Potential {
     double p1[2] = {1.0, 1.0};           // embedding prefactor / exponent
     double core[2] = {3.0, 3.0};         // core repulsion prefactor / exponent
     double rcut = 0, rcut_m = 0;         // global cutoffs from acelibini_
+    std::string element = "Fe", xc = "pbe";
+    double smear = 0.0;
+    std::vector<std::string> header_lines;  // the 14 header lines as read
 };
 
 // Per-atom complex feature vector Z = [A | A2 | A0] (all m, m' stored).
@@ -74,6 +77,13 @@ struct TermList {
     // folding of the gradient onto representatives of conjugate pairs (x, xbar):
     // partner | sign << 16 (1 = odd m+m') | representative << 17 | self-conjugate << 18
     std::vector<uint32_t> fold;
+    // coefficient structure: B function k (orders 2..4 in file order) -> monomials,
+    // rho = sum_k c_k B_k with B_k = Re sum_t u_{k,t} prod Z[idx_t]; CSR by k and by t
+    std::vector<int32_t> ub_ptr, ub_t;
+    std::vector<double> ub_u;
+    std::vector<int32_t> ut_ptr, ut_k;
+    std::vector<double> ut_u;
+    std::vector<int32_t> rec_t;  // monomial of each gradient record (-1: padding)
     int n2 = 0, n3 = 0, n4 = 0;  // raw (before merging) m-combination counts per order
     int raw = 0;
 };
@@ -103,6 +113,9 @@ void reset_error();
 // potential.cpp
 bool parse_potential(const std::string& path, const std::string& element, int xc, double smear,
                      Potential& pot, std::string& err);
+// writes the potential with its current parameters in the file format it was read from
+// (record order of src/util.F90:1110-1362: par, core, rad_atomic, rad_magnetic, clu 1..4)
+bool write_potential(const std::string& path, const Potential& pot, std::string& err);
 // basis.cpp
 bool check_header(const Header& h, std::string& err);
 Basis enumerate_basis(const Header& h);

@@ -58,10 +58,15 @@ bool parse_potential(const std::string& path, const std::string& element, int xc
     if (!xcname) { err = "Unknown XC functional received by acelibini."; return false; }
     std::string line;
     std::vector<std::vector<std::string>> hdr;
+    pot.header_lines.clear();
     for (int i = 0; i < 14; ++i) {
         if (!std::getline(in, line)) { err = "malformed potential header in " + path; return false; }
+        pot.header_lines.push_back(line);
         hdr.push_back(split(line));
     }
+    pot.element = element;
+    pot.xc = xcname;
+    pot.smear = smear;
     auto need = [&](int row, size_t n) { return hdr[row].size() >= n; };
     if (!need(1, 7) || !need(2, 2) || !need(3, 2) || !need(4, 2) || !need(5, 3) || !need(6, 10)) {
         err = "malformed potential header in " + path;
@@ -180,6 +185,50 @@ bool parse_potential(const std::string& path, const std::string& element, int xc
             if (p[0] != element || p[1] != element) { err = "Error when reading radial parameters."; return false; }
             pot.core[norder - 1] = to_d(p[2]);
         }
+    }
+    return true;
+}
+
+}  // namespace magace
+
+namespace magace {
+
+bool write_potential(const std::string& path, const Potential& pot, std::string& err) {
+    std::ofstream out(path);
+    if (!out) {
+        err = "cannot write " + path;
+        return false;
+    }
+    out.precision(17);
+    const Header& h = pot.h;
+    const std::string& E = pot.element;
+    for (const auto& l : pot.header_lines) out << l << "\n";
+    for (int k = 0; k < 2; ++k) out << " par " << k + 1 << " 1\n " << E << " " << pot.p1[k] << " F\n";
+    for (int k = 0; k < 2; ++k)
+        out << " core " << k + 1 << " 0\n " << E << " " << E << " " << std::fabs(pot.core[k]) << " F\n";
+    for (int nb = 1; nb <= h.nb_a; ++nb)
+        for (int nr = 1; nr <= h.nr3_a; ++nr)
+            for (int l = 0; l <= h.l3a; ++l)
+                out << " rad_atomic 0 0\n " << E << " " << E << " " << nb << " " << nr << " " << l << " "
+                    << pot.crad_a[((size_t)(nr - 1) * (h.l3a + 1) + l) * h.nb_a + (nb - 1)] << " T\n";
+    for (int nb = 1; nb <= h.nb_m; ++nb)
+        for (int nr = 1; nr <= h.nr2_m; ++nr)
+            for (int l = 0; l <= h.l2m; ++l)
+                out << " rad_magnetic 0 0\n " << E << " " << E << " " << nb << " " << nr << " " << l << " "
+                    << pot.crad_m[((size_t)(nr - 1) * (h.l2m + 1) + l) * h.nb_m + (nb - 1)] << " T\n";
+    auto clu = [&](int order, const int* idx, int n, double c) {
+        out << " clu " << order << " 1\n";
+        for (int i = 0; i < order; ++i) out << " " << E;
+        for (int i = 0; i < n; ++i) out << " " << idx[i];
+        out << " " << c << " T\n";
+    };
+    for (size_t k = 0; k < pot.basis.b1.size(); ++k) clu(1, pot.basis.b1[k].data(), 1, pot.c1[k]);
+    for (size_t k = 0; k < pot.basis.b2.size(); ++k) clu(2, pot.basis.b2[k].data(), 4, pot.c2[k]);
+    for (size_t k = 0; k < pot.basis.b3.size(); ++k) clu(3, pot.basis.b3[k].data(), 9, pot.c3[k]);
+    for (size_t k = 0; k < pot.basis.b4.size(); ++k) clu(4, pot.basis.b4[k].data(), 15, pot.c4[k]);
+    if (!out) {
+        err = "write failed: " + path;
+        return false;
     }
     return true;
 }

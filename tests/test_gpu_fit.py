@@ -1,0 +1,179 @@
+"""Fitting gradients on the GPU (magace_fit_*, fit.EnergyForces).
+
+Pins: the energy part of dL/dc is sum_i gE_s(i) fc F' B_k,i, checked against the per-atom B
+vectors of the oracle (the reference's prepB restated); the full gradient (energy and force
+terms) against central differences of L(c) = sum_s gE_s E_s + sum_j gF_j . F_j, which are
+exact up to rounding when E is linear in c (nfemb 0) and, Richardson-extrapolated, checked
+to 1e-7 for nonlinear embeddings with core repulsion and the inner cutoff active.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import HEADERS, PKG
+
+capi, potgen, systems, fit, nbr = PKG.capi, PKG.potgen, PKG.systems, PKG.fit, PKG.neighbors
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+if not torch.cuda.is_available():
+    pytest.skip("no GPU visible", allow_module_level=True)
+
+DEV = torch.device("cuda:0")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def structures(seed, sizes=((2, 2, 2), (3, 2, 2), (2, 2, 1))):
+    rng = np.random.default_rng(seed)
+    out = []
+    for n1, n2, n3 in sizes:
+        pos, cell = systems.bcc_box(n1, n2, n3, 2.83 * rng.uniform(0.98, 1.02))
+        pos = systems.jitter(pos, rng, 0.08)
+        sdir, smag = systems.spins_uniform(len(pos), rng)
+        out.append(dict(pos=pos, cell=cell, sdir=sdir, smag=smag))
+    return out
+
+
+def load(tmp_path, header, seed, c_scale=1e-2):
+    pot = str(tmp_path / "Fe.pbe.in")
+    h = HEADERS[header]
+    with open(pot, "w") as f:
+        f.write(potgen.render(h, capi.enumerate_basis(h), seed, c_scale=c_scale))
+    capi.init_file(pot)
+    return pot
+
+
+def loss(batch, c, gE, gF):
+    capi.fit_set_coefficients(c)
+    E = torch.zeros(batch.nstruc, dtype=torch.float64, device=DEV)
+    F = torch.zeros(batch.natom, 3, dtype=torch.float64, device=DEV)
+    batch.forward(E, F)
+    torch.cuda.synchronize()
+    return float((E * gE).sum() + (F * gF).sum())
+
+
+def gradient(batch, c, gE, gF):
+    capi.fit_set_coefficients(c)
+    E = torch.zeros(batch.nstruc, dtype=torch.float64, device=DEV)
+    F = torch.zeros(batch.natom, 3, dtype=torch.float64, device=DEV)
+    batch.forward(E, F)
+    g = torch.zeros(len(c), dtype=torch.float64, device=DEV)
+    batch.backward(gE, gF.contiguous(), g)
+    return g.cpu().numpy()
+
+
+def test_energy_gradient_equals_oracle_b_vectors(tmp_path):
+    say("fit: energy gradient vs oracle B vectors (lmax-4 header)")
+    pot = load(tmp_path, "H500", 2040)
+    structs = structures(1)
+    batch = fit.Batch(structs, 5.0)
+    c = capi.fit_get_coefficients()
+    gE = torch.tensor([1.0, -2.0, 0.5], dtype=torch.float64, device=DEV)
+    gF = torch.zeros(batch.natom, 3, dtype=torch.float64, device=DEV)
+    g = gradient(batch, c, gE, gF)
+    o = O.Oracle.load(pot)
+    want = np.zeros_like(g)
+    for s, w in zip(structs, gE.cpu().numpy()):
+        a = nbr.neilist(s["pos"], s["sdir"] * s["smag"][:, None], s["cell"], 5.0)
+        want += w * o.bvectors(a).sum(axis=0)
+    err = np.abs(g - want).max() / np.abs(want).max()
+    say("  rel err %.2e" % err)
+    assert err < 1e-10
+
+
+@pytest.mark.parametrize("header,seed,tol", [("H500", 2041, 1e-9), ("H500CORE", 2042, 1e-7), ("HFE", 2043, 1e-7)])
+def test_full_gradient_matches_central_differences(tmp_path, header, seed, tol):
+    say("fit: dL/dc vs central differences,", header)
+    load(tmp_path, header, seed)
+    batch = fit.Batch(structures(seed), 5.0)
+    rng = np.random.default_rng(seed)
+    c0 = capi.fit_get_coefficients()
+    gE = torch.tensor(rng.normal(size=batch.nstruc), dtype=torch.float64, device=DEV)
+    gF = torch.tensor(rng.normal(size=(batch.natom, 3)), dtype=torch.float64, device=DEV)
+    g = gradient(batch, c0, gE, gF)
+
+    def central(d):
+        # linear embedding: L is linear in c and one central difference is exact for any step;
+        # nonlinear embeddings: Richardson extrapolation of steps h and h/2 (error O(h^4))
+        if header == "H500":
+            h = 1e-2
+            return (loss(batch, c0 + h * d, gE, gF) - loss(batch, c0 - h * d, gE, gF)) / (2 * h)
+        h = 1e-5
+        f1 = (loss(batch, c0 + h * d, gE, gF) - loss(batch, c0 - h * d, gE, gF)) / (2 * h)
+        f2 = (loss(batch, c0 + 0.5 * h * d, gE, gF) - loss(batch, c0 - 0.5 * h * d, gE, gF)) / h
+        return (4.0 * f2 - f1) / 3.0
+
+    for trial in range(3):
+        d = rng.normal(size=len(c0))
+        fd = central(d)
+        an = float(g @ d)
+        say("  trial %d: analytic %.10e  central difference %.10e" % (trial, an, fd))
+        assert abs(an - fd) <= tol * max(1.0, abs(fd)), (an, fd)
+    # single coefficients (first B1, a middle one, the last order-4 function)
+    for k in (0, len(c0) // 3, len(c0) - 1):
+        d = np.zeros(len(c0))
+        d[k] = 1.0
+        fd = central(d)
+        assert abs(g[k] - fd) <= tol * max(1.0, abs(fd)), (k, g[k], fd)
+    capi.fit_set_coefficients(c0)
+
+
+def test_torch_training_step_reduces_loss(tmp_path):
+    """Teacher/student: forces and energies of a teacher potential are the targets; the
+    autograd gradient of the student's loss matches a central difference and Adam steps
+    reduce the loss."""
+    say("fit: torch autograd loss and optimiser steps")
+    load(tmp_path, "H500", 2044)
+    batch = fit.Batch(structures(7), 5.0)
+    c_teacher = capi.fit_get_coefficients()
+    E_ref = torch.zeros(batch.nstruc, dtype=torch.float64, device=DEV)
+    F_ref = torch.zeros(batch.natom, 3, dtype=torch.float64, device=DEV)
+    capi.fit_set_coefficients(c_teacher)
+    batch.forward(E_ref, F_ref)
+    rng = np.random.default_rng(3)
+    c = torch.tensor(c_teacher + 0.3 * np.abs(c_teacher).mean() * rng.normal(size=len(c_teacher)),
+                     dtype=torch.float64, device=DEV, requires_grad=True)
+
+    def L(cc):
+        E, F = fit.EnergyForces.apply(cc, batch)
+        return ((E - E_ref) ** 2).sum() / batch.nstruc + 10.0 * ((F - F_ref) ** 2).sum() / batch.natom
+
+    l0 = L(c)
+    l0.backward()
+    g = c.grad.detach().cpu().numpy()
+    d = rng.normal(size=len(g))
+    h = 1e-4 * np.abs(c_teacher).mean()
+    with torch.no_grad():
+        cd = torch.tensor(d, dtype=torch.float64, device=DEV)
+        fd = (float(L(c + h * cd)) - float(L(c - h * cd))) / (2 * h)
+    say("  analytic %.8e  central %.8e" % (g @ d, fd))
+    assert abs(g @ d - fd) <= 1e-6 * max(1e-12, abs(fd))
+    opt = torch.optim.Adam([c], lr=2e-3 * np.abs(c_teacher).mean())
+    losses = [float(l0)]
+    for _ in range(20):
+        opt.zero_grad()
+        l = L(c)
+        l.backward()
+        opt.step()
+        losses.append(float(l))
+    say("  loss %.4e -> %.4e" % (losses[0], losses[-1]))
+    assert losses[-1] < 0.5 * losses[0]
+
+
+def test_config_batch_fit_step(tmp_path):
+    """The batched-fit workload (256 configurations, lmax-4 header): one forward + backward."""
+    say("fit: 256-configuration batch")
+    load(tmp_path, "H500", 2029)
+    structs = systems.config_batch()
+    batch = fit.Batch(structs, 5.0)
+    c = torch.tensor(capi.fit_get_coefficients(), dtype=torch.float64, device=DEV, requires_grad=True)
+    E, F = fit.EnergyForces.apply(c, batch)
+    (E.sum() + (F ** 2).sum()).backward()
+    assert torch.isfinite(c.grad).all()
+    say("  atoms", batch.natom, "pairs", batch.npairs, "|grad|", float(c.grad.norm()))
