@@ -83,6 +83,49 @@ def md_leg(pkg, td, steps=100, warmup=20):
             "energy_drift_per_atom": (st["epot"] + st["ekin"] - st0["epot"] - st0["ekin"]) / n}
 
 
+def fit_leg(pkg, td, steps=5):
+    """Secondary measurement: one batched fit step = energies and forces of 256 periodic bcc Fe
+    configurations (14-1023 atoms, log-uniform sizes, strain +-3%, 0-2 vacancies; seed 2029)
+    and the gradient of a loss in all 489 coefficients of the lmax-4 header (FP64): the
+    device neighbour build runs once, each step is magace_fit_forward + magace_fit_backward."""
+    import torch
+    capi, potgen, systems, fit = pkg.capi, pkg.potgen, pkg.systems, pkg.fit
+    h = dict(potgen.H500)
+    pot = os.path.join(td, "Fe_fit.pbe.in")
+    with open(pot, "w") as f:
+        f.write(potgen.render(h, capi.enumerate_basis(h), 2029))
+    capi.init_file(pot)
+    structs = systems.config_batch()
+    t0 = time.perf_counter()
+    batch = fit.Batch(structs, 5.0)
+    torch.cuda.synchronize()
+    t_nb = time.perf_counter() - t0
+    dev = batch.device
+    c = capi.fit_get_coefficients()
+    E = torch.zeros(batch.nstruc, dtype=torch.float64, device=dev)
+    F = torch.zeros(batch.natom, 3, dtype=torch.float64, device=dev)
+    gE = torch.ones(batch.nstruc, dtype=torch.float64, device=dev)
+    gF = torch.ones(batch.natom, 3, dtype=torch.float64, device=dev)
+    g = torch.zeros(len(c), dtype=torch.float64, device=dev)
+
+    def step():
+        batch.forward(E, F)
+        batch.backward(gE, gF, g)
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    return {"workload": "batched fit step: 256 bcc Fe configurations (%d atoms, %d pairs), lmax-4 header, "
+                        "energies + forces + dL/dc (489 coefficients), FP64" % (batch.natom, batch.npairs),
+            "configs": batch.nstruc, "atoms": batch.natom, "ms_per_step": 1e3 * dt,
+            "atom_steps_per_s": batch.natom / dt, "configs_per_s": batch.nstruc / dt,
+            "neighbour_build_s": t_nb}
+
+
 def cpu_baseline(pkg, pot, a, natom_sample, use_reference=True):
     """Reference libace (oracle/_ref, serial Fortran) on the first natom_sample atoms of the
     same workload; falls back to the oracle's C restatement (OpenMP) if the reference
@@ -119,7 +162,8 @@ def main():
     ap.add_argument("--ncell", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=400)
-    ap.add_argument("--no-md", action="store_true", help="skip the 128000-atom spin-lattice MD leg")
+    ap.add_argument("--no-md", action="store_true",
+                    help="skip the secondary legs (128000-atom spin-lattice MD, 256-configuration fit step)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain 512 MiB device copy (the reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -231,6 +275,10 @@ def main():
     achieved = fm[kernels[dom]] * N / (km[dom] * 1e-3) / 1e12
     total_tf = fm["total"] * res["mixed"]["value"] / 1e12
 
+    fitres = None
+    if not args.no_md:
+        fitres = fit_leg(pkg, td)
+        log("fit (256 configurations):", {k: v for k, v in fitres.items() if k != "workload"})
     mdres = None
     if not args.no_md:
         mdres = md_leg(pkg, td)
@@ -275,6 +323,7 @@ def main():
         "cpu_baseline": cpu,
         "neighbor_build_ms": nb_ms,
         "spin_lattice_md": mdres,
+        "batched_fit_step": fitres,
         "value_with_rebuild_every_10": N / (1e-3 * (res["mixed"]["ms_per_step"] + nb_ms / 10.0)),
     }
     if cpu:
