@@ -70,6 +70,8 @@ struct Context {
     DevTables<float> t32;
     uint4* crec64 = nullptr;
     uint4* crec32 = nullptr;
+    double* cw2_64 = nullptr;  // density-2 record weights (ndensity 2)
+    float* cw2_32 = nullptr;
     int2* chunks = nullptr;
     uint32_t* fold = nullptr;
     Buffers buf;
@@ -115,7 +117,7 @@ void dfree(T*& p) {
 
 Context::~Context() {
     dfree(t64.lut_a); dfree(t64.lut_m); dfree(t32.lut_a); dfree(t32.lut_m);
-    dfree(crec64); dfree(crec32); dfree(chunks); dfree(fold);
+    dfree(crec64); dfree(crec32); dfree(chunks); dfree(fold); dfree(cw2_64); dfree(cw2_32);
     dfree(buf.Z); dfree(buf.W); dfree(buf.ehc); dfree(buf.rho1); dfree(buf.dE); dfree(buf.edf); dfree(buf.rho);
     dfree(buf.err);
     dfree(fit.Zdot); dfree(fit.ehc_dot); dfree(fit.alpha); dfree(fit.beta); dfree(fit.M); dfree(fit.idx);
@@ -127,6 +129,9 @@ Context::~Context() {
         for (auto& x : e) (void)hipEventDestroy(x);
     if (stream) (void)hipStreamDestroy(stream);
 }
+
+template <typename T>
+void set_c1(DevParams<T>& P, const Model& M);
 
 template <typename T>
 void fill_params(DevParams<T>& P, const Model& M) {
@@ -144,11 +149,22 @@ void fill_params(DevParams<T>& P, const Model& M) {
     P.lhm = (h.l2m + 1) * (h.l2m + 2) / 2;
     P.nchunks = (int)(M.terms.chunks.size() / 2);
     P.nfemb = h.nfemb;
-    P.p1a = M.pot.p1[0];
-    P.p1b = M.pot.p1[1];
+    P.nd = h.ndensity;
+    for (int d = 0; d < 2; ++d) {
+        P.p1[d][0] = M.pot.p1[d][0];
+        P.p1[d][1] = M.pot.p1[d][1];
+    }
     P.ecut = h.ecut;
     P.decut = h.decut;
-    for (int k = 0; k < 32; ++k) P.c1[k] = k < (int)M.pot.c1.size() ? M.pot.c1[k] : 0.0;
+    set_c1(P, M);
+}
+
+template <typename T>
+void set_c1(DevParams<T>& P, const Model& M) {
+    const size_t nb1 = M.pot.basis.b1.size();
+    for (int d = 0; d < 2; ++d)
+        for (int k = 0; k < 32; ++k)
+            P.c1[d][k] = d < M.pot.h.ndensity && k < (int)nb1 ? M.pot.c1[d * nb1 + k] : 0.0;
 }
 
 template <typename T>
@@ -194,7 +210,7 @@ int build_context(Potential&& pot, std::unique_ptr<Context>& out) {
 // 16-byte gradient records (factor indices a | b << 16, c; weight as FP64 bits or FP32 bits)
 int upload_records(Context& c) {
     const TermList& T = c.model.terms;
-    const size_t nc = T.cw.size();
+    const size_t nc = T.ncontrib();
     std::vector<uint4> r64(nc), r32(nc);
     for (size_t i = 0; i < nc; ++i) {
         const uint16_t* q = &T.cidx[4 * i];
@@ -210,6 +226,18 @@ int upload_records(Context& c) {
     }
     HIPCHK(hipMemcpy(c.crec64, r64.data(), nc * sizeof(uint4), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c.crec32, r32.data(), nc * sizeof(uint4), hipMemcpyHostToDevice));
+    if (T.nd > 1) {
+        if (!c.cw2_64) {
+            HIPCHK(hipMalloc((void**)&c.cw2_64, nc * sizeof(double) + 64));
+            HIPCHK(hipMalloc((void**)&c.cw2_32, nc * sizeof(float) + 64));
+        }
+        std::vector<float> w32(nc);
+        for (size_t i = 0; i < nc; ++i) w32[i] = (float)T.cw[nc + i];
+        HIPCHK(hipMemcpy(c.cw2_64, T.cw.data() + nc, nc * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c.cw2_32, w32.data(), nc * sizeof(float), hipMemcpyHostToDevice));
+        c.P64.cw2 = c.cw2_64;
+        c.P32.cw2 = c.cw2_32;
+    }
     return 0;
 }
 
@@ -217,22 +245,25 @@ int upload_records(Context& c) {
 int refresh_weights(Context& c) {
     Potential& pot = c.model.pot;
     TermList& T = c.model.terms;
-    std::vector<double> cv;
-    cv.insert(cv.end(), pot.c2.begin(), pot.c2.end());
-    cv.insert(cv.end(), pot.c3.begin(), pot.c3.end());
-    cv.insert(cv.end(), pot.c4.begin(), pot.c4.end());
-    for (size_t t = 0; t + 1 < T.ut_ptr.size(); ++t) {
-        double w = 0.0;
-        for (int e = T.ut_ptr[t]; e < T.ut_ptr[t + 1]; ++e) w += cv[T.ut_k[e]] * T.ut_u[e];
-        T.w[t] = w;
+    const Basis& B = pot.basis;
+    const size_t n2 = B.b2.size(), n3 = B.b3.size(), n4 = B.b4.size(), nb = n2 + n3 + n4;
+    const size_t nt = T.nterms(), nr = T.ncontrib();
+    for (int d = 0; d < T.nd; ++d) {
+        std::vector<double> cv;
+        cv.insert(cv.end(), pot.c2.begin() + d * n2, pot.c2.begin() + (d + 1) * n2);
+        cv.insert(cv.end(), pot.c3.begin() + d * n3, pot.c3.begin() + (d + 1) * n3);
+        cv.insert(cv.end(), pot.c4.begin() + d * n4, pot.c4.begin() + (d + 1) * n4);
+        (void)nb;
+        for (size_t t = 0; t < nt; ++t) {
+            double w = 0.0;
+            for (int e = T.ut_ptr[t]; e < T.ut_ptr[t + 1]; ++e) w += cv[T.ut_k[e]] * T.ut_u[e];
+            T.w[d * nt + t] = w;
+        }
+        for (size_t r = 0; r < nr; ++r) T.cw[d * nr + r] = T.rec_t[r] >= 0 ? T.w[d * nt + T.rec_t[r]] : 0.0;
     }
-    for (size_t r = 0; r < T.cw.size(); ++r) T.cw[r] = T.rec_t[r] >= 0 ? T.w[T.rec_t[r]] : 0.0;
-    for (int n = 0; n < 32; ++n) {
-        const double v = n < (int)pot.c1.size() ? pot.c1[n] : 0.0;
-        c.P64.c1[n] = v;
-        c.P32.c1[n] = v;
-    }
-    if (c.uploaded && !T.cw.empty()) return upload_records(c);
+    set_c1(c.P64, c.model);
+    set_c1(c.P32, c.model);
+    if (c.uploaded && nr) return upload_records(c);
     return 0;
 }
 
@@ -248,7 +279,7 @@ int upload(Context& c) {
     if (upload_table(c.t64.lut_m, M.spl.magnetic)) return MAGACE_ERR_DEVICE;
     if (upload_table(c.t32.lut_a, M.spl.atomic)) return MAGACE_ERR_DEVICE;
     if (upload_table(c.t32.lut_m, M.spl.magnetic)) return MAGACE_ERR_DEVICE;
-    const size_t nc = M.terms.cw.size(), nch = M.terms.chunks.size() / 2;
+    const size_t nc = M.terms.ncontrib(), nch = M.terms.chunks.size() / 2;
     HIPCHK(hipMalloc((void**)&c.crec64, nc * sizeof(uint4) + 64));
     HIPCHK(hipMalloc((void**)&c.crec32, nc * sizeof(uint4) + 64));
     HIPCHK(hipMalloc((void**)&c.chunks, nch * sizeof(int2) + 64));
@@ -278,11 +309,12 @@ int ensure_atoms(Context& c, size_t natom) {
     HIPCHK(hipMalloc(&c.buf.Z, cap * nz * 2 * sizeof(double)));
     HIPCHK(hipMalloc(&c.buf.W, cap * nz * 2 * sizeof(double)));
     HIPCHK(hipMalloc((void**)&c.buf.ehc, cap * sizeof(double)));
-    HIPCHK(hipMalloc((void**)&c.buf.rho1, cap * sizeof(double)));
-    HIPCHK(hipMalloc((void**)&c.buf.dE, cap * sizeof(double)));
+    const size_t nd = (size_t)c.model.pot.h.ndensity;
+    HIPCHK(hipMalloc((void**)&c.buf.rho1, cap * nd * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&c.buf.dE, cap * nd * sizeof(double)));
     HIPCHK(hipMalloc((void**)&c.buf.edf, cap * sizeof(double)));
     dfree(c.buf.rho);
-    HIPCHK(hipMalloc((void**)&c.buf.rho, cap * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&c.buf.rho, cap * nd * sizeof(double)));
     c.buf.cap_atoms = cap;
     return 0;
 }
@@ -597,7 +629,7 @@ int magace_model_terms(uint16_t* idx, double* w, int cap) {
         return -MAGACE_ERR_STATE;
     }
     const TermList& T = g_ctx->model.terms;
-    const int n = (int)T.w.size();
+    const int n = (int)T.nterms();  // weights of density 1
     if (idx && w && cap >= n) {
         std::memcpy(idx, T.idx.data(), (size_t)n * 4 * sizeof(uint16_t));
         std::memcpy(w, T.w.data(), (size_t)n * sizeof(double));
@@ -669,6 +701,10 @@ int magace_fit_forward(int natom, int nei, const int* offsets, const int* nlist,
         return MAGACE_ERR_STATE;
     }
     Context& c = *g_ctx;
+    if (c.model.pot.h.ndensity != 1) {
+        set_error(MAGACE_ERR_INPUT, "fitting supports ndensity 1");
+        return MAGACE_ERR_INPUT;
+    }
     if (natom < 0 || nei < 0 || nstruc < 0 || ((nstruc > 0 || estruct) && !atom_struct) || !forces) {
         set_error(MAGACE_ERR_INPUT, "magace_fit_forward: invalid sizes");
         return MAGACE_ERR_INPUT;
@@ -731,7 +767,7 @@ int magace_fit_backward(int natom, int nei, const int* offsets, const int* nlist
         HIPCHK(hipMemcpy(F.ub_ptr, T.ub_ptr.data(), T.ub_ptr.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(F.ub_t, T.ub_t.data(), T.ub_t.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(F.ub_u, T.ub_u.data(), T.ub_u.size() * sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(hipMalloc((void**)&F.M, 2 * T.w.size() * sizeof(double) + 64));
+        HIPCHK(hipMalloc((void**)&F.M, 2 * T.nterms() * sizeof(double) + 64));
     }
     if ((size_t)natom > F.cap_atoms) {
         dfree(F.Zdot); dfree(F.ehc_dot); dfree(F.alpha); dfree(F.beta);
@@ -744,7 +780,7 @@ int magace_fit_backward(int natom, int nei, const int* offsets, const int* nlist
     }
     const int nb234 = (int)T.ub_ptr.size() - 1;
     HIPCHK(fit_backward(c.P64, c.shape_class, natom, offsets, nlist, gF, dlist, elist, dlist_momi, dlist_mom,
-                        elist_mom, gE ? atom_struct : nullptr, gE, c.buf, F, (int)T.w.size(), nb234, grad, s));
+                        elist_mom, gE ? atom_struct : nullptr, gE, c.buf, F, (int)T.nterms(), nb234, grad, s));
     return 0;
 }
 
@@ -754,9 +790,9 @@ int magace_model_info(long long* info, int n) {
         return MAGACE_ERR_STATE;
     }
     const Model& M = g_ctx->model;
-    long long v[10] = {(long long)M.z.nz, (long long)M.terms.w.size(), (long long)M.terms.raw, M.terms.n2,
+    long long v[10] = {(long long)M.z.nz, (long long)M.terms.nterms(), (long long)M.terms.raw, M.terms.n2,
                        M.terms.n3, M.terms.n4, g_ctx->shape_class, (long long)M.z.oA2,
-                       (long long)M.terms.cw.size(), (long long)(M.terms.chunks.size() / 2)};
+                       (long long)M.terms.ncontrib(), (long long)(M.terms.chunks.size() / 2)};
     for (int i = 0; i < n && i < 10; ++i) info[i] = v[i];
     return 0;
 }

@@ -22,7 +22,7 @@ namespace magace {
 bool check_header(const Header& h, std::string& err) {
     auto fail = [&](const char* m) { err = m; return false; };
     if (h.nr5_a || h.nr5_m || h.nr6_a || h.nr6_m) return fail("body order > 4 is not supported");
-    if (h.ndensity != 1) return fail("ndensity must be 1");
+    if (h.ndensity < 1 || h.ndensity > 2) return fail("ndensity must be 1 or 2 (src/forces.F90:1066-1085)");
     if (h.nfemb < 0 || h.nfemb > 2) return fail("embedding function not implemented (nfemb 0..2)");
     // limit checks of src/util.F90:233-272
     if (h.l3a < h.l4a) return fail(" l3max_atomic needs to be largest.");
@@ -345,12 +345,18 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
     for (size_t t = 0; t < rows.size(); ++t) keys[rows[t].second] = (int)t;
     // U (by B function and transposed by monomial), then w_t = sum_k c_k u_{k,t}
     const size_t nt = rows.size();
+    const int nd = h.ndensity;
+    T.nd = nd;
     T.ub_ptr.assign(1, 0);
     std::vector<std::vector<std::pair<int, double>>> byt(nt);
-    std::vector<double> cvec;
-    for (double c : pot.c2) cvec.push_back(c);
-    for (double c : pot.c3) cvec.push_back(c);
-    for (double c : pot.c4) cvec.push_back(c);
+    // coefficients of the B functions of orders 2..4, density-major
+    const size_t n2 = B.b2.size(), n3 = B.b3.size(), n4 = B.b4.size(), nb = n2 + n3 + n4;
+    std::vector<double> cvec(nb * nd);
+    for (int d = 0; d < nd; ++d) {
+        for (size_t k = 0; k < n2; ++k) cvec[d * nb + k] = pot.c2[d * n2 + k];
+        for (size_t k = 0; k < n3; ++k) cvec[d * nb + n2 + k] = pot.c3[d * n3 + k];
+        for (size_t k = 0; k < n4; ++k) cvec[d * nb + n2 + n3 + k] = pot.c4[d * n4 + k];
+    }
     for (size_t k = 0; k < per_b.size(); ++k) {
         for (const auto& e : per_b[k]) {
             if (e.second == 0.0) continue;
@@ -362,16 +368,18 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
         T.ub_ptr.push_back((int32_t)T.ub_t.size());
     }
     T.ut_ptr.assign(1, 0);
-    T.w.assign(nt, 0.0);
+    T.w.assign(nt * nd, 0.0);
     for (size_t t = 0; t < nt; ++t) {
-        double w = 0.0;
         for (const auto& e : byt[t]) {
             T.ut_k.push_back(e.first);
             T.ut_u.push_back(e.second);
-            w += cvec[e.first] * e.second;
+        }
+        for (int d = 0; d < nd; ++d) {
+            double w = 0.0;
+            for (const auto& e : byt[t]) w += cvec[d * nb + e.first] * e.second;
+            T.w[d * nt + t] = w;
         }
         T.ut_ptr.push_back((int32_t)T.ut_k.size());
-        T.w[t] = w;
         for (int i = 0; i < 4; ++i) T.idx.push_back((uint16_t)rows[t].first[i]);
     }
     // transposed gradient: contributions grouped by target
@@ -467,7 +475,7 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
                 T.cidx[4 * r] = (uint16_t)e.first[0];
                 T.cidx[4 * r + 1] = (uint16_t)e.first[1];
                 T.cidx[4 * r + 2] = (uint16_t)e.first[2];
-                T.cw[r] = T.w[e.second];
+                T.cw[r] = T.w[e.second];  // density 1; further densities appended below
                 T.rec_t[r] = e.second;
                 rem.erase(rem.begin() + best);
             }
@@ -477,6 +485,12 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
             T.chunks.push_back((int32_t)(base + L));
             T.chunks.push_back(ch.x | (((ch.cnt + 3) & ~3) << 16));
         }
+    }
+    if (nd > 1) {
+        const size_t nr = T.rec_t.size();
+        T.cw.resize(nr * nd, 0.0);
+        for (int d = 1; d < nd; ++d)
+            for (size_t r = 0; r < nr; ++r) T.cw[d * nr + r] = T.rec_t[r] >= 0 ? T.w[d * nt + T.rec_t[r]] : 0.0;
     }
     // representatives: A entries with m' > 0, or m' = 0 and m >= 0; A2/A0 entries with m' >= 0
     T.fold.assign(z.nz, 0u);

@@ -245,18 +245,19 @@ __global__ void __launch_bounds__(256) k_density(DevParams<T> P, const int* __re
         T w;
         bool unit;
         const T* row = magnetic_row(P, mu, w, unit, err);
-        double rho1 = 0.0;
+        double rho1[2] = {0.0, 0.0};
         for (int k = 0; k < P.nb_m + nfm; ++k) {
             T v = unit ? (T)1 : (T)0, dv;
             if (row) horner(row + 4 * k, w, (T)P.rs_m, v, dv);
             if (k < P.nb_m) {
-                rho1 += P.c1[k] * ((double)v - 1.0);
+                rho1[0] += P.c1[0][k] * ((double)v - 1.0);
+                rho1[1] += P.c1[1][k] * ((double)v - 1.0);
                 sMc[nfm + k] = v;
             } else {
                 sMc[k - P.nb_m] = v;
             }
         }
-        rho1_out[atom] = rho1;
+        for (int d = 0; d < P.nd; ++d) rho1_out[(size_t)atom * P.nd + d] = rho1[d];
         ylm_half<T, L>((T)emomi[3 * atom], (T)emomi[3 * atom + 1], (T)emomi[3 * atom + 2], P.l2m, sYc);
         *sEhc = 0.0;
     }
@@ -461,7 +462,9 @@ __device__ inline void embed(int nfemb, double rho, double wpre, double mexp, do
     }
 }
 
-template <typename T>
+// ND2: two densities (rho_d = sum_k c_{k,d} B_k, E = fc sum_d F(rho_d) + e_hc,
+// src/forces.F90:976-1089): a second weight per record, a second gradient array in LDS
+template <typename T, bool ND2>
 __global__ void __launch_bounds__(256) k_adjoint(DevParams<T> P, const cplx<T>* __restrict__ Zin,
                                                 cplx<T>* __restrict__ Wout,
                                                 const double* __restrict__ ehc,
@@ -471,17 +474,20 @@ __global__ void __launch_bounds__(256) k_adjoint(DevParams<T> P, const cplx<T>* 
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<T>* sZ = (cplx<T>*)smem;           // [nz+1], sZ[nz] = 1
     cplx<T>* sW = sZ + (P.nz + 1);          // [nz+1]
-    double* red = (double*)(sW + (P.nz + 1));
-    __shared__ double s_scale;
+    cplx<T>* sW2 = sW + (P.nz + 1);         // [nz+1] (ND2 only)
+    double* red = (double*)(sW + (ND2 ? 2 : 1) * (P.nz + 1));
+    __shared__ double s_scale[2];
     const int atom = blockIdx.x, tid = threadIdx.x;
     Zin += (size_t)atom * P.nz;
     for (int x = tid; x < P.nz; x += blockDim.x) {
         sZ[x] = Zin[x];
         sW[x] = {0, 0};
+        if (ND2) sW2[x] = {0, 0};
     }
     if (tid == 0) {
         sZ[P.nz] = {1, 0};
         sW[P.nz] = {0, 0};
+        if (ND2) sW2[P.nz] = {0, 0};
     }
     __syncthreads();
     // Gather form of the gradient: a thread sums one chunk of contributions
@@ -491,16 +497,21 @@ __global__ void __launch_bounds__(256) k_adjoint(DevParams<T> P, const cplx<T>* 
     // must match the record layout built by build_terms.  rho = Re sum over A0
     // targets of Z[x] * dRho/dZ[x] (every monomial has exactly one A0 factor), summed in
     // FP64 once per chunk.
-    double rho = 0.0;
+    double rho = 0.0, rho2 = 0.0;
     for (int c = tid; c < P.nchunks; c += blockDim.x) {
         const int2 ch = P.chunks[c];
         const int x = ch.y & 0xffff, cnt = ch.y >> 16;
         const uint4* __restrict__ rec = P.crec + ch.x;  // records of this lane, stride 64
-        cplx<T> acc = {0, 0};
+        const T* __restrict__ w2p = ND2 ? P.cw2 + ch.x : nullptr;
+        cplx<T> acc = {0, 0}, acc2 = {0, 0};
         for (int k = 0; k < cnt; k += 4) {
             uint4 r[4];
+            T w2[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) r[u] = rec[(size_t)(k + u) * 64];
+            for (int u = 0; u < 4; ++u) {
+                r[u] = rec[(size_t)(k + u) * 64];
+                if (ND2) w2[u] = w2p[(size_t)(k + u) * 64];
+            }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const T w = rec_weight<T>(r[u]);
@@ -508,19 +519,40 @@ __global__ void __launch_bounds__(256) k_adjoint(DevParams<T> P, const cplx<T>* 
                 const cplx<T> pr = cmul(cmul(za, zb), zc);
                 acc.re = fma(w, pr.re, acc.re);
                 acc.im = fma(w, pr.im, acc.im);
+                if (ND2) {
+                    acc2.re = fma(w2[u], pr.re, acc2.re);
+                    acc2.im = fma(w2[u], pr.im, acc2.im);
+                }
             }
         }
-        if (x >= P.oA0) rho += (double)re_mul(sZ[x], acc);
+        if (x >= P.oA0) {
+            rho += (double)re_mul(sZ[x], acc);
+            if (ND2) rho2 += (double)re_mul(sZ[x], acc2);
+        }
         lds_add(&sW[x].re, acc.re);
         lds_add(&sW[x].im, acc.im);
+        if (ND2) {
+            lds_add(&sW2[x].re, acc2.re);
+            lds_add(&sW2[x].im, acc2.im);
+        }
     }
     const double tot = block_sum<T>(rho, red);
+    double tot2 = 0.0;
+    if (ND2) {
+        __syncthreads();
+        tot2 = block_sum<T>(rho2, red + 4);
+    }
     if (tid == 0) {
-        // E_i = f_in(e_hc) F(rho) + e_hc (src/forces.F90:1104-1128)
-        const double r = tot + rho1[atom];
-        rho_out[atom] = r;
-        double F, dF;
-        embed(P.nfemb, r, P.p1a, P.p1b, F, dF);
+        // E_i = f_in(e_hc) sum_d F(rho_d) + e_hc (src/forces.F90:1066-1128)
+        const int nd = ND2 ? 2 : 1;
+        double Fs = 0.0, dF[2] = {0.0, 0.0};
+        for (int d = 0; d < nd; ++d) {
+            const double r = (d == 0 ? tot : tot2) + rho1[(size_t)atom * nd + d];
+            rho_out[(size_t)atom * nd + d] = r;
+            double F;
+            embed(P.nfemb, r, P.p1[d][0], P.p1[d][1], F, dF[d]);
+            Fs += F;
+        }
         const double e = ehc[atom], elow = P.ecut - P.decut;
         double fc = 1.0, dfc = 0.0;
         if (e >= P.ecut) { fc = 0.0; dfc = 0.0; }
@@ -529,30 +561,34 @@ __global__ void __launch_bounds__(256) k_adjoint(DevParams<T> P, const cplx<T>* 
             fc = 0.5 * (1.0 + cos(pi * (e - elow) / P.decut));
             dfc = -0.5 * sin(pi * (e - elow) / P.decut) * pi / P.decut;
         }
-        eout[atom] = fc * F + e;
-        dEout[atom] = fc * dF;
-        edfout[atom] = dfc * F + 1.0;
-        s_scale = fc * dF;
+        eout[atom] = fc * Fs + e;
+        for (int d = 0; d < nd; ++d) {
+            dEout[(size_t)atom * nd + d] = fc * dF[d];
+            s_scale[d] = fc * dF[d];
+        }
+        edfout[atom] = dfc * Fs + 1.0;
     }
     __syncthreads();
     // W_eff on representatives of conjugate pairs: since Z_xbar = s conj(Z_x),
     // Re(W_x dZ_x + W_xbar dZ_xbar) = Re((W_x + s conj(W_xbar)) dZ_x); zero elsewhere
     // (the reference's half-range loops with factor 2, src/forces.F90:460-614)
-    const T sc = (T)s_scale;
+    const T sc = (T)s_scale[0], sc2 = ND2 ? (T)s_scale[1] : (T)0;
     Wout += (size_t)atom * P.nz;
     for (int x = tid; x < P.nz; x += blockDim.x) {
         const uint32_t f = P.fold[x];
         cplx<T> w = {0, 0};
         if (f & (1u << 17)) {
             w = sW[x];
+            if (ND2) w = {sc * w.re + sc2 * sW2[x].re, sc * w.im + sc2 * sW2[x].im};
             if (!(f & (1u << 18))) {
-                const cplx<T> b = sW[f & 0xffff];
+                cplx<T> b = sW[f & 0xffff];
+                if (ND2) b = {sc * b.re + sc2 * sW2[f & 0xffff].re, sc * b.im + sc2 * sW2[f & 0xffff].im};
                 const T s = (f & (1u << 16)) ? (T)-1 : (T)1;
                 w.re += s * b.re;
                 w.im -= s * b.im;
             }
         }
-        Wout[x] = {sc * w.re, sc * w.im};
+        Wout[x] = ND2 ? w : cplx<T>{sc * w.re, sc * w.im};
     }
 }
 
@@ -838,12 +874,12 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
         const T* rowm = magnetic_row(P, mu_d, wm, unit, err);
         const T rsm = (T)P.rs_m;
         // B1: dE/dg^m_n = dE/drho c1_n
-        const double dE = dEin[atom];
+        double dE[2] = {dEin[(size_t)atom * P.nd], P.nd > 1 ? dEin[(size_t)atom * P.nd + 1] : 0.0};
         T s1 = 0;
         for (int n = 0; n < P.nb_m; ++n) {
             T v = unit ? (T)1 : (T)0, dv = 0;
             if (rowm) horner(rowm + 4 * n, wm, rsm, v, dv);
-            s1 += (T)(dE * P.c1[n]) * dv;
+            s1 += (T)(dE[0] * P.c1[0][n] + dE[1] * P.c1[1][n]) * dv;
         }
         T Tx = s1 * sx, Ty = s1 * sy, Tz = s1 * sz;
         cplx<T> phm[LM + 1];
@@ -1141,7 +1177,7 @@ __global__ void __launch_bounds__(256) k_fit_atom(DevParams<double> P, const cpl
     const double tot = block_sum<double>(s, red);
     if (threadIdx.x == 0) {
         double F, dF, d2F, fc, dfc;
-        embed2(P.nfemb, rho[atom], P.p1a, P.p1b, F, dF, d2F);
+        embed2(P.nfemb, rho[atom], P.p1[0][0], P.p1[0][1], F, dF, d2F);
         inner_cutoff(ehc[atom], P.ecut, P.decut, fc, dfc);
         const double sc = fc * dF;
         const double rho_dot = sc != 0.0 ? tot / sc : 0.0;
@@ -1210,7 +1246,7 @@ __global__ void k_fit_b1(DevParams<double> P, int natom, const double* __restric
     for (int n = 0; n < P.nb_m; ++n) {
         double v = unit ? 1.0 : 0.0, dv;
         if (row) horner(row + 4 * n, w, P.rs_m, v, dv);
-        atomicAdd(&grad[n], alpha[atom] * (v - 1.0));
+        atomicAdd(&grad[n], alpha[atom] * (v - 1.0));  // one density (fitting requires ndensity 1)
     }
 }
 
@@ -1348,20 +1384,22 @@ static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, co
     Pd.lhm = P.lhm; Pd.nm3 = P.nm3; Pd.nb_a = P.nb_a; Pd.nb_m = P.nb_m;
     const size_t lds1 = density_lds_bytes(Pd, CH, sizeof(T));
     auto k1 = k_density<T, L, CH, NB, NM3X, NB2, NR2X>;
-    auto k2 = k_adjoint<T>;
+    const bool nd2 = P.nd == 2;
+    const void* k2 = nd2 ? (const void*)k_adjoint<T, true> : (const void*)k_adjoint<T, false>;
     auto k3 = k_forces<T, L, L, NM3X, NBAX, NRAX, NR2X>;
     hipError_t e = hipSuccess;
     // the dynamic-LDS limit is raised once per kernel instantiation (to the workgroup
     // maximum), so that evaluations can be captured into graphs without further API calls
     static bool attributes_set = false;
-    static size_t limit[3] = {0, 0, 0};
+    static size_t limit[4] = {0, 0, 0, 0};
     if (!attributes_set) {
         int dev = 0, maxsh = 0;
         if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
         if ((e = hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev)) != hipSuccess)
             return e;
-        const void* ks[3] = {(const void*)k1, (const void*)k2, (const void*)k3};
-        for (int i = 0; i < 3; ++i) {
+        const void* ks[4] = {(const void*)k1, (const void*)k_adjoint<T, false>, (const void*)k3,
+                             (const void*)k_adjoint<T, true>};
+        for (int i = 0; i < 4; ++i) {
             hipFuncAttributes fa;
             if ((e = hipFuncGetAttributes(&fa, ks[i])) != hipSuccess) return e;
             limit[i] = (size_t)maxsh > fa.sharedSizeBytes ? (size_t)maxsh - fa.sharedSizeBytes : 0;
@@ -1371,16 +1409,20 @@ static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, co
         }
         attributes_set = true;
     }
-    const size_t lds2 = 2 * (size_t)(P.nz + 1) * sizeof(cplx<T>) + 8 * sizeof(double);
+    const size_t lds2 = (nd2 ? 3 : 2) * (size_t)(P.nz + 1) * sizeof(cplx<T>) + 8 * sizeof(double);
     const size_t lds3 = (size_t)P.nz * sizeof(cplx<T>);
-    if (lds1 > limit[0] || lds2 > limit[1] || lds3 > limit[2]) return hipErrorInvalidValue;
+    if (lds1 > limit[0] || lds2 > limit[nd2 ? 3 : 1] || lds3 > limit[2]) return hipErrorInvalidValue;
     if (ev) (void)hipEventRecord(ev[0], s);
     hipLaunchKernelGGL(k1, dim3(natom), dim3(256), lds1, s, P, off, dlist, elist, dmomi, emomi, dmom,
                        emom, (cplx<T>*)B.Z, B.ehc, B.rho1, B.err);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[1], s);
-    hipLaunchKernelGGL(k2, dim3(natom), dim3(256), lds2, s, P, (const cplx<T>*)B.Z, (cplx<T>*)B.W, B.ehc,
-                       B.rho1, eout, B.dE, B.edf, B.rho);
+    if (nd2)
+        hipLaunchKernelGGL((k_adjoint<T, true>), dim3(natom), dim3(256), lds2, s, P, (const cplx<T>*)B.Z,
+                           (cplx<T>*)B.W, B.ehc, B.rho1, eout, B.dE, B.edf, B.rho);
+    else
+        hipLaunchKernelGGL((k_adjoint<T, false>), dim3(natom), dim3(256), lds2, s, P, (const cplx<T>*)B.Z,
+                           (cplx<T>*)B.W, B.ehc, B.rho1, eout, B.dE, B.edf, B.rho);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(k3, dim3(natom), dim3(64), lds3, s, P, off, dlist, elist, dmomi, emomi, dmom, emom,

@@ -29,8 +29,11 @@ struct DevParams {
     const uint32_t* fold = nullptr;  // conjugate-pair folding of W (see TermList::fold)
     int nchunks = 0;
     int nfemb = 0;
-    double p1a = 1, p1b = 1, ecut = 0, decut = 1;
-    double c1[32] = {0};  // coefficients of B1_n = g^m_n(mu_i) - 1
+    int nd = 1;                  // densities (1 or 2)
+    const T* cw2 = nullptr;      // record weights of density 2 (same order as crec)
+    double p1[2][2] = {{1, 1}, {1, 1}};  // per density: embedding prefactor, exponent
+    double ecut = 0, decut = 1;
+    double c1[2][32] = {{0}};    // per density: coefficients of B1_n = g^m_n(mu_i) - 1
 };
 
 // Error bits raised by kernels (reported by the host API).
@@ -44,10 +47,10 @@ struct Buffers {
     void* Z = nullptr;       // [natom][nz] complex
     void* W = nullptr;       // [natom][nz] complex
     double* ehc = nullptr;   // core sum
-    double* rho1 = nullptr;  // B1 part of rho
-    double* dE = nullptr;    // fcut * F'(rho)
+    double* rho1 = nullptr;  // B1 part of rho, [natom][nd]
+    double* dE = nullptr;    // fcut * F'(rho), [natom][nd]
     double* edf = nullptr;   // d f_in/d e_hc * F + 1
-    double* rho = nullptr;   // rho_i (including the B1 part)
+    double* rho = nullptr;   // rho_i (including the B1 part), [natom][nd]
     int* err = nullptr;
     size_t cap_atoms = 0;
 };

@@ -37,10 +37,11 @@ struct Basis {
 struct Potential {
     Header h;
     Basis basis;
-    std::vector<double> c1, c2, c3, c4;  // expansion coefficients, one per B function
+    // expansion coefficients, one per B function and density: c[d * nvar + k]
+    std::vector<double> c1, c2, c3, c4;
     std::vector<double> crad_a;          // [nr3_a][l3a+1][nb_a]
     std::vector<double> crad_m;          // [nr2_m][l2m+1][nb_m]
-    double p1[2] = {1.0, 1.0};           // embedding prefactor / exponent
+    double p1[2][2] = {{1.0, 1.0}, {1.0, 1.0}};  // per density: embedding prefactor, exponent
     double core[2] = {3.0, 3.0};         // core repulsion prefactor / exponent
     double rcut = 0, rcut_m = 0;         // global cutoffs from acelibini_
     std::string element = "Fe", xc = "pbe";
@@ -69,9 +70,9 @@ struct ZLayout {
 struct TermList {
     static constexpr int kChunk = 32;
     std::vector<uint16_t> idx;   // 4 per term (slot 0 = A0 factor)
-    std::vector<double> w;
+    std::vector<double> w;       // merged weights, density-major: w[d * nterms + t]
     std::vector<uint16_t> cidx;  // 4 per contribution: the three other factors + padding
-    std::vector<double> cw;      // weight per contribution
+    std::vector<double> cw;      // weight per contribution, density-major (d * ncontrib + r)
     std::vector<int32_t> chunks; // 2 per chunk: first record, target | (count << 16); records of
                                  // a chunk are 64 apart (wave-interleaved layout)
     // folding of the gradient onto representatives of conjugate pairs (x, xbar):
@@ -86,6 +87,9 @@ struct TermList {
     std::vector<int32_t> rec_t;  // monomial of each gradient record (-1: padding)
     int n2 = 0, n3 = 0, n4 = 0;  // raw (before merging) m-combination counts per order
     int raw = 0;
+    int nd = 1;                  // densities
+    size_t nterms() const { return idx.size() / 4; }
+    size_t ncontrib() const { return rec_t.size(); }
 };
 
 // Spline tables in node-major layout: row n holds [4 coeffs] for every function family.

@@ -101,17 +101,18 @@ bool parse_potential(const std::string& path, const std::string& element, int xc
 
     pot.basis = enumerate_basis(h);
     const Basis& B = pot.basis;
-    pot.c1.assign(B.b1.size(), 0.0);
-    pot.c2.assign(B.b2.size(), 0.0);
-    pot.c3.assign(B.b3.size(), 0.0);
-    pot.c4.assign(B.b4.size(), 0.0);
+    const int ndn = h.ndensity;
+    pot.c1.assign(B.b1.size() * ndn, 0.0);
+    pot.c2.assign(B.b2.size() * ndn, 0.0);
+    pot.c3.assign(B.b3.size() * ndn, 0.0);
+    pot.c4.assign(B.b4.size() * ndn, 0.0);
     pot.crad_a.assign((size_t)h.nr3_a * (h.l3a + 1) * h.nb_a, 0.0);
     for (int n = 0; n < h.nr3_a; ++n)
         for (int l = 0; l <= h.l3a; ++l) pot.crad_a[((size_t)n * (h.l3a + 1) + l) * h.nb_a + n] = 1.0;
     pot.crad_m.assign((size_t)h.nr2_m * (h.l2m + 1) * h.nb_m, 0.0);
     for (int n = 0; n < h.nr2_m; ++n)
         for (int l = 0; l <= h.l2m; ++l) pot.crad_m[((size_t)n * (h.l2m + 1) + l) * h.nb_m + n] = 1.0;
-    pot.p1[0] = pot.p1[1] = 1.0;
+    pot.p1[0][0] = pot.p1[0][1] = pot.p1[1][0] = pot.p1[1][1] = 1.0;
     pot.core[0] = pot.core[1] = 3.0;
 
     // index tuple -> B function positions (a tuple can only occur once per order)
@@ -119,10 +120,10 @@ bool parse_potential(const std::string& path, const std::string& element, int xc
     for (size_t k = 0; k < B.b2.size(); ++k) m2.emplace(tuple_key(B.b2[k]), k);
     for (size_t k = 0; k < B.b3.size(); ++k) m3.emplace(tuple_key(B.b3[k]), k);
     for (size_t k = 0; k < B.b4.size(); ++k) m4.emplace(tuple_key(B.b4[k]), k);
-    auto set_match = [](auto& map, const auto& basis, const auto& tup, std::vector<double>& c, double v) {
+    auto set_match = [](auto& map, const auto& basis, const auto& tup, std::vector<double>& c, size_t o, double v) {
         auto rng = map.equal_range(tuple_key(tup));
         for (auto it = rng.first; it != rng.second; ++it)
-            if (basis[it->second] == tup) c[it->second] = v;
+            if (basis[it->second] == tup) c[o + it->second] = v;
     };
 
     while (std::getline(in, line)) {
@@ -150,23 +151,24 @@ bool parse_potential(const std::string& path, const std::string& element, int xc
             const double v = to_d(p[p.size() - 2]);
             if (norder == 1 && ni >= 1) {
                 for (size_t k = 0; k < B.b1.size(); ++k)
-                    if (B.b1[k][0] == iv[0]) pot.c1[k] = v;
+                    if (B.b1[k][0] == iv[0]) pot.c1[(nd - 1) * B.b1.size() + k] = v;
             } else if (norder == 2 && ni >= 4) {
                 std::array<int, 4> a;
                 std::copy(iv.begin(), iv.begin() + 4, a.begin());
-                set_match(m2, B.b2, a, pot.c2, v);
+                set_match(m2, B.b2, a, pot.c2, (nd - 1) * B.b2.size(), v);
             } else if (norder == 3 && ni >= 9) {
                 std::array<int, 9> a;
                 std::copy(iv.begin(), iv.begin() + 9, a.begin());
-                set_match(m3, B.b3, a, pot.c3, v);
+                set_match(m3, B.b3, a, pot.c3, (nd - 1) * B.b3.size(), v);
             } else if (norder == 4 && ni >= 15) {
                 std::array<int, 15> a;
                 std::copy(iv.begin(), iv.begin() + 15, a.begin());
-                set_match(m4, B.b4, a, pot.c4, v);
+                set_match(m4, B.b4, a, pot.c4, (nd - 1) * B.b4.size(), v);
             }
         } else if (type == "par") {
             if (norder < 1 || norder > 2) { err = "norder < nvarp1 required for par."; return false; }
-            if (p.size() >= 2 && p[0] == element && nd >= 1 && nd <= h.ndensity) pot.p1[norder - 1] = to_d(p[1]);
+            if (p.size() >= 2 && p[0] == element && nd >= 1 && nd <= h.ndensity)
+                pot.p1[nd - 1][norder - 1] = to_d(p[1]);
         } else if (type == "rad_atomic" || type == "rad_magnetic") {
             if (p.size() < 6) continue;
             if (p[0] != element || p[1] != element) { err = "Error when reading radial parameters."; return false; }
@@ -203,7 +205,10 @@ bool write_potential(const std::string& path, const Potential& pot, std::string&
     const Header& h = pot.h;
     const std::string& E = pot.element;
     for (const auto& l : pot.header_lines) out << l << "\n";
-    for (int k = 0; k < 2; ++k) out << " par " << k + 1 << " 1\n " << E << " " << pot.p1[k] << " F\n";
+    const int nd = h.ndensity;
+    for (int d = 0; d < nd; ++d)
+        for (int k = 0; k < 2; ++k)
+            out << " par " << k + 1 << " " << d + 1 << "\n " << E << " " << pot.p1[d][k] << " F\n";
     for (int k = 0; k < 2; ++k)
         out << " core " << k + 1 << " 0\n " << E << " " << E << " " << std::fabs(pot.core[k]) << " F\n";
     for (int nb = 1; nb <= h.nb_a; ++nb)
@@ -216,16 +221,19 @@ bool write_potential(const std::string& path, const Potential& pot, std::string&
             for (int l = 0; l <= h.l2m; ++l)
                 out << " rad_magnetic 0 0\n " << E << " " << E << " " << nb << " " << nr << " " << l << " "
                     << pot.crad_m[((size_t)(nr - 1) * (h.l2m + 1) + l) * h.nb_m + (nb - 1)] << " T\n";
-    auto clu = [&](int order, const int* idx, int n, double c) {
-        out << " clu " << order << " 1\n";
+    auto clu = [&](int order, const int* idx, int n, double c, int d) {
+        out << " clu " << order << " " << d + 1 << "\n";
         for (int i = 0; i < order; ++i) out << " " << E;
         for (int i = 0; i < n; ++i) out << " " << idx[i];
         out << " " << c << " T\n";
     };
-    for (size_t k = 0; k < pot.basis.b1.size(); ++k) clu(1, pot.basis.b1[k].data(), 1, pot.c1[k]);
-    for (size_t k = 0; k < pot.basis.b2.size(); ++k) clu(2, pot.basis.b2[k].data(), 4, pot.c2[k]);
-    for (size_t k = 0; k < pot.basis.b3.size(); ++k) clu(3, pot.basis.b3[k].data(), 9, pot.c3[k]);
-    for (size_t k = 0; k < pot.basis.b4.size(); ++k) clu(4, pot.basis.b4[k].data(), 15, pot.c4[k]);
+    const Basis& B = pot.basis;
+    for (int d = 0; d < nd; ++d) {
+        for (size_t k = 0; k < B.b1.size(); ++k) clu(1, B.b1[k].data(), 1, pot.c1[d * B.b1.size() + k], d);
+        for (size_t k = 0; k < B.b2.size(); ++k) clu(2, B.b2[k].data(), 4, pot.c2[d * B.b2.size() + k], d);
+        for (size_t k = 0; k < B.b3.size(); ++k) clu(3, B.b3[k].data(), 9, pot.c3[d * B.b3.size() + k], d);
+        for (size_t k = 0; k < B.b4.size(); ++k) clu(4, B.b4[k].data(), 15, pot.c4[d * B.b4.size() + k], d);
+    }
     if (!out) {
         err = "write failed: " + path;
         return false;

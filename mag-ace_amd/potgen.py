@@ -32,6 +32,9 @@ H2000 = dict(BASE, nb_a=10, nb_m=3, nr2_m=2, l2m=6, nr3_a=7, nr3_m=1, l3a=6, l3m
 #: embedding exponent
 HFE = dict(BASE, nb_a=16, nb_m=16, nr2_m=5, l2m=2, nr3_a=4, nr3_m=3, l3a=1, l3m=2,
            nr4_a=3, nr4_m=3, l4a=0, l4m=0, nfemb=1, p1a=1.0, p1b=0.75)
+#: lmax-4 header with two densities, E = F(rho_1) + F(rho_2) (src/forces.F90:1066-1085), a
+#: nonlinear embedding on the second density
+H500ND2 = dict(H500, ndensity=2, nfemb=1, p1a=1.0, p1b=1.0, p1a_2=0.7, p1b_2=0.6)
 #: tiny header for fast unit tests
 HTINY = dict(BASE, nb_a=4, nb_m=2, nr2_m=2, l2m=2, nr3_a=2, nr3_m=1, l3a=2, l3m=1,
              nr4_a=2, nr4_m=1, l4a=2, l4m=1)
@@ -89,6 +92,19 @@ def render(header: dict, basis: dict, seed: int, element: str = "Fe", xc: str = 
             L.append(" clu %d 1" % order)
             L.append(" %s %s %s T" % (els, " ".join(str(int(v)) for v in row),
                                       _f(rng.normal(0.0, c_scale))))
+    # further densities (ndensity 2): their par and clu records follow, so that files with one
+    # density are unchanged
+    for nd in range(2, int(h.get("ndensity", 1)) + 1):
+        for k, key in ((1, "p1a_%d" % nd), (2, "p1b_%d" % nd)):
+            L.append(" par %d %d" % (k, nd))
+            L.append(" %s %s F" % (element, _f(h.get(key, 1.0))))
+        for order in (1, 2, 3, 4):
+            idx = np.asarray(basis.get(order, np.zeros((0, 1), np.int32)))
+            els = " ".join([element] * order)
+            for row in idx:
+                L.append(" clu %d %d" % (order, nd))
+                L.append(" %s %s %s T" % (els, " ".join(str(int(v)) for v in row),
+                                          _f(rng.normal(0.0, c_scale))))
     return "\n".join(L) + "\n"
 
 

@@ -58,3 +58,25 @@ def test_written_file_reads_in_the_oracle_and_the_reference(tmp_path):
     if os.path.exists(O.REF_DRIVER):
         ref = O.RefDriver(out).run(a)
         np.testing.assert_allclose(ref["eout"], e_w, rtol=1e-10, atol=1e-14)
+
+
+def test_two_densities_write_roundtrip(tmp_path):
+    """ndensity 2: per-density coefficients and embedding parameters survive the writer, and
+    the oracle and the reference read the written file to the same energies."""
+    h = HEADERS["H500ND2"]
+    pot = str(tmp_path / "Fe.pbe.in")
+    with open(pot, "w") as f:
+        f.write(potgen.render(h, capi.enumerate_basis(h), 2052))
+    capi.init_file(pot)
+    c = capi.fit_get_coefficients()
+    assert len(c) == 2 * sum(capi.basis_counts())
+    out = str(tmp_path / "w.pbe.in")
+    capi.write_potential(out)
+    capi.init_file(out)
+    assert np.array_equal(capi.fit_get_coefficients(), c)
+    s = systems.make_system(2, 4, "uniform")
+    a = systems.pack_acelib(s["pos"], s["cell"], s["sdir"], s["smag"], 5.0)
+    e0 = O.Oracle.load(pot).acelib(a)["eout"]
+    np.testing.assert_allclose(O.Oracle.load(out).acelib(a)["eout"], e0, rtol=1e-13, atol=1e-14)
+    if os.path.exists(O.REF_DRIVER):
+        np.testing.assert_allclose(O.RefDriver(out).run(a)["eout"], e0, rtol=1e-10, atol=1e-14)
