@@ -738,14 +738,27 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
 #pragma unroll
                 for (int m = 1; m <= LA; ++m) pha[m] = cmul(pha[m - 1], cplx<T>{ex, ey});
                 LegRows<T, LA> Lr;
+                // spline coefficients of row l are loaded one iteration ahead, so the gathers
+                // from the table overlap the arithmetic of the previous l
+                T cq[NRAX][4];
+#pragma unroll
+                for (int n = 0; n < NRAX; ++n)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) cq[n][k] = n < P.nr3a ? rowa[4 * (P.nb_a + n) + k] : (T)0;
                 for (int l = 0; l <= P.l3a; ++l) {
                     Lr.step(l, ez);
                     T R[NRAX], dR[NRAX];
 #pragma unroll
                     for (int n = 0; n < NRAX; ++n) {
-                        R[n] = 0;
-                        dR[n] = 0;
-                        if (n < P.nr3a) horner(rowa + 4 * (P.nb_a + l * P.nr3a + n), wa, rsa, R[n], dR[n]);
+                        R[n] = fma(wa, fma(wa, fma(wa, cq[n][3], cq[n][2]), cq[n][1]), cq[n][0]);
+                        dR[n] = fma(wa, fma(wa, (T)3 * cq[n][3], (T)2 * cq[n][2]), cq[n][1]) * rsa;
+                    }
+                    if (l < P.l3a) {
+                        const T* nrow = rowa + 4 * (P.nb_a + (l + 1) * P.nr3a);
+#pragma unroll
+                        for (int n = 0; n < NRAX; ++n)
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) cq[n][k] = n < P.nr3a ? nrow[4 * n + k] : (T)0;
                     }
 #pragma unroll
                     for (int am = 0; am <= LA; ++am) {
