@@ -1201,7 +1201,9 @@ __global__ void __launch_bounds__(256) k_fit_atom(DevParams<double> P, const cpl
 }
 
 // M_t += sum_i alpha_i P_t,i + beta_i D_u P_t,i over a chunk of atoms (Z, Zdot of one atom in
-// LDS at a time); grid = (monomial tiles, atom chunks)
+// LDS at a time); each thread owns kFitMpt monomials so that an atom's Z and Zdot are staged
+// once per kFitMpt * 256 monomials; grid = (monomial tiles, atom chunks)
+constexpr int kFitMpt = 8;
 __global__ void __launch_bounds__(256) k_fit_accum(int nterms, int nz, const ushort4* __restrict__ idx,
                                                   const cplx<double>* __restrict__ Z,
                                                   const cplx<double>* __restrict__ Zdot,
@@ -1210,11 +1212,16 @@ __global__ void __launch_bounds__(256) k_fit_accum(int nterms, int nz, const ush
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<double>* sZ = (cplx<double>*)smem;  // [nz + 1]
     cplx<double>* sD = sZ + (nz + 1);          // [nz + 1]
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    ushort4 ix = {0, 0, 0, 0};
-    if (t < nterms) ix = idx[t];
+    const int t0 = blockIdx.x * blockDim.x * kFitMpt + threadIdx.x;
+    ushort4 ix[kFitMpt];
+    double accr[kFitMpt], acci[kFitMpt];
+#pragma unroll
+    for (int j = 0; j < kFitMpt; ++j) {
+        const int t = t0 + j * blockDim.x;
+        ix[j] = t < nterms ? idx[t] : ushort4{0, 0, 0, 0};
+        accr[j] = acci[j] = 0.0;
+    }
     const int a0 = blockIdx.y * chunk, a1 = min(natom, a0 + chunk);
-    double accr = 0.0, acci = 0.0;
     for (int a = a0; a < a1; ++a) {
         __syncthreads();
         for (int x = threadIdx.x; x < nz; x += blockDim.x) {
@@ -1226,9 +1233,12 @@ __global__ void __launch_bounds__(256) k_fit_accum(int nterms, int nz, const ush
             sD[nz] = {0, 0};
         }
         __syncthreads();
-        if (t < nterms) {
-            const cplx<double> z0 = sZ[ix.x], z1 = sZ[ix.y], z2 = sZ[ix.z], z3 = sZ[ix.w];
-            const cplx<double> d0 = sD[ix.x], d1 = sD[ix.y], d2 = sD[ix.z], d3 = sD[ix.w];
+        const double al = alpha[a], be = beta[a];
+#pragma unroll
+        for (int j = 0; j < kFitMpt; ++j) {
+            if (t0 + j * (int)blockDim.x >= nterms) break;
+            const cplx<double> z0 = sZ[ix[j].x], z1 = sZ[ix[j].y], z2 = sZ[ix[j].z], z3 = sZ[ix[j].w];
+            const cplx<double> d0 = sD[ix[j].x], d1 = sD[ix[j].y], d2 = sD[ix[j].z], d3 = sD[ix[j].w];
             const cplx<double> z01 = cmul(z0, z1), z23 = cmul(z2, z3);
             const cplx<double> Pt = cmul(z01, z23);
             // product rule: d(z0 z1 z2 z3) = (d0 z1 + z0 d1) z23 + z01 (d2 z3 + z2 d3)
@@ -1237,14 +1247,17 @@ __global__ void __launch_bounds__(256) k_fit_accum(int nterms, int nz, const ush
             cfma(r, z2, d3);
             cplx<double> Pd = cmul(l, z23);
             cfma(Pd, z01, r);
-            const double al = alpha[a], be = beta[a];
-            accr += al * Pt.re + be * Pd.re;
-            acci += al * Pt.im + be * Pd.im;
+            accr[j] += al * Pt.re + be * Pd.re;
+            acci[j] += al * Pt.im + be * Pd.im;
         }
     }
-    if (t < nterms) {
-        atomicAdd(&M[2 * (size_t)t], accr);
-        atomicAdd(&M[2 * (size_t)t + 1], acci);
+#pragma unroll
+    for (int j = 0; j < kFitMpt; ++j) {
+        const int t = t0 + j * blockDim.x;
+        if (t < nterms) {
+            atomicAdd(&M[2 * (size_t)t], accr[j]);
+            atomicAdd(&M[2 * (size_t)t + 1], acci[j]);
+        }
     }
 }
 
@@ -1334,9 +1347,9 @@ hipError_t fit_backward(const DevParams<double>& P, int cls, int natom, const in
                         const double* dmom, const double* emom, const int* atom_struct, const double* gE,
                         Buffers& B, FitBuffers& F, int nterms, int nb234, double* grad, hipStream_t s) {
     if (natom <= 0) return hipSuccess;
-    hipError_t e = cls == 0 ? run_jvp_class<6, 16, 4, 4, 4, 2>(P, natom, off, nlist, du, dlist, elist, dmom, emom,
+    hipError_t e = cls == 0 ? run_jvp_class<6, 32, 4, 4, 4, 2>(P, natom, off, nlist, du, dlist, elist, dmom, emom,
                                                                (cplx<double>*)F.Zdot, F.ehc_dot, B.err, s)
-                            : run_jvp_class<2, 16, 1, 27, 2, 5>(P, natom, off, nlist, du, dlist, elist, dmom, emom,
+                            : run_jvp_class<2, 32, 1, 27, 2, 5>(P, natom, off, nlist, du, dlist, elist, dmom, emom,
                                                                 (cplx<double>*)F.Zdot, F.ehc_dot, B.err, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_fit_atom, dim3(natom), dim3(256), 0, s, P, (const cplx<double>*)B.W,
@@ -1347,7 +1360,7 @@ hipError_t fit_backward(const DevParams<double>& P, int cls, int natom, const in
     hipLaunchKernelGGL(k_fill_double, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s, grad, ng);
     const int chunk = 64;
     const size_t lds = 2 * (size_t)(P.nz + 1) * sizeof(cplx<double>);
-    dim3 grid((unsigned)((nterms + 255) / 256), (unsigned)((natom + chunk - 1) / chunk));
+    dim3 grid((unsigned)((nterms + 256 * kFitMpt - 1) / (256 * kFitMpt)), (unsigned)((natom + chunk - 1) / chunk));
     hipLaunchKernelGGL(k_fit_accum, grid, dim3(256), lds, s, nterms, P.nz, (const ushort4*)F.idx,
                        (const cplx<double>*)B.Z, (const cplx<double>*)F.Zdot, F.alpha, F.beta, natom, chunk, F.M);
     hipLaunchKernelGGL(k_fit_b1, dim3((natom + 255) / 256), dim3(256), 0, s, P, natom, dmomi, F.alpha, grad, B.err);
