@@ -242,6 +242,15 @@ def main():
         log("%s: %.3f ms/step, %.3e atom-steps/s, kernels [density, adjoint, forces] ms %s" %
             (prec, res[prec]["ms_per_step"], res[prec]["value"], np.round(kms / max(cnt, 1), 4).tolist()))
 
+    # the boundary as LAMMPS uses it: acelib_ on host arrays (copies over PCIe included)
+    capi.set_precision(capi.MIXED)
+    capi.acelib(a)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        capi.acelib(a)
+    host_ms = 1e3 * (time.perf_counter() - t0) / 3
+    log("acelib_ on host arrays (mixed): %.3f ms/call, %.3e atom-steps/s" % (host_ms, N / (1e-3 * host_ms)))
+
     # neighbour rebuild on the device (same pair set and order as the reference neilist)
     neighbors = importlib.import_module("magace_amd.neighbors")
     pos_t = T(s["pos"])
@@ -328,6 +337,8 @@ def main():
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"},
         "cpu_baseline": cpu,
         "neighbor_build_ms": nb_ms,
+        "acelib_host_arrays": {"ms_per_call": host_ms, "atom_steps_per_s": N / (1e-3 * host_ms),
+                               "note": "acelib_ with host arrays, PCIe copies included (mixed)"},
         "spin_lattice_md": mdres,
         "batched_fit_step": fitres,
         "value_with_rebuild_every_10": N / (1e-3 * (res["mixed"]["ms_per_step"] + nb_ms / 10.0)),
