@@ -19,7 +19,8 @@ LIB = os.path.join(HERE, "libmagace.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MAGACE_ARCH", "gfx950")
 
-SOURCES = ["device.hip", "neighbors.hip", "md.hip", "api.cpp", "basis.cpp", "potential.cpp", "splines.cpp", "pair_host.cpp"]
+SOURCES = ["device.hip", "neighbors.hip", "md.hip", "api.cpp", "basis.cpp", "potential.cpp", "splines.cpp", "pair_host.cpp",
+           "host_pool.cpp"]
 # neighbour lists reproduce the reference's FP64 values bit for bit: no contraction
 EXTRA = {"neighbors.hip": ["-ffp-contract=off"]}
 # packed FP32 instructions are disabled on purpose (the compiler would otherwise form
@@ -45,7 +46,7 @@ def build(jobs: int = 4) -> str:
     os.makedirs(OUT, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, SOURCES))
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", LIB + ".tmp"] + objs
+    cmd = [HIPCC, "-shared", "-fPIC", "-pthread", "--offload-arch=" + ARCH, "-o", LIB + ".tmp"] + objs
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (p.stdout, p.stderr))

@@ -108,16 +108,24 @@ def finalize():
     lib().magace_finalize()
 
 
-def acelib(a: dict):
-    """Call acelib_ on host arrays in the pair_spin_ace layout (see systems.pack_acelib)."""
+def acelib(a: dict, out: dict | None = None):
+    """Call acelib_ on host arrays in the pair_spin_ace layout (see systems.pack_acelib).
+
+    `out`, if given, is a dict returned by an earlier call on a list of the same size; its
+    arrays are written in place (a caller like LAMMPS keeps its output arrays between steps)."""
     L = lib()
     natom = int(len(a["nstart"]))
     nei = int(len(a["dlist"]))
     nall = int(a.get("nall", natom))
-    eout = np.zeros(max(natom, 1))
-    flist = np.zeros(max(3 * nei, 1))
-    tlist = np.zeros(max(3 * nei, 1))
-    tlist0 = np.zeros(max(3 * natom, 1))
+    if out is not None:
+        eout, flist, tlist, tlist0 = out["_buffers"]
+        if len(eout) != max(natom, 1) or len(flist) != max(3 * nei, 1):
+            raise ValueError("out was made for a list of another size")
+    else:
+        eout = np.zeros(max(natom, 1))
+        flist = np.zeros(max(3 * nei, 1))
+        tlist = np.zeros(max(3 * nei, 1))
+        tlist0 = np.zeros(max(3 * natom, 1))
     ci = lambda v: C.byref(C.c_int(v))
     pad_i = lambda x: np.ascontiguousarray(x if len(x) else np.zeros(1), dtype=np.int32)
     pad_d = lambda x: np.ascontiguousarray(x if len(x) else np.zeros(1), dtype=np.float64)
@@ -126,7 +134,8 @@ def acelib(a: dict):
               pad_d(a["dlist_momi"]), pad_d(a["elist_momi"]), pad_d(a["dlist_mom"]), pad_d(a["elist_mom"]),
               tlist, tlist0)
     _check()
-    return dict(eout=eout[:natom], flist=flist[:3 * nei], tlist=tlist[:3 * nei], tlist0=tlist0[:3 * natom])
+    return dict(eout=eout[:natom], flist=flist[:3 * nei], tlist=tlist[:3 * nei], tlist0=tlist0[:3 * natom],
+                _buffers=(eout, flist, tlist, tlist0))
 
 
 def compute_device(natom, nei, offsets, dlist, elist, dmomi, emomi, dmom, emom, eout, flist, tlist,

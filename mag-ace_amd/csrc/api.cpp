@@ -8,10 +8,13 @@
 // in HBM.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <array>
+#include <atomic>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -19,6 +22,7 @@
 
 #include "../../include/magace.h"
 #include "device.hpp"
+#include "host_pool.hpp"
 #include "magace.hpp"
 
 namespace magace {
@@ -75,14 +79,22 @@ struct Context {
     int2* chunks = nullptr;
     uint32_t* fold = nullptr;
     Buffers buf;
-    // staging for acelib_ host arrays
-    size_t cap_pairs = 0, cap_atoms = 0;
-    int* d_off = nullptr;
-    double *d_dl = nullptr, *d_el = nullptr, *d_dm = nullptr, *d_em = nullptr;
-    double *d_dmi = nullptr, *d_emi = nullptr;
-    double *d_eout = nullptr, *d_fl = nullptr, *d_tl = nullptr, *d_t0 = nullptr;
     std::vector<int> h_off;
     hipStream_t stream = nullptr;
+    // acelib_ on host arrays runs in chunks of whole atoms.  Each of two slots holds one
+    // chunk: its inputs, packed by the host into pinned memory and copied in one transfer to
+    // the device, and its outputs, copied back in one transfer.  While the kernels run on
+    // one slot, the copies and the host's packing and unpacking work on the other.
+    struct Slot {
+        size_t cap_in = 0, cap_out = 0;  // bytes
+        char* h_in = nullptr;            // pinned: chunk inputs (layout in ChunkLayout)
+        char* d_in = nullptr;            // device copy of h_in
+        char* d_out = nullptr;           // device: eout | tlist0 | flist | tlist
+        char* h_out = nullptr;           // pinned copy of d_out
+        hipEvent_t ev_in = nullptr, ev_comp = nullptr, ev_done = nullptr;
+    };
+    std::array<Slot, 2> slots;
+    hipStream_t s_in = nullptr, s_out = nullptr;
     bool uploaded = false;
     // fitting: adjoint buffers, pair outputs of the fit forward, device coefficient structure
     FitBuffers fit;
@@ -123,8 +135,17 @@ Context::~Context() {
     dfree(fit.Zdot); dfree(fit.ehc_dot); dfree(fit.alpha); dfree(fit.beta); dfree(fit.M); dfree(fit.idx);
     dfree(fit.ub_ptr); dfree(fit.ub_t); dfree(fit.ub_u);
     dfree(f_fl); dfree(f_tl); dfree(f_t0); dfree(f_eout);
-    dfree(d_off); dfree(d_dl); dfree(d_el); dfree(d_dm); dfree(d_em); dfree(d_dmi); dfree(d_emi);
-    dfree(d_eout); dfree(d_fl); dfree(d_tl); dfree(d_t0);
+    for (auto& sl : slots) {
+        dfree(sl.d_in);
+        dfree(sl.d_out);
+        if (sl.h_in) (void)hipHostFree(sl.h_in);
+        if (sl.h_out) (void)hipHostFree(sl.h_out);
+        if (sl.ev_in) (void)hipEventDestroy(sl.ev_in);
+        if (sl.ev_comp) (void)hipEventDestroy(sl.ev_comp);
+        if (sl.ev_done) (void)hipEventDestroy(sl.ev_done);
+    }
+    if (s_in) (void)hipStreamDestroy(s_in);
+    if (s_out) (void)hipStreamDestroy(s_out);
     for (auto& e : events)
         for (auto& x : e) (void)hipEventDestroy(x);
     if (stream) (void)hipStreamDestroy(stream);
@@ -321,10 +342,11 @@ int ensure_atoms(Context& c, size_t natom) {
 
 int run_device(Context& c, int natom, const int* off, const double* dlist, const double* elist,
                const double* dmomi, const double* emomi, const double* dmom, const double* emom,
-               double* eout, double* flist, double* tlist, double* tlist0, hipStream_t s) {
+               double* eout, double* flist, double* tlist, double* tlist0, hipStream_t s,
+               bool clear_err = true) {
     if (int rc = upload(c)) return rc;
     if (int rc = ensure_atoms(c, (size_t)natom)) return rc;
-    HIPCHK(clear_flags(c.buf.err, s));
+    if (clear_err) HIPCHK(clear_flags(c.buf.err, s));
     hipEvent_t* ev = nullptr;
     if (c.profiling) {
         if (c.events_used == c.events.size()) {
@@ -343,6 +365,63 @@ int run_device(Context& c, int natom, const int* off, const double* dlist, const
                                tlist0, c.buf, s, c.shape_class, ev);
     HIPCHK(e);
     return 0;
+}
+
+// pairs per acelib_ chunk: MAGACE_ACELIB_CHUNK_PAIRS if set (1024 .. 2^30), else 393216; read
+// at every call.  Each of the two slots holds, in pinned host memory and again on the
+// device, 112 bytes per pair and 68 per atom of one chunk (44 MB at the default).
+int chunk_pairs() {
+    if (const char* e = std::getenv("MAGACE_ACELIB_CHUNK_PAIRS")) {
+        const long v = std::atol(e);
+        if (v >= 1024 && v <= (1L << 30)) return (int)v;
+    }
+    return 393216;
+}
+
+// Byte offsets of one acelib_ chunk of na atoms and np pairs.  Inputs: dlist, elist,
+// dlist_mom, elist_mom, dlist_momi, elist_momi, then the chunk's pair offsets (na + 1 ints,
+// relative to its first pair); outputs: eout, tlist0, flist, tlist.
+struct ChunkLayout {
+    size_t na, np;
+    size_t i_dl, i_el, i_dm, i_em, i_dmi, i_emi, i_off, in_bytes;
+    size_t o_eout, o_t0, o_fl, o_tl, out_bytes;
+    ChunkLayout(size_t na_, size_t np_) : na(na_), np(np_) {
+        i_dl = 0;
+        i_el = i_dl + 8 * np;
+        i_dm = i_el + 24 * np;
+        i_em = i_dm + 8 * np;
+        i_dmi = i_em + 24 * np;
+        i_emi = i_dmi + 8 * na;
+        i_off = i_emi + 24 * na;
+        in_bytes = i_off + 4 * (na + 1);
+        o_eout = 0;
+        o_t0 = o_eout + 8 * na;
+        o_fl = o_t0 + 24 * na;
+        o_tl = o_fl + 24 * np;
+        out_bytes = o_tl + 24 * np;
+    }
+};
+
+// Atom boundaries of the chunks (cut[0] = 0, ..., cut.back() = natom).  Chunks hold whole
+// atoms and at most cmax pairs (unless one atom has more).  The first chunks grow from
+// cmax/4 and the last ones shrink by halves down to cmax/4, so that the host's first packing
+// and last unpacking, which nothing overlaps, are short; smaller chunks would cost more in
+// kernel latency than they save (each chunk's three kernels take 0.3 ms at least).
+std::vector<int> chunk_plan(const std::vector<int>& off, int cmax) {
+    const int n = (int)off.size() - 1;
+    const long total = off[n];
+    const long base = std::max(1L, (long)cmax / 4);
+    std::vector<int> cut{0};
+    for (int i = 0, k = 0; i < n; ++k) {
+        const long rest = total - off[i];
+        long target = std::min<long>(cmax, base << std::min(k, 20));
+        target = std::min(target, std::max(base, (rest + 1) / 2));
+        const int a0 = i;
+        ++i;
+        while (i < n && off[i + 1] - off[a0] <= target) ++i;
+        cut.push_back(i);
+    }
+    return cut;
 }
 
 int check_kernel_errors(Context& c, hipStream_t s) {
@@ -486,11 +565,24 @@ void acelib_(int* natom, int* nall, int* nei, int* nlist, int* occ, int* occlist
             set_error(MAGACE_ERR_INPUT, "Cannot handle multiple species.");
             return;
         }
-    for (int p = 0; p < P; ++p)
-        if (occlist[p] != 1) {
+    {
+        // occlist is as long as the pair arrays: checked by the copy threads
+        HostPool& pool = HostPool::get();
+        const int nt = std::min(4 * pool.size(), P / 65536 + 1);
+        std::atomic<bool> other{false};
+        pool.run(nt, [&](int t) {
+            const long lo = (long)P * t / nt, hi = (long)P * (t + 1) / nt;
+            for (long p = lo; p < hi; ++p)
+                if (occlist[p] != 1) {
+                    other = true;
+                    return;
+                }
+        });
+        if (other) {
             set_error(MAGACE_ERR_INPUT, "Cannot handle multiple species.");
             return;
         }
+    }
     // pair ranges must tile 1..nei in atom order, as pair_spin_ace builds them
     // (lammps/pair_spin_ace.cpp:457,524)
     c.h_off.resize(N + 1);
@@ -513,44 +605,94 @@ void acelib_(int* natom, int* nall, int* nei, int* nlist, int* occ, int* occlist
         set_error(MAGACE_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e) + " (" + what + ")");
         return true;
     };
-    if ((size_t)P > c.cap_pairs) {
-        dfree(c.d_dl); dfree(c.d_el); dfree(c.d_dm); dfree(c.d_em); dfree(c.d_fl); dfree(c.d_tl);
-        const size_t cap = (size_t)P + P / 4 + 64;
-        if (fail(hipMalloc((void**)&c.d_dl, cap * 8), "alloc") || fail(hipMalloc((void**)&c.d_el, cap * 24), "alloc") ||
-            fail(hipMalloc((void**)&c.d_dm, cap * 8), "alloc") || fail(hipMalloc((void**)&c.d_em, cap * 24), "alloc") ||
-            fail(hipMalloc((void**)&c.d_fl, cap * 24), "alloc") || fail(hipMalloc((void**)&c.d_tl, cap * 24), "alloc"))
-            return;
-        c.cap_pairs = cap;
-    }
-    if ((size_t)N + 1 > c.cap_atoms) {
-        dfree(c.d_off); dfree(c.d_dmi); dfree(c.d_emi); dfree(c.d_eout); dfree(c.d_t0);
-        const size_t cap = (size_t)N + N / 4 + 64;
-        if (fail(hipMalloc((void**)&c.d_off, cap * 4), "alloc") || fail(hipMalloc((void**)&c.d_dmi, cap * 8), "alloc") ||
-            fail(hipMalloc((void**)&c.d_emi, cap * 24), "alloc") || fail(hipMalloc((void**)&c.d_eout, cap * 8), "alloc") ||
-            fail(hipMalloc((void**)&c.d_t0, cap * 24), "alloc"))
-            return;
-        c.cap_atoms = cap;
-    }
     if (upload(c)) return;
+    if (!c.s_in) {
+        if (fail(hipStreamCreateWithFlags(&c.s_in, hipStreamNonBlocking), "stream") ||
+            fail(hipStreamCreateWithFlags(&c.s_out, hipStreamNonBlocking), "stream"))
+            return;
+        for (auto& sl : c.slots)
+            if (fail(hipEventCreateWithFlags(&sl.ev_in, hipEventDisableTiming), "event") ||
+                fail(hipEventCreateWithFlags(&sl.ev_comp, hipEventDisableTiming), "event") ||
+                fail(hipEventCreateWithFlags(&sl.ev_done, hipEventDisableTiming), "event"))
+                return;
+    }
+    const std::vector<int> cut = chunk_plan(c.h_off, chunk_pairs());
+    const size_t nchunk = cut.size() - 1;
+    size_t need_in = 0, need_out = 0;
+    for (size_t k = 0; k < nchunk; ++k) {
+        const ChunkLayout L(cut[k + 1] - cut[k], c.h_off[cut[k + 1]] - c.h_off[cut[k]]);
+        need_in = std::max(need_in, L.in_bytes);
+        need_out = std::max(need_out, L.out_bytes);
+    }
+    for (auto& sl : c.slots) {
+        if (need_in > sl.cap_in) {
+            dfree(sl.d_in);
+            if (sl.h_in) (void)hipHostFree(sl.h_in);
+            sl.h_in = nullptr;
+            sl.cap_in = 0;
+            if (fail(hipHostMalloc((void**)&sl.h_in, need_in, hipHostMallocDefault), "pinned alloc") ||
+                fail(hipMalloc((void**)&sl.d_in, need_in), "alloc"))
+                return;
+            sl.cap_in = need_in;
+        }
+        if (need_out > sl.cap_out) {
+            dfree(sl.d_out);
+            if (sl.h_out) (void)hipHostFree(sl.h_out);
+            sl.h_out = nullptr;
+            sl.cap_out = 0;
+            if (fail(hipHostMalloc((void**)&sl.h_out, need_out, hipHostMallocDefault), "pinned alloc") ||
+                fail(hipMalloc((void**)&sl.d_out, need_out), "alloc"))
+                return;
+            sl.cap_out = need_out;
+        }
+    }
     hipStream_t s = c.stream;
-    const auto H2D = hipMemcpyHostToDevice;
-    if (fail(hipMemcpyAsync(c.d_off, c.h_off.data(), (N + 1) * 4, H2D, s), "copy") ||
-        fail(hipMemcpyAsync(c.d_dl, dlist, (size_t)P * 8, H2D, s), "copy") ||
-        fail(hipMemcpyAsync(c.d_el, elist, (size_t)P * 24, H2D, s), "copy") ||
-        fail(hipMemcpyAsync(c.d_dm, dlist_mom, (size_t)P * 8, H2D, s), "copy") ||
-        fail(hipMemcpyAsync(c.d_em, elist_mom, (size_t)P * 24, H2D, s), "copy") ||
-        fail(hipMemcpyAsync(c.d_dmi, dlist_momi, (size_t)N * 8, H2D, s), "copy") ||
-        fail(hipMemcpyAsync(c.d_emi, elist_momi, (size_t)N * 24, H2D, s), "copy"))
-        return;
-    if (run_device(c, N, c.d_off, c.d_dl, c.d_el, c.d_dmi, c.d_emi, c.d_dm, c.d_em, c.d_eout, c.d_fl, c.d_tl,
-                   c.d_t0, s))
-        return;
-    const auto D2H = hipMemcpyDeviceToHost;
-    if (fail(hipMemcpyAsync(eout, c.d_eout, (size_t)N * 8, D2H, s), "copy") ||
-        fail(hipMemcpyAsync(flist, c.d_fl, (size_t)P * 24, D2H, s), "copy") ||
-        fail(hipMemcpyAsync(tlist, c.d_tl, (size_t)P * 24, D2H, s), "copy") ||
-        fail(hipMemcpyAsync(tlist0, c.d_t0, (size_t)N * 24, D2H, s), "copy"))
-        return;
+    HostPool& pool = HostPool::get();
+    if (fail(clear_flags(c.buf.err, s), "clear")) return;
+    // chunk k's outputs, once its kernels are done, into the caller's arrays
+    auto unpack = [&](size_t k) {
+        const Context::Slot& sl = c.slots[k & 1];
+        if (fail(hipEventSynchronize(sl.ev_done), "sync")) return false;
+        const size_t a0 = cut[k], p0 = c.h_off[a0];
+        const ChunkLayout L(cut[k + 1] - a0, c.h_off[cut[k + 1]] - p0);
+        const char* o = sl.h_out;
+        parallel_copy(pool, {{eout + a0, o + L.o_eout, 8 * L.na},
+                             {tlist0 + 3 * a0, o + L.o_t0, 24 * L.na},
+                             {flist + 3 * p0, o + L.o_fl, 24 * L.np},
+                             {tlist + 3 * p0, o + L.o_tl, 24 * L.np}});
+        return true;
+    };
+    for (size_t k = 0; k < nchunk; ++k) {
+        // the slot's previous chunk, k - 2, is done and unpacked
+        Context::Slot& sl = c.slots[k & 1];
+        const size_t a0 = cut[k], p0 = c.h_off[a0];
+        const ChunkLayout L(cut[k + 1] - a0, c.h_off[cut[k + 1]] - p0);
+        char* in = sl.h_in;
+        int* off = (int*)(in + L.i_off);
+        for (size_t i = 0; i <= L.na; ++i) off[i] = c.h_off[a0 + i] - (int)p0;
+        parallel_copy(pool, {{in + L.i_dl, dlist + p0, 8 * L.np},
+                             {in + L.i_el, elist + 3 * p0, 24 * L.np},
+                             {in + L.i_dm, dlist_mom + p0, 8 * L.np},
+                             {in + L.i_em, elist_mom + 3 * p0, 24 * L.np},
+                             {in + L.i_dmi, dlist_momi + a0, 8 * L.na},
+                             {in + L.i_emi, elist_momi + 3 * a0, 24 * L.na}});
+        if (fail(hipMemcpyAsync(sl.d_in, in, L.in_bytes, hipMemcpyHostToDevice, c.s_in), "copy") ||
+            fail(hipEventRecord(sl.ev_in, c.s_in), "event") || fail(hipStreamWaitEvent(s, sl.ev_in, 0), "event"))
+            return;
+        const char* d = sl.d_in;
+        char* o = sl.d_out;
+        if (run_device(c, (int)L.na, (const int*)(d + L.i_off), (const double*)(d + L.i_dl),
+                       (const double*)(d + L.i_el), (const double*)(d + L.i_dmi), (const double*)(d + L.i_emi),
+                       (const double*)(d + L.i_dm), (const double*)(d + L.i_em), (double*)(o + L.o_eout),
+                       (double*)(o + L.o_fl), (double*)(o + L.o_tl), (double*)(o + L.o_t0), s, false))
+            return;
+        if (fail(hipEventRecord(sl.ev_comp, s), "event") || fail(hipStreamWaitEvent(c.s_out, sl.ev_comp, 0), "event") ||
+            fail(hipMemcpyAsync(sl.h_out, sl.d_out, L.out_bytes, hipMemcpyDeviceToHost, c.s_out), "copy") ||
+            fail(hipEventRecord(sl.ev_done, c.s_out), "event"))
+            return;
+        if (k > 0 && !unpack(k - 1)) return;
+    }
+    if (nchunk > 0 && !unpack(nchunk - 1)) return;
     if (check_kernel_errors(c, s)) return;
 }
 

@@ -868,13 +868,22 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                 Fz = fma(s, ez, Fz);
             }
         }
-        if (active) {
-            flist[3 * (size_t)p] = (double)Fx;
-            flist[3 * (size_t)p + 1] = (double)Fy;
-            flist[3 * (size_t)p + 2] = (double)Fz;
-            tlist[3 * (size_t)p] = (double)Tx;
-            tlist[3 * (size_t)p + 1] = (double)Ty;
-            tlist[3 * (size_t)p + 2] = (double)Tz;
+        // store transposed across the wave: lane L writes element L + 64 r of the 64 pairs'
+        // 192 consecutive doubles of flist (and of tlist), so that every store instruction
+        // covers whole lines, also when the outputs live in host memory and the stores cross
+        // PCIe.  Element i belongs to pair i / 3, component i % 3.
+        const int cnt = 3 * min(64, nn - c0);
+        double* __restrict__ fo = flist + 3 * (size_t)(p0 + c0);
+        double* __restrict__ to = tlist + 3 * (size_t)(p0 + c0);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int i = lane + 64 * r, src = i / 3, comp = i - 3 * src;
+            const T f0 = __shfl(Fx, src), f1 = __shfl(Fy, src), f2 = __shfl(Fz, src);
+            const T t0 = __shfl(Tx, src), t1 = __shfl(Ty, src), t2 = __shfl(Tz, src);
+            if (i < cnt) {
+                fo[i] = (double)(comp == 0 ? f0 : comp == 1 ? f1 : f2);
+                to[i] = (double)(comp == 0 ? t0 : comp == 1 ? t1 : t2);
+            }
         }
     }
     // ---- self term T0 = dE_i/dm_i over m' >= 0 (src/forces.F90:712-749)
