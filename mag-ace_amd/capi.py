@@ -27,7 +27,8 @@ EXPORTS = ("acelibini_", "acelib_", "magace_last_error", "magace_error_string", 
            "magace_pair_compute", "magace_model_terms", "magace_profile", "magace_profile_read",
            "magace_neighbors_device", "magace_packstructures", "magace_md_create", "magace_md_run",
            "magace_md_read", "magace_md_destroy", "magace_fit_ncoef", "magace_fit_get_coefficients",
-           "magace_fit_set_coefficients", "magace_fit_forward", "magace_fit_backward", "magace_write_potential")
+           "magace_fit_set_coefficients", "magace_fit_forward", "magace_fit_backward", "magace_write_potential",
+           "magace_model_records")
 
 ERR_CAPACITY = 4
 
@@ -223,6 +224,21 @@ def model_terms():
     w = np.zeros(max(n, 1))
     L.magace_model_terms(idx.ctypes.data, w.ctypes.data, n)
     return idx[:n].astype(np.int64), w[:n]
+
+
+def model_records():
+    """The adjoint's gradient table: (chunks (nchunk, 2) int32, idx (nrec, 4), w (nrec,))."""
+    L = lib()
+    L.magace_model_records.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, _IP]
+    nch = C.c_int(0)
+    n = L.magace_model_records(None, 0, None, None, 0, C.byref(nch))
+    if n < 0:
+        _check(n)
+    ch = np.zeros((max(nch.value, 1), 2), np.int32)
+    idx = np.zeros((max(n, 1), 4), np.uint16)
+    w = np.zeros(max(n, 1))
+    L.magace_model_records(ch.ctypes.data, nch.value, idx.ctypes.data, w.ctypes.data, n, C.byref(nch))
+    return ch[:nch.value], idx[:n].astype(np.int64), w[:n]
 
 
 def profile(enable=True):

@@ -206,8 +206,8 @@ int build_context(Potential&& pot, std::unique_ptr<Context>& out) {
         return MAGACE_ERR_INPUT;
     }
     M.z = make_layout(M.pot.h);
-    if (M.z.nz >= 65535) {
-        set_error(MAGACE_ERR_INPUT, "feature vector too long for 16-bit term indices");
+    if (16 * (M.z.nz + 1) >= 65536) {
+        set_error(MAGACE_ERR_INPUT, "feature vector too long (the adjoint addresses Z with 16-bit byte offsets)");
         return MAGACE_ERR_INPUT;
     }
     const Header& h = M.pot.h;
@@ -221,6 +221,10 @@ int build_context(Potential&& pot, std::unique_ptr<Context>& out) {
         return MAGACE_ERR_INPUT;
     }
     M.terms = build_terms(M.pot, M.z);
+    if (M.terms.ncontrib() >= (size_t(1) << 27)) {
+        set_error(MAGACE_ERR_INPUT, "gradient record table too large (2^27 records)");
+        return MAGACE_ERR_INPUT;
+    }
     M.spl = build_splines(M.pot);
     fill_params(ctx->P64, M);
     fill_params(ctx->P32, M);
@@ -228,7 +232,9 @@ int build_context(Potential&& pot, std::unique_ptr<Context>& out) {
     return 0;
 }
 
-// 16-byte gradient records (factor indices a | b << 16, c; weight as FP64 bits or FP32 bits)
+// 16-byte gradient records: the three factors as byte offsets into the LDS copy of Z
+// (a | b << 16, c; entry size 16 bytes in FP64, 8 in FP32), then the weight as FP64 bits or
+// FP32 bits.  Offsets save the kernel the index arithmetic; init checks 16 (nz + 1) < 2^16.
 int upload_records(Context& c) {
     const TermList& T = c.model.terms;
     const size_t nc = T.ncontrib();
@@ -241,9 +247,9 @@ int upload_records(Context& c) {
         const float wf = (float)w;
         uint32_t fb;
         std::memcpy(&fb, &wf, 4);
-        r64[i] = make_uint4((uint32_t)q[0] | ((uint32_t)q[1] << 16), q[2], (uint32_t)(wb & 0xffffffffu),
-                            (uint32_t)(wb >> 32));
-        r32[i] = make_uint4((uint32_t)q[0] | ((uint32_t)q[1] << 16), q[2], fb, 0u);
+        r64[i] = make_uint4(((uint32_t)q[0] << 4) | ((uint32_t)q[1] << 20), (uint32_t)q[2] << 4,
+                            (uint32_t)(wb & 0xffffffffu), (uint32_t)(wb >> 32));
+        r32[i] = make_uint4(((uint32_t)q[0] << 3) | ((uint32_t)q[1] << 19), (uint32_t)q[2] << 3, fb, 0u);
     }
     HIPCHK(hipMemcpy(c.crec64, r64.data(), nc * sizeof(uint4), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c.crec32, r32.data(), nc * sizeof(uint4), hipMemcpyHostToDevice));
@@ -775,6 +781,22 @@ int magace_model_terms(uint16_t* idx, double* w, int cap) {
     if (idx && w && cap >= n) {
         std::memcpy(idx, T.idx.data(), (size_t)n * 4 * sizeof(uint16_t));
         std::memcpy(w, T.w.data(), (size_t)n * sizeof(double));
+    }
+    return n;
+}
+
+int magace_model_records(int* chunks, int cap_chunks, uint16_t* idx, double* w, int cap, int* nchunks) {
+    if (!g_ctx) {
+        set_error(MAGACE_ERR_STATE, "library not initialised");
+        return -MAGACE_ERR_STATE;
+    }
+    const TermList& T = g_ctx->model.terms;
+    const int n = (int)T.ncontrib(), nc = (int)(T.chunks.size() / 2);
+    if (nchunks) *nchunks = nc;
+    if (chunks && idx && w && cap >= n && cap_chunks >= nc) {
+        std::memcpy(chunks, T.chunks.data(), (size_t)nc * 2 * sizeof(int32_t));
+        std::memcpy(idx, T.cidx.data(), (size_t)n * 4 * sizeof(uint16_t));
+        std::memcpy(w, T.cw.data(), (size_t)n * sizeof(double));
     }
     return n;
 }

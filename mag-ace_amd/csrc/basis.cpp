@@ -399,8 +399,8 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
     // neighbouring lanes flush to different targets.  A wave processes 64 consecutive
     // chunks at once; their records are stored interleaved (record k of lane L of group G
     // at G*64*len + k*64 + L), so each step of the 64 lanes reads 64 consecutive 16-byte
-    // records.  Chunks are padded (factor index nz, weight 0) to a multiple of 4 and to the
-    // longest chunk of their group.
+    // records.  Chunks are padded (factor index nz, weight 0) to the longest chunk of their
+    // group, rounded up to a multiple of 4.
     struct Ch { int x, off, cnt; };
     std::vector<std::vector<Ch>> rowchunks(z.nz);
     for (int x = 0; x < z.nz; ++x) {
@@ -409,21 +409,36 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
         for (size_t i = 0; i < v.size(); i += TermList::kChunk)
             rowchunks[x].push_back({x, (int)i, (int)std::min<size_t>(TermList::kChunk, v.size() - i)});
     }
-    // full chunks round robin, then each row's partial tail chunk by decreasing length, so
-    // that the chunks of a wave have (nearly) equal lengths and distinct targets
-    std::vector<Ch> order, tails;
-    for (size_t round = 0;; ++round) {
-        bool any = false;
+    // Order of the chunks.  Chunk c runs on thread c mod 256 in round c / 256, so the order
+    // is a grid of rounds (rows) by 256 lanes (columns); the 64 chunks of a wave in a round
+    // form one storage group.  Full chunks fill the first whole rounds column by column,
+    // targets in decreasing number of full chunks: a lane then meets the same target in
+    // consecutive rounds and keeps its sum in registers (one LDS atomic per run instead of
+    // one per chunk), and the many chunks of the largest targets spread over many lanes
+    // instead of crowding the atomics of a few waves.  The remaining full chunks and the
+    // partial ones (one per target) follow by decreasing length, so that the chunks of a
+    // wave have (nearly) equal lengths.
+    std::vector<Ch> order, rest;
+    {
+        std::vector<int> tg;
         for (int x = 0; x < z.nz; ++x)
-            if (round < rowchunks[x].size()) {
-                any = true;
-                const Ch& ch = rowchunks[x][round];
-                (ch.cnt == TermList::kChunk ? order : tails).push_back(ch);
-            }
-        if (!any) break;
+            if (!rowchunks[x].empty()) tg.push_back(x);
+        auto nfull = [&](int x) {
+            int n = 0;
+            for (const Ch& ch : rowchunks[x]) n += ch.cnt == TermList::kChunk;
+            return n;
+        };
+        std::stable_sort(tg.begin(), tg.end(), [&](int a, int b) { return nfull(a) > nfull(b); });
+        std::vector<Ch> full;
+        for (int x : tg)
+            for (const Ch& ch : rowchunks[x]) (ch.cnt == TermList::kChunk ? full : rest).push_back(ch);
+        const size_t rounds = full.size() / 256;
+        order.resize(rounds * 256);
+        for (size_t q = 0; q < rounds * 256; ++q) order[(q % rounds) * 256 + q / rounds] = full[q];
+        rest.insert(rest.begin(), full.begin() + rounds * 256, full.end());
+        std::stable_sort(rest.begin(), rest.end(), [](const Ch& a, const Ch& b) { return a.cnt > b.cnt; });
+        order.insert(order.end(), rest.begin(), rest.end());
     }
-    std::stable_sort(tails.begin(), tails.end(), [](const Ch& a, const Ch& b) { return a.cnt > b.cnt; });
-    order.insert(order.end(), tails.begin(), tails.end());
     // Inside a chunk the records can be summed in any order.  At step k the 64 lanes read
     // Z[a], Z[b], Z[c] of their k-th records from LDS (three reads, random addresses), and
     // LDS serves a wave in lane groups whose accesses must fall in distinct banks.  The
@@ -480,10 +495,12 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
                 rem.erase(rem.begin() + best);
             }
         }
+        // every chunk of the group records the group's length: the lanes of a wave step
+        // through the same number of records (those past a chunk's end have weight 0)
         for (int L = 0; L < nl; ++L) {
             const Ch& ch = order[g0 + L];
             T.chunks.push_back((int32_t)(base + L));
-            T.chunks.push_back(ch.x | (((ch.cnt + 3) & ~3) << 16));
+            T.chunks.push_back(ch.x | (len << 16));
         }
     }
     if (nd > 1) {

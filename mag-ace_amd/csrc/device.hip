@@ -462,6 +462,17 @@ __device__ inline void embed(int nfemb, double rho, double wpre, double mexp, do
     }
 }
 
+// one complex entry of an LDS array at a byte offset, as a single aligned vector read
+template <typename T> __device__ __forceinline__ cplx<T> lds_cplx(const char* base, uint32_t off);
+template <> __device__ __forceinline__ cplx<float> lds_cplx<float>(const char* base, uint32_t off) {
+    const float2 v = *(const float2*)(base + off);
+    return {v.x, v.y};
+}
+template <> __device__ __forceinline__ cplx<double> lds_cplx<double>(const char* base, uint32_t off) {
+    const double2 v = *(const double2*)(base + off);
+    return {v.x, v.y};
+}
+
 // ND2: two densities (rho_d = sum_k c_{k,d} B_k, E = fc sum_d F(rho_d) + e_hc,
 // src/forces.F90:976-1089): a second weight per record, a second gradient array in LDS
 template <typename T, bool ND2>
@@ -492,49 +503,72 @@ __global__ void __launch_bounds__(256) k_adjoint(DevParams<T> P, const cplx<T>* 
     __syncthreads();
     // Gather form of the gradient: a thread sums one chunk of contributions
     // w * Z[a] Z[b] Z[c] to a single target x in registers (four 16-byte records per step,
-    // loads issued together; the 64 lanes of a wave read consecutive records) and adds it
-    // to LDS with one atomic.  The chunk -> thread mapping (chunk c on thread c mod 256)
-    // must match the record layout built by build_terms.  rho = Re sum over A0
-    // targets of Z[x] * dRho/dZ[x] (every monomial has exactly one A0 factor), summed in
-    // FP64 once per chunk.
+    // loads issued together; the 64 lanes of a wave read consecutive records).  Thread t
+    // takes chunks t + 256 j; the chunk -> thread mapping must match the record layout built
+    // by build_terms.  While the thread's next chunk has the same target the sum stays in
+    // registers; it goes to LDS with one atomic when the target changes.  Each block starts
+    // at a different j (rotated by its index), so that the blocks of an XCD stream different
+    // parts of the record table at any moment instead of all reading the same L2 lines.
+    // rho = Re sum over A0 targets of Z[x] * dRho/dZ[x] (every monomial has exactly one A0
+    // factor), summed in FP64 once per flush.
     double rho = 0.0, rho2 = 0.0;
-    for (int c = tid; c < P.nchunks; c += blockDim.x) {
-        const int2 ch = P.chunks[c];
-        const int x = ch.y & 0xffff, cnt = ch.y >> 16;
-        const uint4* __restrict__ rec = P.crec + ch.x;  // records of this lane, stride 64
-        const T* __restrict__ w2p = ND2 ? P.cw2 + ch.x : nullptr;
-        cplx<T> acc = {0, 0}, acc2 = {0, 0};
-        for (int k = 0; k < cnt; k += 4) {
-            uint4 r[4];
-            T w2[4];
+    const int nthr = (int)blockDim.x;
+    const int nj = (P.nchunks + nthr - 1) / nthr;
+    const int rot = nj > 0 ? (int)(blockIdx.x % (unsigned)nj) : 0;
+    cplx<T> acc = {0, 0}, acc2 = {0, 0};
+    int j = rot;
+    int2 ch = tid + j * nthr < P.nchunks ? P.chunks[tid + j * nthr] : make_int2(0, 0);
+    for (int j0 = 0; j0 < nj; ++j0) {
+        const int c = tid + j * nthr;
+        const int jn = j + 1 < nj ? j + 1 : 0;
+        const int cn = tid + jn * nthr;
+        const bool has_next = j0 + 1 < nj && cn < P.nchunks;
+        const int2 chn = has_next ? P.chunks[cn] : make_int2(0, 0);
+        if (c < P.nchunks) {
+            const int x = ch.y & 0xffff, cnt = ch.y >> 16;
+            const uint4* __restrict__ rec = P.crec + ch.x;  // records of this lane, stride 64
+            const T* __restrict__ w2p = ND2 ? P.cw2 + ch.x : nullptr;
+            for (int k = 0; k < cnt; k += 4) {
+                uint4 r[4];
+                T w2[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                r[u] = rec[(size_t)(k + u) * 64];
-                if (ND2) w2[u] = w2p[(size_t)(k + u) * 64];
-            }
+                for (int u = 0; u < 4; ++u) {
+                    r[u] = rec[(size_t)(k + u) * 64];
+                    if (ND2) w2[u] = w2p[(size_t)(k + u) * 64];
+                }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const T w = rec_weight<T>(r[u]);
-                const cplx<T> za = sZ[r[u].x & 0xffff], zb = sZ[r[u].x >> 16], zc = sZ[r[u].y & 0xffff];
-                const cplx<T> pr = cmul(cmul(za, zb), zc);
-                acc.re = fma(w, pr.re, acc.re);
-                acc.im = fma(w, pr.im, acc.im);
-                if (ND2) {
-                    acc2.re = fma(w2[u], pr.re, acc2.re);
-                    acc2.im = fma(w2[u], pr.im, acc2.im);
+                for (int u = 0; u < 4; ++u) {
+                    const T w = rec_weight<T>(r[u]);
+                    const char* zb0 = (const char*)sZ;  // records hold byte offsets into sZ
+                    const cplx<T> za = lds_cplx<T>(zb0, r[u].x & 0xffff);
+                    const cplx<T> zb = lds_cplx<T>(zb0, r[u].x >> 16);
+                    const cplx<T> zc = lds_cplx<T>(zb0, r[u].y);
+                    const cplx<T> pr = cmul(cmul(za, zb), zc);
+                    acc.re = fma(w, pr.re, acc.re);
+                    acc.im = fma(w, pr.im, acc.im);
+                    if (ND2) {
+                        acc2.re = fma(w2[u], pr.re, acc2.re);
+                        acc2.im = fma(w2[u], pr.im, acc2.im);
+                    }
                 }
             }
+            if (!(has_next && (chn.y & 0xffff) == x)) {
+                if (x >= P.oA0) {
+                    rho += (double)re_mul(sZ[x], acc);
+                    if (ND2) rho2 += (double)re_mul(sZ[x], acc2);
+                }
+                lds_add(&sW[x].re, acc.re);
+                lds_add(&sW[x].im, acc.im);
+                if (ND2) {
+                    lds_add(&sW2[x].re, acc2.re);
+                    lds_add(&sW2[x].im, acc2.im);
+                }
+                acc = {0, 0};
+                acc2 = {0, 0};
+            }
         }
-        if (x >= P.oA0) {
-            rho += (double)re_mul(sZ[x], acc);
-            if (ND2) rho2 += (double)re_mul(sZ[x], acc2);
-        }
-        lds_add(&sW[x].re, acc.re);
-        lds_add(&sW[x].im, acc.im);
-        if (ND2) {
-            lds_add(&sW2[x].re, acc2.re);
-            lds_add(&sW2[x].im, acc2.im);
-        }
+        ch = chn;
+        j = jn;
     }
     const double tot = block_sum<T>(rho, red);
     double tot2 = 0.0;

@@ -22,10 +22,11 @@ struct DevParams {
     int nb_a = 0, nb_m = 0, nr2m = 0, l2m = 0, nr3a = 0, nr3m = 0, l3a = 0, l3m = 0;
     int lsa = 0, lsm = 0, lsm3 = 0, nm3 = 0, oA2 = 0, oA0 = 0, nz = 0;
     int lha = 0, lhm = 0;  // (l+1)(l+2)/2 for l3a, l2m
-    // gradient contributions, one 16-byte record each: x = a | b << 16, y = c, (z, w) = weight
+    // gradient contributions, one 16-byte record each: x = a | b << 16, y = c as byte offsets
+    // of the factors in the adjoint's LDS copy of Z (index * sizeof(cplx<T>)), (z, w) = weight
     // (FP64 bits as lo/hi words, or FP32 bits in z)
     const uint4* crec = nullptr;
-    const int2* chunks = nullptr;    // (start, target | count << 16), count a multiple of 4
+    const int2* chunks = nullptr;    // (start, target | count << 16), count = its group's length
     const uint32_t* fold = nullptr;  // conjugate-pair folding of W (see TermList::fold)
     int nchunks = 0;
     int nfemb = 0;

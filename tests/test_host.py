@@ -156,3 +156,33 @@ print("counts", capi.basis_counts())
     assert p.returncode == 0, p.stderr
     assert "multi: Multiple elements" in p.stdout
     assert "counts [2, 36, 30, 186]" in p.stdout
+
+
+@pytest.mark.parametrize("hname", ["HTINY", "H500", "H2000", "H500ND2"])
+def test_gradient_records_equal_the_monomial_derivative(hname, tmp_path):
+    """The adjoint's transposed table reproduces dRho/dZ[x] of the monomial table for a
+    random complex Z, entry by entry: the brute-force derivative differentiates every slot
+    of every monomial.  Each chunk header holds its group's length: the 64 chunks of a
+    group have equal lengths, and records past a chunk's end carry weight 0."""
+    pot = write_potential(str(tmp_path / "Fe.pbe.in"), hname, 31)
+    capi.init_file(pot)
+    nz = capi.model_info()["nz"]
+    idx, w = capi.model_terms()
+    ch, ridx, rw = capi.model_records()
+    rng = np.random.default_rng(5)
+    Z = rng.normal(size=nz + 1) + 1j * rng.normal(size=nz + 1)
+    Z[nz] = 1.0
+    want = np.zeros(nz + 1, complex)
+    F = Z[idx]
+    for s in range(4):
+        np.add.at(want, idx[:, s], w * np.prod(np.delete(F, s, axis=1), axis=1))
+    first, x, cnt = ch[:, 0], ch[:, 1] & 0xFFFF, ch[:, 1] >> 16
+    assert np.all(x < nz) and np.all(cnt % 4 == 0) and np.all((cnt >= 4) & (cnt <= 32))
+    for g0 in range(0, len(ch), 64):
+        assert np.all(cnt[g0:g0 + 64] == cnt[g0]) and np.all(first[g0:g0 + 64] == first[g0] + np.arange(len(ch[g0:g0 + 64])))
+    got = np.zeros(nz + 1, complex)
+    for c in range(len(ch)):
+        r = first[c] + 64 * np.arange(cnt[c])
+        got[x[c]] += np.sum(rw[r] * Z[ridx[r, 0]] * Z[ridx[r, 1]] * Z[ridx[r, 2]])
+    scale = np.abs(want[:nz]).max()
+    assert np.abs(got[:nz] - want[:nz]).max() <= 1e-12 * scale
