@@ -274,7 +274,19 @@ __global__ void __launch_bounds__(256) k_density(DevParams<T> P, const int* __re
 
     const int nblkA = (P.nr3a + NB - 1) / NB, nblkA2 = (P.nb_a + NB2 - 1) / NB2;
     const int nTA = P.lsa * nblkA, nTA2 = P.lsm * nblkA2;
-    for (int round = 0; round * nthr < nTA + nTA2; ++round) {
+    // When the A and A2 column tasks leave most threads idle (FP32, lmax-4 headers: 75 tasks
+    // for 256 threads), each A task's neighbours are split over SA threads (neighbour j on
+    // part j mod SA); the parts meet in LDS at the end, in the staging area, which is free by
+    // then.  Measured on the 16000-atom workloads: lmax 4 FP32 0.61 -> 0.53 ms; with SA = 2
+    // (lmax 6) and in FP64 the extra barriers and scratch traffic cost more than they gain.
+    int SA = 1;
+    if constexpr (sizeof(T) == 4 && !SPLIT) if (nTA + nTA2 <= nthr) {
+        SA = min(4, (nthr - nTA2) / max(nTA, 1));
+        const size_t room = (size_t)((const char*)sEhc - (const char*)smem);
+        while (SA > 1 && (size_t)(SA - 1) * nTA * NB * NM3X * sizeof(cplx<T>) > room) --SA;
+        if (SA < 3) SA = 1;
+    }
+    for (int round = 0; round * nthr < (sizeof(T) == 4 && !SPLIT ? SA : 1) * nTA + nTA2; ++round) {
         const int task = round * nthr + tid;
         cplx<T> acc[NB > NB2 ? NB : NB2][NM3X > NR2X ? NM3X : NR2X];
 #pragma unroll
@@ -282,10 +294,15 @@ __global__ void __launch_bounds__(256) k_density(DevParams<T> P, const int* __re
 #pragma unroll
             for (int b = 0; b < (NM3X > NR2X ? NM3X : NR2X); ++b) acc[a][b] = {0, 0};
         // decode task
-        const bool isA = task < nTA, isA2 = !isA && task < nTA + nTA2;
+        // only light FP32 headers (classic staging) split tasks; the others keep a
+        // compile-time SA = 1
+        constexpr bool kSplit = sizeof(T) == 4 && !SPLIT;
+        const int sa = kSplit ? SA : 1;
+        const bool isA = task < sa * nTA, isA2 = !isA && task < sa * nTA + nTA2;
+        const int ta = kSplit && isA ? task % nTA : task, part = kSplit && isA ? task / nTA : 0;
         int lm = 0, l = 0, m = 0, n0 = 0;
-        if (isA) { lm = task % P.lsa; n0 = (task / P.lsa) * NB; }
-        if (isA2) { lm = (task - nTA) % P.lsm; n0 = ((task - nTA) / P.lsm) * NB2; }
+        if (isA) { lm = ta % P.lsa; n0 = (ta / P.lsa) * NB; }
+        if (isA2) { lm = (task - sa * nTA) % P.lsm; n0 = ((task - sa * nTA) / P.lsm) * NB2; }
         while ((l + 1) * (l + 1) <= lm) ++l;
         m = lm - l * l - l;
 
@@ -413,7 +430,7 @@ __global__ void __launch_bounds__(256) k_density(DevParams<T> P, const int* __re
             }
             if (isA) {
                 // A[n][lm][q] += R_{l,n} Y_lm H_q   (src/a_basis.F90:248-291)
-                for (int j = 0; j < nj; ++j) {
+                for (int j = part; j < nj; j += sa) {
                     const cplx<T> y = yfull(sY + j * P.lha, l, m);
                     cplx<T> h[NM3X];
 #pragma unroll
@@ -450,7 +467,27 @@ __global__ void __launch_bounds__(256) k_density(DevParams<T> P, const int* __re
                 }
             }
         }
-        if (isA) {
+        if (kSplit && sa > 1) {  // one round only: every thread of the block reaches these barriers
+            cplx<T>* scr = (cplx<T>*)smem;  // [SA - 1][nTA][NB][NM3X]
+            __syncthreads();
+            if (isA && part > 0)
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+#pragma unroll
+                    for (int q = 0; q < NM3X; ++q) scr[(((part - 1) * nTA + ta) * NB + b) * NM3X + q] = acc[b][q];
+            __syncthreads();
+            if (isA && part == 0)
+                for (int h = 1; h < sa; ++h)
+#pragma unroll
+                    for (int b = 0; b < NB; ++b)
+#pragma unroll
+                        for (int q = 0; q < NM3X; ++q) {
+                            const cplx<T> v = scr[(((h - 1) * nTA + ta) * NB + b) * NM3X + q];
+                            acc[b][q].re += v.re;
+                            acc[b][q].im += v.im;
+                        }
+        }
+        if (isA && part == 0) {
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const int n = n0 + b;
