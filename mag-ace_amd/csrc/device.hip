@@ -210,8 +210,11 @@ __device__ __forceinline__ cplx<T> yfull(const cplx<T>* half, int l, int m) {
 // ============================================================================ k_density
 // Shapes: NB n-block of A columns, NM3X max magnetic columns of A, NB2 n-block of A2,
 // NR2X max magnetic radial index of A2.
+// FP32, shape class 0: at least 5 waves per SIMD (the register budget drops to 102; measured
+// 0.91 -> 0.77 ms on the lmax-6 workload, 0.52 -> 0.46 on lmax 4).  FP64 needs its ~170
+// registers, and class 1 (27 magnetic columns) would spill.
 template <typename T, int L, int CH, int NB, int NM3X, int NB2, int NR2X, bool SPLIT>
-__global__ void __launch_bounds__(256) k_density(DevParams<T> P, const int* __restrict__ off,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && L == 6 ? 5 : 1, 8))) k_density(DevParams<T> P, const int* __restrict__ off,
                                                 const double* __restrict__ dlist,
                                                 const double* __restrict__ elist,
                                                 const double* __restrict__ dmomi,
