@@ -210,11 +210,12 @@ __device__ __forceinline__ cplx<T> yfull(const cplx<T>* half, int l, int m) {
 // ============================================================================ k_density
 // Shapes: NB n-block of A columns, NM3X max magnetic columns of A, NB2 n-block of A2,
 // NR2X max magnetic radial index of A2.
-// FP32, shape class 0: at least 5 waves per SIMD (the register budget drops to 102; measured
-// 0.91 -> 0.77 ms on the lmax-6 workload, 0.52 -> 0.46 on lmax 4).  FP64 needs its ~170
-// registers, and class 1 (27 magnetic columns) would spill.
+// Shape class 0: at least 5 waves per SIMD in FP32 (the register budget drops to 102; measured
+// 0.91 -> 0.77 ms on the lmax-6 workload, 0.52 -> 0.46 on lmax 4) and 3 in FP64 (168 registers;
+// at 169 the occupancy halves and the kernel takes 1.85 instead of 1.39 ms).  Class 1 (27
+// magnetic columns) would spill.
 template <typename T, int L, int CH, int NB, int NM3X, int NB2, int NR2X, bool SPLIT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && L == 6 ? 5 : 1, 8))) k_density(DevParams<T> P, const int* __restrict__ off,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6 ? (sizeof(T) == 4 ? 5 : 3) : 1, 8))) k_density(DevParams<T> P, const int* __restrict__ off,
                                                 const double* __restrict__ dlist,
                                                 const double* __restrict__ elist,
                                                 const double* __restrict__ dmomi,
@@ -289,7 +290,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
         while (SA > 1 && (size_t)(SA - 1) * nTA * NB * NM3X * sizeof(cplx<T>) > room) --SA;
         if (SA < 3) SA = 1;
     }
-    for (int round = 0; round * nthr < (sizeof(T) == 4 && !SPLIT ? SA : 1) * nTA + nTA2; ++round) {
+    // A2 tasks start at a wave boundary when that still fits in one round: a wave holding both
+    // kinds would run both loops one after the other
+    const int nAt = (sizeof(T) == 4 && !SPLIT ? SA : 1) * nTA;
+    const int a2base = ((nAt + 63) & ~63) + nTA2 <= nthr ? (nAt + 63) & ~63 : nAt;
+    for (int round = 0; round * nthr < a2base + nTA2; ++round) {
         const int task = round * nthr + tid;
         cplx<T> acc[NB > NB2 ? NB : NB2][NM3X > NR2X ? NM3X : NR2X];
 #pragma unroll
@@ -301,11 +306,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
         // compile-time SA = 1
         constexpr bool kSplit = sizeof(T) == 4 && !SPLIT;
         const int sa = kSplit ? SA : 1;
-        const bool isA = task < sa * nTA, isA2 = !isA && task < sa * nTA + nTA2;
+        const bool isA = task < sa * nTA, isA2 = task >= a2base && task < a2base + nTA2;
         const int ta = kSplit && isA ? task % nTA : task, part = kSplit && isA ? task / nTA : 0;
         int lm = 0, l = 0, m = 0, n0 = 0;
         if (isA) { lm = ta % P.lsa; n0 = (ta / P.lsa) * NB; }
-        if (isA2) { lm = (task - sa * nTA) % P.lsm; n0 = ((task - sa * nTA) / P.lsm) * NB2; }
+        if (isA2) { lm = (task - a2base) % P.lsm; n0 = ((task - a2base) / P.lsm) * NB2; }
         while ((l + 1) * (l + 1) <= lm) ++l;
         m = lm - l * l - l;
 
