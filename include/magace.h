@@ -191,7 +191,8 @@ int magace_model_info(long long *info, int n);
 int magace_model_terms(unsigned short *idx, double *w, int cap);
 /* The transposed gradient table the adjoint kernel streams (dRho/dZ[x] gathered per target
  * x, the gather form of src/forces.F90:84-349): chunks[2c] = first record, chunks[2c+1] =
- * x | count << 16 (count = the length of the chunk's group of 64); record r holds the
+ * x | count << 16 | sole << 22 (count = the length of the chunk's group of 64, sole = the
+ * target's only chunk); record r holds the
  * factors idx[4r..4r+2] and the weight w[r] (density 1); a chunk contributes
  * sum_j w Z[a_j] Z[b_j] Z[c_j] over its count records, 64 apart.  Copies when the caps
  * suffice; returns the record count (chunk count in *nchunks).  Host-side introspection. */

@@ -496,11 +496,14 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
             }
         }
         // every chunk of the group records the group's length: the lanes of a wave step
-        // through the same number of records (those past a chunk's end have weight 0)
+        // through the same number of records (those past a chunk's end have weight 0).  Bit
+        // 22 marks the only chunk of its target: its sum is the target's whole gradient and
+        // is stored, not added.
         for (int L = 0; L < nl; ++L) {
             const Ch& ch = order[g0 + L];
+            const bool sole = rowchunks[ch.x].size() == 1;
             T.chunks.push_back((int32_t)(base + L));
-            T.chunks.push_back(ch.x | (len << 16));
+            T.chunks.push_back(ch.x | (len << 16) | (sole ? 1 << 22 : 0));
         }
     }
     if (nd > 1) {

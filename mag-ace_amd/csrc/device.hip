@@ -614,7 +614,8 @@ __global__ void __launch_bounds__(256) k_adjoint(DevParams<T> P, const cplx<T>* 
     // loads issued together; the 64 lanes of a wave read consecutive records).  Thread t
     // takes chunks t + 256 j; the chunk -> thread mapping must match the record layout built
     // by build_terms.  While the thread's next chunk has the same target the sum stays in
-    // registers; it goes to LDS with one atomic when the target changes.  Each block starts
+    // registers; it goes to LDS with one atomic when the target changes (a plain store when
+    // the chunk is its target's only one).  Each block starts
     // at a different j (rotated by its index), so that the blocks of an XCD stream different
     // parts of the record table at any moment instead of all reading the same L2 lines.
     // rho = Re sum over A0 targets of Z[x] * dRho/dZ[x] (every monomial has exactly one A0
@@ -633,7 +634,8 @@ __global__ void __launch_bounds__(256) k_adjoint(DevParams<T> P, const cplx<T>* 
         const bool has_next = j0 + 1 < nj && cn < P.nchunks;
         const int2 chn = has_next ? P.chunks[cn] : make_int2(0, 0);
         if (c < P.nchunks) {
-            const int x = ch.y & 0xffff, cnt = ch.y >> 16;
+            const int x = ch.y & 0xffff, cnt = (ch.y >> 16) & 63;
+            const bool sole = (ch.y >> 22) & 1;  // the target's only chunk
             const uint4* __restrict__ rec = P.crec + ch.x;  // records of this lane, stride 64
             const T* __restrict__ w2p = ND2 ? P.cw2 + ch.x : nullptr;
             for (int k = 0; k < cnt; k += 4) {
@@ -665,11 +667,16 @@ __global__ void __launch_bounds__(256) k_adjoint(DevParams<T> P, const cplx<T>* 
                     rho += (double)re_mul(sZ[x], acc);
                     if (ND2) rho2 += (double)re_mul(sZ[x], acc2);
                 }
-                lds_add(&sW[x].re, acc.re);
-                lds_add(&sW[x].im, acc.im);
-                if (ND2) {
-                    lds_add(&sW2[x].re, acc2.re);
-                    lds_add(&sW2[x].im, acc2.im);
+                if (sole) {  // nothing else contributes to x: a plain store
+                    sW[x] = acc;
+                    if (ND2) sW2[x] = acc2;
+                } else {
+                    lds_add(&sW[x].re, acc.re);
+                    lds_add(&sW[x].im, acc.im);
+                    if (ND2) {
+                        lds_add(&sW2[x].re, acc2.re);
+                        lds_add(&sW2[x].im, acc2.im);
+                    }
                 }
                 acc = {0, 0};
                 acc2 = {0, 0};

@@ -26,7 +26,7 @@ struct DevParams {
     // of the factors in the adjoint's LDS copy of Z (index * sizeof(cplx<T>)), (z, w) = weight
     // (FP64 bits as lo/hi words, or FP32 bits in z)
     const uint4* crec = nullptr;
-    const int2* chunks = nullptr;    // (start, target | count << 16), count = its group's length
+    const int2* chunks = nullptr;    // (start, target | count << 16 | sole << 22), count = its group's length
     const uint32_t* fold = nullptr;  // conjugate-pair folding of W (see TermList::fold)
     int nchunks = 0;
     int nfemb = 0;

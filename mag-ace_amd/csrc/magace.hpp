@@ -73,9 +73,10 @@ struct TermList {
     std::vector<double> w;       // merged weights, density-major: w[d * nterms + t]
     std::vector<uint16_t> cidx;  // 4 per contribution: the three other factors + padding
     std::vector<double> cw;      // weight per contribution, density-major (d * ncontrib + r)
-    std::vector<int32_t> chunks; // 2 per chunk: first record, target | (count << 16), count = the
-                                 // length of its group of 64 chunks (padded with zero weights);
-                                 // records of a chunk are 64 apart (wave-interleaved layout)
+    std::vector<int32_t> chunks; // 2 per chunk: first record, target | (count << 16) | (sole << 22),
+                                 // count = the length of its group of 64 chunks (padded with
+                                 // zero weights), sole = the target's only chunk; records of a
+                                 // chunk are 64 apart (wave-interleaved layout)
     // folding of the gradient onto representatives of conjugate pairs (x, xbar):
     // partner | sign << 16 (1 = odd m+m') | representative << 17 | self-conjugate << 18
     std::vector<uint32_t> fold;

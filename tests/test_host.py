@@ -163,7 +163,8 @@ def test_gradient_records_equal_the_monomial_derivative(hname, tmp_path):
     """The adjoint's transposed table reproduces dRho/dZ[x] of the monomial table for a
     random complex Z, entry by entry: the brute-force derivative differentiates every slot
     of every monomial.  Each chunk header holds its group's length: the 64 chunks of a
-    group have equal lengths, and records past a chunk's end carry weight 0."""
+    group have equal lengths, and records past a chunk's end carry weight 0; a chunk is flagged
+    sole exactly when it is its target's only one (the kernel then stores instead of adding)."""
     pot = write_potential(str(tmp_path / "Fe.pbe.in"), hname, 31)
     capi.init_file(pot)
     nz = capi.model_info()["nz"]
@@ -176,8 +177,10 @@ def test_gradient_records_equal_the_monomial_derivative(hname, tmp_path):
     F = Z[idx]
     for s in range(4):
         np.add.at(want, idx[:, s], w * np.prod(np.delete(F, s, axis=1), axis=1))
-    first, x, cnt = ch[:, 0], ch[:, 1] & 0xFFFF, ch[:, 1] >> 16
+    first, x, cnt, sole = ch[:, 0], ch[:, 1] & 0xFFFF, (ch[:, 1] >> 16) & 63, (ch[:, 1] >> 22) & 1
     assert np.all(x < nz) and np.all(cnt % 4 == 0) and np.all((cnt >= 4) & (cnt <= 32))
+    nchunk_of = np.bincount(x, minlength=nz)
+    assert np.array_equal(sole == 1, nchunk_of[x] == 1)  # flagged exactly when the target has one chunk
     for g0 in range(0, len(ch), 64):
         assert np.all(cnt[g0:g0 + 64] == cnt[g0]) and np.all(first[g0:g0 + 64] == first[g0] + np.arange(len(ch[g0:g0 + 64])))
     got = np.zeros(nz + 1, complex)
