@@ -920,26 +920,41 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                             cplx<T> Y = Y0, g[3] = {g0[0], g0[1], g0[2]};
                             if (sg) conj_all(Y, g, am);
                             const int lm = l * l + l + (sg ? -am : am);
+                            // V_n = sum_q W[n][lm][q] H_q.  The force needs
+                            // sum_n Re[V_n (R'_n Y e + R_n/r grad Y)], the torque
+                            // sum_n W[n][lm][q] R_n Y: both are summed over n first
+                            // (S1 = sum_n R'_n V_n, S2 = sum_n R_n V_n, Tq = sum_n R_n W[n][lm][q])
+                            // and meet Y and grad Y once per (l, m)
+                            cplx<T> S1 = {0, 0}, S2 = {0, 0}, Tq[NM3X];
+#pragma unroll
+                            for (int q = 0; q < NM3X; ++q) Tq[q] = {0, 0};
 #pragma unroll
                             for (int n = 0; n < NRAX; ++n) {
                                 if (n >= P.nr3a) break;
                                 const cplx<T>* Wn = W + (n * P.lsa + lm) * P.nm3;
                                 cplx<T> V = {0, 0};
-                                const cplx<T> ry = {R[n] * Y.re, R[n] * Y.im};
 #pragma unroll
                                 for (int q = 0; q < NM3X; ++q) {
                                     if (q < P.nm3 && (qmp[q] > 0 || (qmp[q] == 0 && sg == 0))) {
                                         const cplx<T> wq = Wn[q];
                                         cfma(V, wq, H[q]);
-                                        cfma(U[q], wq, ry);
+                                        Tq[q].re = fma(R[n], wq.re, Tq[q].re);
+                                        Tq[q].im = fma(R[n], wq.im, Tq[q].im);
                                     }
                                 }
-                                // Re[V (R' Y e + R/r grad Y)]
-                                const T a = dR[n] * re_mul(V, Y), b = R[n] * invr;
-                                Fx += fma(a, ex, b * re_mul(V, g[0]));
-                                Fy += fma(a, ey, b * re_mul(V, g[1]));
-                                Fz += fma(a, ez, b * re_mul(V, g[2]));
+                                S1.re = fma(dR[n], V.re, S1.re);
+                                S1.im = fma(dR[n], V.im, S1.im);
+                                S2.re = fma(R[n], V.re, S2.re);
+                                S2.im = fma(R[n], V.im, S2.im);
                             }
+#pragma unroll
+                            for (int q = 0; q < NM3X; ++q)
+                                if (q < P.nm3 && (qmp[q] > 0 || (qmp[q] == 0 && sg == 0))) cfma(U[q], Tq[q], Y);
+                            // Re[S1 Y] e + Re[S2 grad Y] / r
+                            const T a = re_mul(S1, Y);
+                            Fx += fma(a, ex, invr * re_mul(S2, g[0]));
+                            Fy += fma(a, ey, invr * re_mul(S2, g[1]));
+                            Fz += fma(a, ez, invr * re_mul(S2, g[2]));
                         }
                     }
                 }
