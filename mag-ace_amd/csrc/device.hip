@@ -825,14 +825,17 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
     const cplx<T>* W = sW;
     const int nfa = P.nr3a * (P.l3a + 1);
     const double edf = edfin[atom];
-    // magnetic column q of A -> m' (for the representative test)
-    int qmp[NM3X];
+    // magnetic column q of A -> m' (for the representative test): column q takes part with
+    // m >= 0 (use0) when m' >= 0, with m < 0 (use1, the conjugated half) when m' > 0
+    bool use0[NM3X], use1[NM3X];
 #pragma unroll
     for (int q = 0; q < NM3X; ++q) {
         const int lq = q % P.lsm3;
         int lp = 0;
         while ((lp + 1) * (lp + 1) <= lq) ++lp;
-        qmp[q] = lq - lp * lp - lp;
+        const int mp = lq - lp * lp - lp;
+        use0[q] = q < P.nm3 && mp >= 0;
+        use1[q] = q < P.nm3 && mp > 0;
     }
 
     for (int c0 = 0; c0 < nn; c0 += 64) {
@@ -925,21 +928,38 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                             // sum_n W[n][lm][q] R_n Y: both are summed over n first
                             // (S1 = sum_n R'_n V_n, S2 = sum_n R_n V_n, Tq = sum_n R_n W[n][lm][q])
                             // and meet Y and grad Y once per (l, m)
-                            cplx<T> S1 = {0, 0}, S2 = {0, 0}, Tq[NM3X];
+                            constexpr int NTQ = NM3X <= 8 ? NM3X : 1;
+                            cplx<T> S1 = {0, 0}, S2 = {0, 0}, Tq[NTQ];
 #pragma unroll
-                            for (int q = 0; q < NM3X; ++q) Tq[q] = {0, 0};
+                            for (int q = 0; q < NTQ; ++q) Tq[q] = {0, 0};
 #pragma unroll
                             for (int n = 0; n < NRAX; ++n) {
                                 if (n >= P.nr3a) break;
                                 const cplx<T>* Wn = W + (n * P.lsa + lm) * P.nm3;
+                                // the row's reads are issued together, outside the uniform
+                                // branches, so their LDS latency overlaps (a read inside each
+                                // branch waited for its own result)
+                                // (wide classes, 27 columns, read inside the branches: the
+                                // hoisted values would not fit in registers)
+                                constexpr bool kHoist = NM3X <= 8;
+                                cplx<T> wv[kHoist ? NM3X : 1];
+                                if constexpr (kHoist) {
+#pragma unroll
+                                    for (int q = 0; q < NM3X; ++q) wv[q] = Wn[q < P.nm3 ? q : 0];
+                                }
                                 cplx<T> V = {0, 0};
+                                const cplx<T> ry = {R[n] * Y.re, R[n] * Y.im};
 #pragma unroll
                                 for (int q = 0; q < NM3X; ++q) {
-                                    if (q < P.nm3 && (qmp[q] > 0 || (qmp[q] == 0 && sg == 0))) {
-                                        const cplx<T> wq = Wn[q];
+                                    if (sg ? use1[q] : use0[q]) {
+                                        const cplx<T> wq = kHoist ? wv[kHoist ? q : 0] : Wn[q];
                                         cfma(V, wq, H[q]);
-                                        Tq[q].re = fma(R[n], wq.re, Tq[q].re);
-                                        Tq[q].im = fma(R[n], wq.im, Tq[q].im);
+                                        if constexpr (kHoist) {
+                                            Tq[kHoist ? q : 0].re = fma(R[n], wq.re, Tq[kHoist ? q : 0].re);
+                                            Tq[kHoist ? q : 0].im = fma(R[n], wq.im, Tq[kHoist ? q : 0].im);
+                                        } else {
+                                            cfma(U[q], wq, ry);  // wide classes: Tq would not fit
+                                        }
                                     }
                                 }
                                 S1.re = fma(dR[n], V.re, S1.re);
@@ -947,9 +967,11 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                                 S2.re = fma(R[n], V.re, S2.re);
                                 S2.im = fma(R[n], V.im, S2.im);
                             }
+                            if constexpr (NM3X <= 8) {
 #pragma unroll
-                            for (int q = 0; q < NM3X; ++q)
-                                if (q < P.nm3 && (qmp[q] > 0 || (qmp[q] == 0 && sg == 0))) cfma(U[q], Tq[q], Y);
+                                for (int q = 0; q < NTQ; ++q)
+                                    if (sg ? use1[q] : use0[q]) cfma(U[q], Tq[q], Y);
+                            }
                             // Re[S1 Y] e + Re[S2 grad Y] / r
                             const T a = re_mul(S1, Y);
                             Fx += fma(a, ex, invr * re_mul(S2, g[0]));
