@@ -1189,26 +1189,32 @@ __global__ void __launch_bounds__(256) k_density_jvp(DevParams<double> P, const 
 
     const int nblkA = (P.nr3a + NB - 1) / NB, nblkA2 = (P.nb_a + NB2 - 1) / NB2;
     const int nTA = P.lsa * nblkA, nTA2 = P.lsm * nblkA2;
-    for (int round = 0; round * nthr < nTA + nTA2; ++round) {
+    // A2 tasks start at a wave boundary when that fits in one round (no wave runs both loops)
+    const int a2base = ((nTA + 63) & ~63) + nTA2 <= nthr ? (nTA + 63) & ~63 : nTA;
+    for (int round = 0; round * nthr < a2base + nTA2; ++round) {
         const int task = round * nthr + tid;
         cplx<T> acc[NB > NB2 ? NB : NB2][NM3X > NR2X ? NM3X : NR2X];
 #pragma unroll
         for (int a = 0; a < (NB > NB2 ? NB : NB2); ++a)
 #pragma unroll
             for (int b = 0; b < (NM3X > NR2X ? NM3X : NR2X); ++b) acc[a][b] = {0, 0};
-        const bool isA = task < nTA, isA2 = !isA && task < nTA + nTA2;
+        const bool isA = task < nTA, isA2 = task >= a2base && task < a2base + nTA2;
         int lm = 0, l = 0, m = 0, n0 = 0;
         if (isA) { lm = task % P.lsa; n0 = (task / P.lsa) * NB; }
-        if (isA2) { lm = (task - nTA) % P.lsm; n0 = ((task - nTA) / P.lsm) * NB2; }
+        if (isA2) { lm = (task - a2base) % P.lsm; n0 = ((task - a2base) / P.lsm) * NB2; }
         while ((l + 1) * (l + 1) <= lm) ++l;
         m = lm - l * l - l;
         for (int c0 = 0; c0 < nn; c0 += CH) {
             const int nj = nn - c0 < CH ? nn - c0 : CH;
             __syncthreads();
-            if (tid < CH) {  // atomic side: values and their derivatives along u
-                const int j = tid;
-                if (j < nj) {
-                    const int p = p0 + c0 + j;
+            // staging of pair c0 + j spread over the block (as in k_density): role 0 the bond
+            // Y_lm and D_u Y_lm and the core term, roles 1..nsplit shares of the radial
+            // functions and their derivatives along u, role nsplit + 1 the magnetic side
+            const int j = tid % CH, role = tid / CH, nsplit = nthr / CH - 2;
+            const bool live = j < nj;
+            const int p = p0 + c0 + (live ? j : 0);
+            if (role <= nsplit) {
+                if (live) {
                     const int jn = nlist[p];
                     const double dux = uix - du[3 * (size_t)jn], duy = uiy - du[3 * (size_t)jn + 1],
                                  duz = uiz - du[3 * (size_t)jn + 2];
@@ -1216,29 +1222,31 @@ __global__ void __launch_bounds__(256) k_density_jvp(DevParams<double> P, const 
                     const double rdot = ex * dux + ey * duy + ez * duz;
                     T w;
                     const T* row = atomic_row(P, dlist[p], w, err);
-                    for (int k = 0; k < P.nb_a; ++k) {
-                        T v = 0, dv = 0;
-                        if (row) horner(row + 4 * k, w, (T)P.rs_a, v, dv);
-                        sGd[j * P.nb_a + k] = dv * rdot;
+                    if (role == 0) {
+                        T cr = 0, dcr = 0;
+                        if (row) horner(row + 4 * (P.nb_a + nfa), w, (T)P.rs_a, cr, dcr);
+                        sCrd[j] = dcr * rdot;
+                        ylm_half_jvp<L>(ex, ey, ez, P.l3a, dux, duy, duz, 1.0 / dlist[p], sY + j * P.lha,
+                                        sYd + j * P.lha);
+                    } else {
+                        const int nf = P.nb_a + nfa, per = (nf + nsplit - 1) / nsplit;
+                        const int k0 = (role - 1) * per, k1 = min(nf, k0 + per);
+                        for (int k = k0; k < k1; ++k) {
+                            T v = 0, dv = 0;
+                            if (row) horner(row + 4 * k, w, (T)P.rs_a, v, dv);
+                            if (k < P.nb_a) {
+                                sGd[j * P.nb_a + k] = dv * rdot;
+                            } else {
+                                sR[j * nfa + (k - P.nb_a)] = v;
+                                sRd[j * nfa + (k - P.nb_a)] = dv * rdot;
+                            }
+                        }
                     }
-                    for (int k = 0; k < nfa; ++k) {
-                        T v = 0, dv = 0;
-                        if (row) horner(row + 4 * (P.nb_a + k), w, (T)P.rs_a, v, dv);
-                        sR[j * nfa + k] = v;
-                        sRd[j * nfa + k] = dv * rdot;
-                    }
-                    T cr = 0, dcr = 0;
-                    if (row) horner(row + 4 * (P.nb_a + nfa), w, (T)P.rs_a, cr, dcr);
-                    sCrd[j] = dcr * rdot;
-                    ylm_half_jvp<L>(ex, ey, ez, P.l3a, dux, duy, duz, 1.0 / dlist[p], sY + j * P.lha,
-                                    sYd + j * P.lha);
-                } else {
+                } else if (role == 0) {
                     sCrd[j] = 0;
                 }
-            } else if (tid < 2 * CH) {  // magnetic side (does not move with positions)
-                const int j = tid - CH;
-                if (j < nj) {
-                    const int p = p0 + c0 + j;
+            } else if (role == nsplit + 1) {  // magnetic side (does not move with positions)
+                if (live) {
                     T w;
                     bool unit;
                     const T* row = magnetic_row(P, dmom[p], w, unit, err);
