@@ -214,7 +214,7 @@ __device__ __forceinline__ cplx<T> yfull(const cplx<T>* half, int l, int m) {
 // 0.91 -> 0.77 ms on the lmax-6 workload, 0.52 -> 0.46 on lmax 4) and 3 in FP64 (168 registers;
 // at 169 the occupancy halves and the kernel takes 1.85 instead of 1.39 ms).  Class 1 (27
 // magnetic columns) would spill.
-template <typename T, int L, int CH, int NB, int NM3X, int NB2, int NR2X, bool SPLIT>
+template <typename T, int L, int CH, int NB, int NM3X, int NB2, int NR2X, bool SPLIT, bool TASKSPLIT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6 ? (sizeof(T) == 4 ? 5 : 3) : 1, 8))) k_density(DevParams<T> P, const int* __restrict__ off,
                                                 const double* __restrict__ dlist,
                                                 const double* __restrict__ elist,
@@ -278,13 +278,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
 
     const int nblkA = (P.nr3a + NB - 1) / NB, nblkA2 = (P.nb_a + NB2 - 1) / NB2;
     const int nTA = P.lsa * nblkA, nTA2 = P.lsm * nblkA2;
-    // When the A and A2 column tasks leave most threads idle (FP32, lmax-4 headers: 75 tasks
-    // for 256 threads), each A task's neighbours are split over SA threads (neighbour j on
-    // part j mod SA); the parts meet in LDS at the end, in the staging area, which is free by
-    // then.  Measured on the 16000-atom workloads: lmax 4 FP32 0.61 -> 0.53 ms; with SA = 2
-    // (lmax 6) and in FP64 the extra barriers and scratch traffic cost more than they gain.
+    // TASKSPLIT (light headers): when the A and A2 column tasks leave most threads idle
+    // (lmax-4 headers: 75 tasks for 256 threads), each A task's neighbours are split over SA
+    // threads (neighbour j on part j mod SA); the parts meet in LDS at the end, in the staging
+    // area, which is free by then.  With SA = 2 (lmax 6) the extra barriers and scratch
+    // traffic cost more than they gain, so SA < 3 falls back to 1.
     int SA = 1;
-    if constexpr (sizeof(T) == 4 && !SPLIT) if (nTA + nTA2 <= nthr) {
+    if constexpr (TASKSPLIT) if (nTA + nTA2 <= nthr) {
         SA = min(4, (nthr - nTA2) / max(nTA, 1));
         const size_t room = (size_t)((const char*)sEhc - (const char*)smem);
         while (SA > 1 && (size_t)(SA - 1) * nTA * NB * NM3X * sizeof(cplx<T>) > room) --SA;
@@ -292,7 +292,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
     }
     // A2 tasks start at a wave boundary when that still fits in one round: a wave holding both
     // kinds would run both loops one after the other
-    const int nAt = (sizeof(T) == 4 && !SPLIT ? SA : 1) * nTA;
+    const int nAt = (TASKSPLIT ? SA : 1) * nTA;
     const int a2base = ((nAt + 63) & ~63) + nTA2 <= nthr ? (nAt + 63) & ~63 : nAt;
     for (int round = 0; round * nthr < a2base + nTA2; ++round) {
         const int task = round * nthr + tid;
@@ -302,9 +302,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
 #pragma unroll
             for (int b = 0; b < (NM3X > NR2X ? NM3X : NR2X); ++b) acc[a][b] = {0, 0};
         // decode task
-        // only light FP32 headers (classic staging) split tasks; the others keep a
-        // compile-time SA = 1
-        constexpr bool kSplit = sizeof(T) == 4 && !SPLIT;
+        // only light headers split tasks; the others keep a compile-time SA = 1
+        constexpr bool kSplit = TASKSPLIT;
         const int sa = kSplit ? SA : 1;
         const bool isA = task < sa * nTA, isA2 = task >= a2base && task < a2base + nTA2;
         const int ta = kSplit && isA ? task % nTA : task, part = kSplit && isA ? task / nTA : 0;
@@ -1604,7 +1603,7 @@ int supported_nm3_class(int nm3, int nr2m, int nb_a, int l3a, int l2m, int nr3a)
     return -1;
 }
 
-template <typename T, int L, int CH, bool SPLIT, int NB, int NM3X, int NB2, int NR2X, int NRAX, int NBAX>
+template <typename T, int L, int CH, bool SPLIT, bool TASKSPLIT, int NB, int NM3X, int NB2, int NR2X, int NRAX, int NBAX>
 static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, const double* dlist,
                             const double* elist, const double* dmomi, const double* emomi,
                             const double* dmom, const double* emom, double* eout, double* flist,
@@ -1613,7 +1612,7 @@ static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, co
     Pd.nr3a = P.nr3a; Pd.l3a = P.l3a; Pd.nr2m = P.nr2m; Pd.l2m = P.l2m; Pd.lha = P.lha;
     Pd.lhm = P.lhm; Pd.nm3 = P.nm3; Pd.nb_a = P.nb_a; Pd.nb_m = P.nb_m;
     const size_t lds1 = density_lds_bytes(Pd, CH, sizeof(T));
-    auto k1 = k_density<T, L, CH, NB, NM3X, NB2, NR2X, SPLIT>;
+    auto k1 = k_density<T, L, CH, NB, NM3X, NB2, NR2X, SPLIT, TASKSPLIT>;
     const bool nd2 = P.nd == 2;
     const void* k2 = nd2 ? (const void*)k_adjoint<T, true> : (const void*)k_adjoint<T, false>;
     auto k3 = k_forces<T, L, L, NM3X, NBAX, NRAX, NR2X>;
@@ -1671,15 +1670,18 @@ hipError_t Launch<T>::run(const DevParams<T>& P, int natom, const int* off, cons
     // Density staging: with many radial functions per pair (lmax-6 headers) or in FP64 the
     // per-pair staging is spread over the whole block, 32 neighbours per round; light FP32
     // headers stage 64 neighbours per round, one thread per pair and side (measured on the
-    // 16000-atom workload: lmax 6 0.91 vs 1.07 ms, lmax 4 0.80 vs 0.60 ms).
-    const bool spread = sizeof(T) == 8 || P.nb_a + P.nr3a * (P.l3a + 1) >= 40;
-#define MAGACE_RUN(CH, SPLIT)                                                                              \
-    (cls == 0 ? run_class<T, 6, CH, SPLIT, 4, 4, 4, 2, 8, 10>(P, natom, off, dlist, elist, dmomi, emomi, dmom,  \
-                                                              emom, eout, flist, tlist, tlist0, B, s, ev)   \
-              : run_class<T, 2, CH, SPLIT, 1, 27, 2, 5, 4, 16>(P, natom, off, dlist, elist, dmomi, emomi, dmom, \
-                                                               emom, eout, flist, tlist, tlist0, B, s, ev))
-    if (spread) return MAGACE_RUN(32, true);
-    return MAGACE_RUN(64, false);
+    // 16000-atom workload: lmax 6 0.91 vs 1.07 ms, lmax 4 0.80 vs 0.60 ms).  Light headers
+    // also split their A-column tasks over idle threads (lmax 4: FP32 0.61 -> 0.52 ms, FP64
+    // 0.93 -> 0.81 ms; with lmax 6 the runtime split costs FP64 3%).
+    const bool heavy = P.nb_a + P.nr3a * (P.l3a + 1) >= 40;
+#define MAGACE_RUN(CH, SPLIT, TS)                                                                          \
+    (cls == 0 ? run_class<T, 6, CH, SPLIT, TS, 4, 4, 4, 2, 8, 10>(P, natom, off, dlist, elist, dmomi, emomi,  \
+                                                                  dmom, emom, eout, flist, tlist, tlist0, B, s, ev) \
+              : run_class<T, 2, CH, SPLIT, TS, 1, 27, 2, 5, 4, 16>(P, natom, off, dlist, elist, dmomi, emomi,  \
+                                                                   dmom, emom, eout, flist, tlist, tlist0, B, s, ev))
+    if (heavy) return MAGACE_RUN(32, true, false);
+    if (sizeof(T) == 8) return MAGACE_RUN(32, true, true);
+    return MAGACE_RUN(64, false, true);
 #undef MAGACE_RUN
 }
 
