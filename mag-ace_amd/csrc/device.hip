@@ -797,7 +797,10 @@ __device__ __forceinline__ void conj_all(cplx<T>& Y, cplx<T> (&g)[3], int am) {
 // NBAX: atomic radial base bound; NRAX: nr3a bound; NR2X: nr2m bound.
 // W is folded onto representatives of conjugate pairs (A: m' > 0, or m' = 0 and m >= 0;
 // A2, A0: m' >= 0), so only those index pairs are visited.
-template <typename T, int LA, int LM, int NM3X, int NBAX, int NRAX, int NR2X>
+// Q1: the magnetic columns of A are (l', m') = (0,0), (1,-1), (1,0), (1,1) (l3m = 1, one radial
+// function), so the representative columns are known at compile time: 0, 2, 3 for m >= 0 and
+// 3 for m < 0, and no other W entry is read.
+template <typename T, int LA, int LM, int NM3X, int NBAX, int NRAX, int NR2X, bool Q1>
 __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __restrict__ off,
                                                const double* __restrict__ dlist,
                                                const double* __restrict__ elist,
@@ -836,6 +839,11 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
         use0[q] = q < P.nm3 && mp >= 0;
         use1[q] = q < P.nm3 && mp > 0;
     }
+    static_assert(!Q1 || NM3X == 4, "Q1 needs four magnetic columns");
+    auto used = [&](int q, int sg) -> bool {
+        if constexpr (Q1) return sg ? q == 3 : q != 1;
+        return sg ? use1[q] : use0[q];
+    };
 
     for (int c0 = 0; c0 < nn; c0 += 64) {
         const int j = c0 + lane;
@@ -944,13 +952,14 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                                 cplx<T> wv[kHoist ? NM3X : 1];
                                 if constexpr (kHoist) {
 #pragma unroll
-                                    for (int q = 0; q < NM3X; ++q) wv[q] = Wn[q < P.nm3 ? q : 0];
+                                    for (int q = 0; q < NM3X; ++q)
+                                        if (!Q1 || used(q, sg)) wv[q] = Wn[q < P.nm3 ? q : 0];
                                 }
                                 cplx<T> V = {0, 0};
                                 const cplx<T> ry = {R[n] * Y.re, R[n] * Y.im};
 #pragma unroll
                                 for (int q = 0; q < NM3X; ++q) {
-                                    if (sg ? use1[q] : use0[q]) {
+                                    if (used(q, sg)) {
                                         const cplx<T> wq = kHoist ? wv[kHoist ? q : 0] : Wn[q];
                                         cfma(V, wq, H[q]);
                                         if constexpr (kHoist) {
@@ -969,7 +978,7 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                             if constexpr (NM3X <= 8) {
 #pragma unroll
                                 for (int q = 0; q < NTQ; ++q)
-                                    if (sg ? use1[q] : use0[q]) cfma(U[q], Tq[q], Y);
+                                    if (used(q, sg)) cfma(U[q], Tq[q], Y);
                             }
                             // Re[S1 Y] e + Re[S2 grad Y] / r
                             const T a = re_mul(S1, Y);
@@ -1615,20 +1624,25 @@ static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, co
     auto k1 = k_density<T, L, CH, NB, NM3X, NB2, NR2X, SPLIT, TASKSPLIT>;
     const bool nd2 = P.nd == 2;
     const void* k2 = nd2 ? (const void*)k_adjoint<T, true> : (const void*)k_adjoint<T, false>;
-    auto k3 = k_forces<T, L, L, NM3X, NBAX, NRAX, NR2X>;
+    // the Q1 instantiation exists only for the four-column class
+    constexpr bool kQ1 = NM3X == 4;
+    const bool q1 = kQ1 && P.nm3 == 4 && P.lsm3 == 4;
+    auto k3q = k_forces<T, L, L, NM3X, NBAX, NRAX, NR2X, kQ1>;
+    auto k3g = k_forces<T, L, L, NM3X, NBAX, NRAX, NR2X, false>;
+    auto k3 = q1 ? k3q : k3g;
     hipError_t e = hipSuccess;
     // the dynamic-LDS limit is raised once per kernel instantiation (to the workgroup
     // maximum), so that evaluations can be captured into graphs without further API calls
     static bool attributes_set = false;
-    static size_t limit[4] = {0, 0, 0, 0};
+    static size_t limit[5] = {0, 0, 0, 0, 0};
     if (!attributes_set) {
         int dev = 0, maxsh = 0;
         if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
         if ((e = hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev)) != hipSuccess)
             return e;
-        const void* ks[4] = {(const void*)k1, (const void*)k_adjoint<T, false>, (const void*)k3,
-                             (const void*)k_adjoint<T, true>};
-        for (int i = 0; i < 4; ++i) {
+        const void* ks[5] = {(const void*)k1, (const void*)k_adjoint<T, false>, (const void*)k3q,
+                             (const void*)k_adjoint<T, true>, (const void*)k3g};
+        for (int i = 0; i < 5; ++i) {
             hipFuncAttributes fa;
             if ((e = hipFuncGetAttributes(&fa, ks[i])) != hipSuccess) return e;
             limit[i] = (size_t)maxsh > fa.sharedSizeBytes ? (size_t)maxsh - fa.sharedSizeBytes : 0;
@@ -1640,7 +1654,7 @@ static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, co
     }
     const size_t lds2 = (nd2 ? 3 : 2) * (size_t)(P.nz + 1) * sizeof(cplx<T>) + 8 * sizeof(double);
     const size_t lds3 = (size_t)P.nz * sizeof(cplx<T>);
-    if (lds1 > limit[0] || lds2 > limit[nd2 ? 3 : 1] || lds3 > limit[2]) return hipErrorInvalidValue;
+    if (lds1 > limit[0] || lds2 > limit[nd2 ? 3 : 1] || lds3 > limit[q1 ? 2 : 4]) return hipErrorInvalidValue;
     if (ev) (void)hipEventRecord(ev[0], s);
     hipLaunchKernelGGL(k1, dim3(natom), dim3(256), lds1, s, P, off, dlist, elist, dmomi, emomi, dmom,
                        emom, (cplx<T>*)B.Z, B.ehc, B.rho1, B.err);
