@@ -171,8 +171,8 @@ def main():
                     help="first run one plain 512 MiB device copy (the reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build evaluates on one GPU only; multi-GPU runs are not part of it "
-              "(see DESIGN.md for how the path would shard)", file=sys.stderr)
+        print("bench.py: the evaluator runs on one GPU; a multi-GPU run (spatial decomposition "
+              "with ghost exchange) is not implemented", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -126,7 +126,7 @@ def test_pair_compute_matches_python_assembly(tmp_path):
 
 @pytest.mark.parametrize("hname,ncell,spins", [("H500", 10, "uniform"), ("H2000", 5, "cone")])
 def test_larger_system_against_oracle(hname, ncell, spins, tmp_path):
-    """2000-atom bcc Fe (configs[1] shape) and 250 atoms at the lmax-6 header vs the oracle."""
+    """2000-atom bcc Fe (lmax-4 header) and 250 atoms at the lmax-6 header vs the oracle."""
     say("case", hname, ncell, spins)
     pot = write_potential(str(tmp_path / "p.in"), hname, 2027)
     s = PKG.systems.make_system(ncell, 2027, spins)
