@@ -5,19 +5,29 @@
 #include <cstdlib>
 #include <cstring>
 
+#include <unistd.h>
+
 namespace magace {
 
 HostPool& HostPool::get() {
-    static HostPool pool([] {
+    // one pool per process: a child made by fork() inherits the object but not its threads,
+    // so it builds a pool of its own (the parent's copy is left alone; its threads are not
+    // the child's to join)
+    static std::mutex mu;
+    static HostPool* pool = nullptr;
+    static pid_t owner = 0;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!pool || owner != getpid()) {
         int n = (int)std::thread::hardware_concurrency();
         n = std::max(1, std::min(8, n));
         if (const char* s = std::getenv("MAGACE_COPY_THREADS")) {
             const int v = std::atoi(s);
             if (v >= 1 && v <= 64) n = v;
         }
-        return n;
-    }());
-    return pool;
+        pool = new HostPool(n);
+        owner = getpid();
+    }
+    return *pool;
 }
 
 HostPool::HostPool(int nthreads) {

@@ -14,8 +14,9 @@ namespace magace {
 
 class HostPool {
 public:
-    // the process-wide pool; its size is MAGACE_COPY_THREADS if set (1..64), else
-    // min(8, hardware threads)
+    // the process-wide pool, made at the first call (again in a forked child); its size is
+    // MAGACE_COPY_THREADS if set (1..64), else min(8, hardware threads).  It lives until the
+    // process ends.
     static HostPool& get();
 
     explicit HostPool(int nthreads);
