@@ -26,6 +26,9 @@
 #include "magace.hpp"
 
 namespace magace {
+
+static_assert(TermList::kLanes == kAdjointThreads, "the chunk order must be built for the adjoint's workgroup size");
+
 namespace {
 
 std::string g_err;
@@ -74,6 +77,7 @@ struct Context {
     DevTables<float> t32;
     uint4* crec64 = nullptr;
     uint4* crec32 = nullptr;
+    uint4* crec16 = nullptr;  // FP32 weights, offsets of 16-byte entries (two-atom adjoint)
     double* cw2_64 = nullptr;  // density-2 record weights (ndensity 2)
     float* cw2_32 = nullptr;
     int2* chunks = nullptr;
@@ -129,7 +133,7 @@ void dfree(T*& p) {
 
 Context::~Context() {
     dfree(t64.lut_a); dfree(t64.lut_m); dfree(t32.lut_a); dfree(t32.lut_m);
-    dfree(crec64); dfree(crec32); dfree(chunks); dfree(fold); dfree(cw2_64); dfree(cw2_32);
+    dfree(crec64); dfree(crec32); dfree(crec16); dfree(chunks); dfree(fold); dfree(cw2_64); dfree(cw2_32);
     dfree(buf.Z); dfree(buf.W); dfree(buf.ehc); dfree(buf.rho1); dfree(buf.dE); dfree(buf.edf); dfree(buf.rho);
     dfree(buf.err);
     dfree(fit.Zdot); dfree(fit.ehc_dot); dfree(fit.alpha); dfree(fit.beta); dfree(fit.M); dfree(fit.idx);
@@ -238,7 +242,7 @@ int build_context(Potential&& pot, std::unique_ptr<Context>& out) {
 int upload_records(Context& c) {
     const TermList& T = c.model.terms;
     const size_t nc = T.ncontrib();
-    std::vector<uint4> r64(nc), r32(nc);
+    std::vector<uint4> r64(nc), r32(nc), r16(nc);
     for (size_t i = 0; i < nc; ++i) {
         const uint16_t* q = &T.cidx[4 * i];
         const double w = T.cw[i];
@@ -250,9 +254,11 @@ int upload_records(Context& c) {
         r64[i] = make_uint4(((uint32_t)q[0] << 4) | ((uint32_t)q[1] << 20), (uint32_t)q[2] << 4,
                             (uint32_t)(wb & 0xffffffffu), (uint32_t)(wb >> 32));
         r32[i] = make_uint4(((uint32_t)q[0] << 3) | ((uint32_t)q[1] << 19), (uint32_t)q[2] << 3, fb, 0u);
+        r16[i] = make_uint4(((uint32_t)q[0] << 4) | ((uint32_t)q[1] << 20), (uint32_t)q[2] << 4, fb, 0u);
     }
     HIPCHK(hipMemcpy(c.crec64, r64.data(), nc * sizeof(uint4), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c.crec32, r32.data(), nc * sizeof(uint4), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c.crec16, r16.data(), nc * sizeof(uint4), hipMemcpyHostToDevice));
     if (T.nd > 1) {
         if (!c.cw2_64) {
             HIPCHK(hipMalloc((void**)&c.cw2_64, nc * sizeof(double) + 64));
@@ -309,6 +315,7 @@ int upload(Context& c) {
     const size_t nc = M.terms.ncontrib(), nch = M.terms.chunks.size() / 2;
     HIPCHK(hipMalloc((void**)&c.crec64, nc * sizeof(uint4) + 64));
     HIPCHK(hipMalloc((void**)&c.crec32, nc * sizeof(uint4) + 64));
+    HIPCHK(hipMalloc((void**)&c.crec16, nc * sizeof(uint4) + 64));
     HIPCHK(hipMalloc((void**)&c.chunks, nch * sizeof(int2) + 64));
     HIPCHK(hipMalloc((void**)&c.fold, M.terms.fold.size() * sizeof(uint32_t) + 64));
     HIPCHK(hipMemcpy(c.fold, M.terms.fold.data(), M.terms.fold.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -322,6 +329,7 @@ int upload(Context& c) {
     c.P32.lut_a = c.t32.lut_a; c.P32.lut_m = c.t32.lut_m;
     c.P64.crec = c.crec64;
     c.P32.crec = c.crec32;
+    c.P32.crec16 = c.crec16;
     c.P64.chunks = c.P32.chunks = c.chunks;
     c.P64.fold = c.P32.fold = c.fold;
     c.uploaded = true;

@@ -409,8 +409,9 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
         for (size_t i = 0; i < v.size(); i += TermList::kChunk)
             rowchunks[x].push_back({x, (int)i, (int)std::min<size_t>(TermList::kChunk, v.size() - i)});
     }
-    // Order of the chunks.  Chunk c runs on thread c mod 256 in round c / 256, so the order
-    // is a grid of rounds (rows) by 256 lanes (columns); the 64 chunks of a wave in a round
+    // Order of the chunks.  Chunk c runs on thread c mod L in round c / L (L = kLanes, the
+    // adjoint's workgroup size), so the order is a grid of rounds (rows) by L lanes
+    // (columns); the 64 chunks of a wave in a round
     // form one storage group.  Full chunks fill the first whole rounds column by column,
     // targets in decreasing number of full chunks: a lane then meets the same target in
     // consecutive rounds and keeps its sum in registers (one LDS atomic per run instead of
@@ -432,10 +433,11 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
         std::vector<Ch> full;
         for (int x : tg)
             for (const Ch& ch : rowchunks[x]) (ch.cnt == TermList::kChunk ? full : rest).push_back(ch);
-        const size_t rounds = full.size() / 256;
-        order.resize(rounds * 256);
-        for (size_t q = 0; q < rounds * 256; ++q) order[(q % rounds) * 256 + q / rounds] = full[q];
-        rest.insert(rest.begin(), full.begin() + rounds * 256, full.end());
+        const size_t L = TermList::kLanes;
+        const size_t rounds = full.size() / L;
+        order.resize(rounds * L);
+        for (size_t q = 0; q < rounds * L; ++q) order[(q % rounds) * L + q / rounds] = full[q];
+        rest.insert(rest.begin(), full.begin() + rounds * L, full.end());
         std::stable_sort(rest.begin(), rest.end(), [](const Ch& a, const Ch& b) { return a.cnt > b.cnt; });
         order.insert(order.end(), rest.begin(), rest.end());
     }

@@ -580,10 +580,42 @@ template <> __device__ __forceinline__ cplx<double> lds_cplx<double>(const char*
     return {v.x, v.y};
 }
 
+// E_i = f_in(e_hc) sum_d F(rho_d) + e_hc (src/forces.F90:1066-1128) for nd densities whose
+// B parts sum to tots[d]; writes eout, dE_i/drho_d = fc F'(rho_d) (also into scale[d]), the
+// e_hc factor and rho
+template <typename T>
+__device__ inline void atom_energy(const DevParams<T>& P, int nd, int atom, const double* tots,
+                                   const double* __restrict__ ehc, const double* __restrict__ rho1,
+                                   double* __restrict__ eout, double* __restrict__ dEout,
+                                   double* __restrict__ edfout, double* __restrict__ rho_out, double* scale) {
+    double Fs = 0.0, dF[2] = {0.0, 0.0};
+    for (int d = 0; d < nd; ++d) {
+        const double r = tots[d] + rho1[(size_t)atom * nd + d];
+        rho_out[(size_t)atom * nd + d] = r;
+        double F;
+        embed(P.nfemb, r, P.p1[d][0], P.p1[d][1], F, dF[d]);
+        Fs += F;
+    }
+    const double e = ehc[atom], elow = P.ecut - P.decut;
+    double fc = 1.0, dfc = 0.0;
+    if (e >= P.ecut) { fc = 0.0; dfc = 0.0; }
+    else if (e > elow) {
+        const double pi = 3.14159265358979323846;
+        fc = 0.5 * (1.0 + cos(pi * (e - elow) / P.decut));
+        dfc = -0.5 * sin(pi * (e - elow) / P.decut) * pi / P.decut;
+    }
+    eout[atom] = fc * Fs + e;
+    for (int d = 0; d < nd; ++d) {
+        dEout[(size_t)atom * nd + d] = fc * dF[d];
+        scale[d] = fc * dF[d];
+    }
+    edfout[atom] = dfc * Fs + 1.0;
+}
+
 // ND2: two densities (rho_d = sum_k c_{k,d} B_k, E = fc sum_d F(rho_d) + e_hc,
 // src/forces.F90:976-1089): a second weight per record, a second gradient array in LDS
 template <typename T, bool ND2>
-__global__ void __launch_bounds__(256) k_adjoint(DevParams<T> P, const cplx<T>* __restrict__ Zin,
+__global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, const cplx<T>* __restrict__ Zin,
                                                 cplx<T>* __restrict__ Wout,
                                                 const double* __restrict__ ehc,
                                                 const double* __restrict__ rho1,
@@ -611,8 +643,8 @@ __global__ void __launch_bounds__(256) k_adjoint(DevParams<T> P, const cplx<T>* 
     // Gather form of the gradient: a thread sums one chunk of contributions
     // w * Z[a] Z[b] Z[c] to a single target x in registers (four 16-byte records per step,
     // loads issued together; the 64 lanes of a wave read consecutive records).  Thread t
-    // takes chunks t + 256 j; the chunk -> thread mapping must match the record layout built
-    // by build_terms.  While the thread's next chunk has the same target the sum stays in
+    // takes chunks t + kAdjointThreads j; the chunk -> thread mapping must match the record
+    // layout built by build_terms.  While the thread's next chunk has the same target the sum stays in
     // registers; it goes to LDS with one atomic when the target changes (a plain store when
     // the chunk is its target's only one).  Each block starts
     // at a different j (rotated by its index), so that the blocks of an XCD stream different
@@ -688,33 +720,11 @@ __global__ void __launch_bounds__(256) k_adjoint(DevParams<T> P, const cplx<T>* 
     double tot2 = 0.0;
     if (ND2) {
         __syncthreads();
-        tot2 = block_sum<T>(rho2, red + 4);
+        tot2 = block_sum<T>(rho2, red + kAdjointThreads / 64);
     }
     if (tid == 0) {
-        // E_i = f_in(e_hc) sum_d F(rho_d) + e_hc (src/forces.F90:1066-1128)
-        const int nd = ND2 ? 2 : 1;
-        double Fs = 0.0, dF[2] = {0.0, 0.0};
-        for (int d = 0; d < nd; ++d) {
-            const double r = (d == 0 ? tot : tot2) + rho1[(size_t)atom * nd + d];
-            rho_out[(size_t)atom * nd + d] = r;
-            double F;
-            embed(P.nfemb, r, P.p1[d][0], P.p1[d][1], F, dF[d]);
-            Fs += F;
-        }
-        const double e = ehc[atom], elow = P.ecut - P.decut;
-        double fc = 1.0, dfc = 0.0;
-        if (e >= P.ecut) { fc = 0.0; dfc = 0.0; }
-        else if (e > elow) {
-            const double pi = 3.14159265358979323846;
-            fc = 0.5 * (1.0 + cos(pi * (e - elow) / P.decut));
-            dfc = -0.5 * sin(pi * (e - elow) / P.decut) * pi / P.decut;
-        }
-        eout[atom] = fc * Fs + e;
-        for (int d = 0; d < nd; ++d) {
-            dEout[(size_t)atom * nd + d] = fc * dF[d];
-            s_scale[d] = fc * dF[d];
-        }
-        edfout[atom] = dfc * Fs + 1.0;
+        const double tots[2] = {tot, tot2};
+        atom_energy<T>(P, ND2 ? 2 : 1, atom, tots, ehc, rho1, eout, dEout, edfout, rho_out, s_scale);
     }
     __syncthreads();
     // W_eff on representatives of conjugate pairs: since Z_xbar = s conj(Z_x),
@@ -737,6 +747,134 @@ __global__ void __launch_bounds__(256) k_adjoint(DevParams<T> P, const cplx<T>* 
             }
         }
         Wout[x] = ND2 ? w : cplx<T>{sc * w.re, sc * w.im};
+    }
+}
+
+// Two atoms per block (FP32, one density): the atoms' Z and W are interleaved in LDS
+// (float4 = Z0.re, Z0.im, Z1.re, Z1.im), so one 16-byte read fetches a factor for both, and
+// each record (byte offsets of 16-byte entries, FP32 weight) is loaded and decoded once for
+// both atoms.  Same chunk walk, carry, sole stores and rotation as k_adjoint; an odd last
+// atom is paired with itself and its second copy is not written.
+__global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<float> P, const uint4* __restrict__ crec16,
+                                                     const cplx<float>* __restrict__ Zin,
+                                                     cplx<float>* __restrict__ Wout,
+                                                     const double* __restrict__ ehc,
+                                                     const double* __restrict__ rho1, double* __restrict__ eout,
+                                                     double* __restrict__ dEout, double* __restrict__ edfout,
+                                                     double* __restrict__ rho_out, int natom) {
+    using T = float;
+    extern __shared__ __align__(16) unsigned char smem[];
+    float4* sZ = (float4*)smem;                  // [nz+1], sZ[nz] = (1, 0, 1, 0)
+    float4* sW = sZ + (P.nz + 1);                // [nz+1]
+    double* red = (double*)(sW + (P.nz + 1));    // [2 * waves]: one sum per atom and wave
+    __shared__ double s_scale[2];
+    const int a0 = 2 * (int)blockIdx.x, tid = threadIdx.x;
+    const bool two = a0 + 1 < natom;
+    const cplx<T>* Z0 = Zin + (size_t)a0 * P.nz;
+    const cplx<T>* Z1 = two ? Z0 + P.nz : Z0;
+    for (int x = tid; x < P.nz; x += blockDim.x) {
+        const cplx<T> u = Z0[x], v = Z1[x];
+        sZ[x] = make_float4(u.re, u.im, v.re, v.im);
+        sW[x] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    if (tid == 0) {
+        sZ[P.nz] = make_float4(1.f, 0.f, 1.f, 0.f);
+        sW[P.nz] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    __syncthreads();
+    double rho0 = 0.0, rho1s = 0.0;
+    const int nthr = (int)blockDim.x;
+    const int nj = (P.nchunks + nthr - 1) / nthr;
+    const int rot = nj > 0 ? (int)(blockIdx.x % (unsigned)nj) : 0;
+    cplx<T> acc0 = {0, 0}, acc1 = {0, 0};
+    int j = rot;
+    int2 ch = tid + j * nthr < P.nchunks ? P.chunks[tid + j * nthr] : make_int2(0, 0);
+    const char* zb = (const char*)sZ;
+    for (int j0 = 0; j0 < nj; ++j0) {
+        const int c = tid + j * nthr;
+        const int jn = j + 1 < nj ? j + 1 : 0;
+        const int cn = tid + jn * nthr;
+        const bool has_next = j0 + 1 < nj && cn < P.nchunks;
+        const int2 chn = has_next ? P.chunks[cn] : make_int2(0, 0);
+        if (c < P.nchunks) {
+            const int x = ch.y & 0xffff, cnt = (ch.y >> 16) & 63;
+            const bool sole = (ch.y >> 22) & 1;
+            const uint4* __restrict__ rec = crec16 + ch.x;
+            for (int k = 0; k < cnt; k += 4) {
+                uint4 r[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) r[u] = rec[(size_t)(k + u) * 64];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const T w = __uint_as_float(r[u].z);
+                    const float4 za = *(const float4*)(zb + (r[u].x & 0xffff));
+                    const float4 zbv = *(const float4*)(zb + (r[u].x >> 16));
+                    const float4 zc = *(const float4*)(zb + r[u].y);
+                    const cplx<T> p0 = cmul(cmul(cplx<T>{za.x, za.y}, cplx<T>{zbv.x, zbv.y}), cplx<T>{zc.x, zc.y});
+                    const cplx<T> p1 = cmul(cmul(cplx<T>{za.z, za.w}, cplx<T>{zbv.z, zbv.w}), cplx<T>{zc.z, zc.w});
+                    acc0.re = fma(w, p0.re, acc0.re);
+                    acc0.im = fma(w, p0.im, acc0.im);
+                    acc1.re = fma(w, p1.re, acc1.re);
+                    acc1.im = fma(w, p1.im, acc1.im);
+                }
+            }
+            if (!(has_next && (chn.y & 0xffff) == x)) {
+                if (x >= P.oA0) {
+                    const float4 z = sZ[x];
+                    rho0 += (double)re_mul(cplx<T>{z.x, z.y}, acc0);
+                    rho1s += (double)re_mul(cplx<T>{z.z, z.w}, acc1);
+                }
+                if (sole) {
+                    sW[x] = make_float4(acc0.re, acc0.im, acc1.re, acc1.im);
+                } else {
+                    float* wx = (float*)&sW[x];
+                    lds_add(wx, acc0.re);
+                    lds_add(wx + 1, acc0.im);
+                    lds_add(wx + 2, acc1.re);
+                    lds_add(wx + 3, acc1.im);
+                }
+                acc0 = {0, 0};
+                acc1 = {0, 0};
+            }
+        }
+        ch = chn;
+        j = jn;
+    }
+    const double tot0 = block_sum<T>(rho0, red);
+    __syncthreads();
+    const double tot1 = block_sum<T>(rho1s, red + kAdjointThreads / 64);
+    if (tid == 0) {
+        double sc[1];
+        const double t0[1] = {tot0};
+        atom_energy<T>(P, 1, a0, t0, ehc, rho1, eout, dEout, edfout, rho_out, sc);
+        s_scale[0] = sc[0];
+        if (two) {
+            const double t1[1] = {tot1};
+            atom_energy<T>(P, 1, a0 + 1, t1, ehc, rho1, eout, dEout, edfout, rho_out, sc);
+            s_scale[1] = sc[0];
+        }
+    }
+    __syncthreads();
+    // W_eff on representatives of conjugate pairs, as in k_adjoint
+    const T sc0 = (T)s_scale[0], sc1 = two ? (T)s_scale[1] : (T)0;
+    cplx<T>* W0 = Wout + (size_t)a0 * P.nz;
+    cplx<T>* W1 = W0 + P.nz;
+    for (int x = tid; x < P.nz; x += blockDim.x) {
+        const uint32_t f = P.fold[x];
+        float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (f & (1u << 17)) {
+            w = sW[x];
+            if (!(f & (1u << 18))) {
+                const float4 b = sW[f & 0xffff];
+                const T sg = (f & (1u << 16)) ? (T)-1 : (T)1;
+                w.x += sg * b.x;
+                w.y -= sg * b.y;
+                w.z += sg * b.z;
+                w.w -= sg * b.w;
+            }
+        }
+        W0[x] = cplx<T>{sc0 * w.x, sc0 * w.y};
+        if (two) W1[x] = cplx<T>{sc1 * w.z, sc1 * w.w};
     }
 }
 
@@ -1634,7 +1772,7 @@ static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, co
     // the dynamic-LDS limit is raised once per kernel instantiation (to the workgroup
     // maximum), so that evaluations can be captured into graphs without further API calls
     static bool attributes_set = false;
-    static size_t limit[5] = {0, 0, 0, 0, 0};
+    static size_t limit[6] = {0, 0, 0, 0, 0, 0};
     if (!attributes_set) {
         int dev = 0, maxsh = 0;
         if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
@@ -1650,9 +1788,21 @@ static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, co
                 hipSuccess)
                 return e;
         }
+        if constexpr (sizeof(T) == 4) {
+            hipFuncAttributes fa;
+            if ((e = hipFuncGetAttributes(&fa, (const void*)k_adjoint_pair)) != hipSuccess) return e;
+            limit[5] = (size_t)maxsh > fa.sharedSizeBytes ? (size_t)maxsh - fa.sharedSizeBytes : 0;
+            if ((e = hipFuncSetAttribute((const void*)k_adjoint_pair, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)limit[5])) != hipSuccess)
+                return e;
+        }
         attributes_set = true;
     }
-    const size_t lds2 = (nd2 ? 3 : 2) * (size_t)(P.nz + 1) * sizeof(cplx<T>) + 8 * sizeof(double);
+    // FP32 with one density: two atoms per adjoint block (their Z and W interleaved in LDS)
+    const bool pair = sizeof(T) == 4 && !nd2 && P.crec16 != nullptr;
+    const size_t lds2p = 2 * (size_t)(P.nz + 1) * 16 + 2 * (kAdjointThreads / 64) * sizeof(double);
+    if (pair && lds2p > limit[5]) return hipErrorInvalidValue;
+    const size_t lds2 = (nd2 ? 3 : 2) * (size_t)(P.nz + 1) * sizeof(cplx<T>) + 2 * (kAdjointThreads / 64) * sizeof(double);
     const size_t lds3 = (size_t)P.nz * sizeof(cplx<T>);
     if (lds1 > limit[0] || lds2 > limit[nd2 ? 3 : 1] || lds3 > limit[q1 ? 2 : 4]) return hipErrorInvalidValue;
     if (ev) (void)hipEventRecord(ev[0], s);
@@ -1660,11 +1810,18 @@ static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, co
                        emom, (cplx<T>*)B.Z, B.ehc, B.rho1, B.err);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[1], s);
-    if (nd2)
-        hipLaunchKernelGGL((k_adjoint<T, true>), dim3(natom), dim3(256), lds2, s, P, (const cplx<T>*)B.Z,
+    if constexpr (sizeof(T) == 4) {
+        if (pair)
+            hipLaunchKernelGGL(k_adjoint_pair, dim3((natom + 1) / 2), dim3(kAdjointThreads), lds2p, s, P, P.crec16,
+                               (const cplx<T>*)B.Z, (cplx<T>*)B.W, B.ehc, B.rho1, eout, B.dE, B.edf, B.rho,
+                               natom);
+    }
+    if (pair) {
+    } else if (nd2)
+        hipLaunchKernelGGL((k_adjoint<T, true>), dim3(natom), dim3(kAdjointThreads), lds2, s, P, (const cplx<T>*)B.Z,
                            (cplx<T>*)B.W, B.ehc, B.rho1, eout, B.dE, B.edf, B.rho);
     else
-        hipLaunchKernelGGL((k_adjoint<T, false>), dim3(natom), dim3(256), lds2, s, P, (const cplx<T>*)B.Z,
+        hipLaunchKernelGGL((k_adjoint<T, false>), dim3(natom), dim3(kAdjointThreads), lds2, s, P, (const cplx<T>*)B.Z,
                            (cplx<T>*)B.W, B.ehc, B.rho1, eout, B.dE, B.edf, B.rho);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[2], s);

@@ -12,6 +12,10 @@ struct cplx {
     T re, im;
 };
 
+// threads of an adjoint workgroup; chunk c of the gradient records runs on thread
+// c % kAdjointThreads, and the chunk order built with the records assumes this count
+constexpr int kAdjointThreads = 512;
+
 // Everything a kernel needs about the potential, passed by value as a kernel argument.
 template <typename T>
 struct DevParams {
@@ -26,7 +30,10 @@ struct DevParams {
     // of the factors in the adjoint's LDS copy of Z (index * sizeof(cplx<T>)), (z, w) = weight
     // (FP64 bits as lo/hi words, or FP32 bits in z)
     const uint4* crec = nullptr;
-    const int2* chunks = nullptr;    // (start, target | count << 16 | sole << 22), count = its group's length
+    const uint4* crec16 = nullptr;   // FP32 only: the records with offsets of 16-byte entries (two
+                                     // atoms' Z interleaved, k_adjoint_pair)
+    const int2* chunks = nullptr;    // (start, target | count << 16 | sole << 22), count = its group's length;
+                                     // chunk c runs on thread c % kAdjointThreads
     const uint32_t* fold = nullptr;  // conjugate-pair folding of W (see TermList::fold)
     int nchunks = 0;
     int nfemb = 0;

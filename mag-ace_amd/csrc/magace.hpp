@@ -69,6 +69,7 @@ struct ZLayout {
 // every monomial has exactly one A0 factor.
 struct TermList {
     static constexpr int kChunk = 32;
+    static constexpr int kLanes = 512;  // threads per adjoint workgroup the chunk order is built for
     std::vector<uint16_t> idx;   // 4 per term (slot 0 = A0 factor)
     std::vector<double> w;       // merged weights, density-major: w[d * nterms + t]
     std::vector<uint16_t> cidx;  // 4 per contribution: the three other factors + padding
