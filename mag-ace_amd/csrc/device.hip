@@ -337,18 +337,32 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                         sCr[j] = 0;
                     }
                 } else if (role <= nsplit) {
+                    // a share of the pair's radial functions g_n, R_{l,n} and, after them, of
+                    // its magnetic functions M_{l',n'}
                     if (live) {
-                        T w;
-                        const T* row = atomic_row(P, dlist[p], w, err);
-                        const int nf = P.nb_a + nfa, per = (nf + nsplit - 1) / nsplit;
-                        const int k0 = (role - 1) * per, k1 = min(nf, k0 + per);
-                        for (int k = k0; k < k1; ++k) {
-                            T v = 0, dv;
-                            if (row) horner(row + 4 * k, w, (T)P.rs_a, v, dv);
-                            if (k < P.nb_a)
-                                sG[j * P.nb_a + k] = v;
-                            else
-                                sR[j * nfa + (k - P.nb_a)] = v;
+                        const int nf = P.nb_a + nfa, per = (nf + nfm + nsplit - 1) / nsplit;
+                        const int k0 = (role - 1) * per, k1 = min(nf + nfm, k0 + per);
+                        if (k0 < nf) {
+                            T w;
+                            const T* row = atomic_row(P, dlist[p], w, err);
+                            for (int k = k0; k < min(k1, nf); ++k) {
+                                T v = 0, dv;
+                                if (row) horner(row + 4 * k, w, (T)P.rs_a, v, dv);
+                                if (k < P.nb_a)
+                                    sG[j * P.nb_a + k] = v;
+                                else
+                                    sR[j * nfa + (k - P.nb_a)] = v;
+                            }
+                        }
+                        if (k1 > nf) {
+                            T w;
+                            bool unit;
+                            const T* row = magnetic_row(P, dmom[p], w, unit, err);
+                            for (int k = max(k0, nf); k < k1; ++k) {
+                                T v = unit ? (T)1 : (T)0, dv;
+                                if (row) horner(row + 4 * (P.nb_m + k - nf), w, (T)P.rs_m, v, dv);
+                                sM[j * nfm + (k - nf)] = v;
+                            }
                         }
                     }
                 } else if (role == nsplit + 1) {  // magnetic side of pair c0+j
@@ -356,23 +370,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                         T w;
                         bool unit;
                         const T* row = magnetic_row(P, dmom[p], w, unit, err);
-                        for (int k = 0; k < nfm; ++k) {
-                            T v = unit ? (T)1 : (T)0, dv;
-                            if (row) horner(row + 4 * (P.nb_m + k), w, (T)P.rs_m, v, dv);
-                            sM[j * nfm + k] = v;
-                        }
                         cplx<T>* yh = sYm + j * P.lhm;
                         ylm_half<T, L>((T)emom[3 * p], (T)emom[3 * p + 1], (T)emom[3 * p + 2], P.l2m, yh);
-                        // H_q = M_{l',n'} Y_{l'm'} for the magnetic columns of A (l' <= l3m)
-                        for (int q = 0; q < P.nm3; ++q) {
-                            const int np = q / P.lsm3, lmp = q % P.lsm3;
-                            int lp = 0;
-                            while ((lp + 1) * (lp + 1) <= lmp) ++lp;
-                            const int mp = lmp - lp * lp - lp;
-                            const cplx<T> y = yfull(yh, lp, mp);
-                            const T mv = sM[j * nfm + lp * P.nr2m + np];
-                            sH[j * P.nm3 + q] = {mv * y.re, mv * y.im};
-                        }
+                        // H_q = M_{l',n'} Y_{l'm'} for the magnetic columns of A (l' <= l3m,
+                        // q = n' lsm3 + l'^2 + l' + m'); the few M it needs are evaluated here
+                        // rather than waiting for the spline roles
+                        for (int np = 0; np * P.lsm3 < P.nm3; ++np)
+                            for (int lp = 0; lp <= P.l3m; ++lp) {
+                                T mv = unit ? (T)1 : (T)0, dv;
+                                if (row) horner(row + 4 * (P.nb_m + lp * P.nr2m + np), w, (T)P.rs_m, mv, dv);
+                                for (int mp = -lp; mp <= lp; ++mp) {
+                                    const int q = np * P.lsm3 + lp * lp + lp + mp;
+                                    const cplx<T> y = yfull(yh, lp, mp);
+                                    sH[j * P.nm3 + q] = {mv * y.re, mv * y.im};
+                                }
+                            }
                     }
                 }
             } else {
