@@ -312,6 +312,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
         if (isA2) { lm = (task - a2base) % P.lsm; n0 = ((task - a2base) / P.lsm) * NB2; }
         while ((l + 1) * (l + 1) <= lm) ++l;
         m = lm - l * l - l;
+        // Y_{l,-|m|} = (-1)^|m| conj(Y_{l,|m|}): the loops read the stored m >= 0 entry and
+        // flip the sign of its imaginary part; (-1)^|m| is applied once, at the store
+        const int lh = l * (l + 1) / 2 + (m < 0 ? -m : m);
+        const T cim = m < 0 ? (T)-1 : (T)1;
+        const T cst = (m < 0 && ((-m) & 1)) ? (T)-1 : (T)1;
 
         for (int c0 = 0; c0 < nn; c0 += CH) {
             const int nj = nn - c0 < CH ? nn - c0 : CH;
@@ -448,35 +453,44 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                 *sEhc += s;
             }
             if (isA) {
-                // A[n][lm][q] += R_{l,n} Y_lm H_q   (src/a_basis.F90:248-291)
+                // A[n][lm][q] += R_{l,n} Y_lm H_q   (src/a_basis.F90:248-291).  Columns past nm3
+                // and radial indices past nr3a are read without a test: those reads stay inside
+                // the staging area, and their sums land in slots that are never stored.
                 for (int j = part; j < nj; j += sa) {
-                    const cplx<T> y = yfull(sY + j * P.lha, l, m);
+                    const cplx<T> yv = sY[j * P.lha + lh];
+                    const cplx<T> y = {yv.re, cim * yv.im};
                     cplx<T> h[NM3X];
 #pragma unroll
-                    for (int q = 0; q < NM3X; ++q) h[q] = q < P.nm3 ? sH[j * P.nm3 + q] : cplx<T>{0, 0};
+                    for (int q = 0; q < NM3X; ++q) h[q] = sH[j * P.nm3 + q];
+                    const T* rr = sR + j * nfa + l * P.nr3a + n0;
+                    // Y H_q once per pair and column, then two real FMAs per radial index
+                    cplx<T> yh[NM3X];
+#pragma unroll
+                    for (int q = 0; q < NM3X; ++q) yh[q] = cmul(y, h[q]);
 #pragma unroll
                     for (int b = 0; b < NB; ++b) {
-                        const int n = n0 + b;
-                        const T r = n < P.nr3a ? sR[j * nfa + l * P.nr3a + n] : (T)0;
-                        const cplx<T> ry = {r * y.re, r * y.im};
+                        const T r = rr[b];
 #pragma unroll
-                        for (int q = 0; q < NM3X; ++q) cfma(acc[b][q], ry, h[q]);
+                        for (int q = 0; q < NM3X; ++q) {
+                            acc[b][q].re = fma(r, yh[q].re, acc[b][q].re);
+                            acc[b][q].im = fma(r, yh[q].im, acc[b][q].im);
+                        }
                     }
                 }
             } else if (isA2) {
-                // A2[n][n'][l'm'] += g_n M_{l',n'} Y_{l'm'}   (src/a_basis.F90:193-201)
+                // A2[n][n'][l'm'] += g_n M_{l',n'} Y_{l'm'}   (src/a_basis.F90:193-201); unused
+                // slots as above
                 for (int j = 0; j < nj; ++j) {
-                    const cplx<T> y = yfull(sYm + j * P.lhm, l, m);
+                    const cplx<T> yv = sYm[j * P.lhm + lh];
+                    const cplx<T> y = {yv.re, cim * yv.im};
+                    const T* mr = sM + j * nfm + l * P.nr2m;
                     cplx<T> my[NR2X];
 #pragma unroll
-                    for (int q = 0; q < NR2X; ++q) {
-                        const T mv = q < P.nr2m ? sM[j * nfm + l * P.nr2m + q] : (T)0;
-                        my[q] = {mv * y.re, mv * y.im};
-                    }
+                    for (int q = 0; q < NR2X; ++q) my[q] = {mr[q] * y.re, mr[q] * y.im};
+                    const T* gr = sG + j * P.nb_a + n0;
 #pragma unroll
                     for (int b = 0; b < NB2; ++b) {
-                        const int n = n0 + b;
-                        const T g = n < P.nb_a ? sG[j * P.nb_a + n] : (T)0;
+                        const T g = gr[b];
 #pragma unroll
                         for (int q = 0; q < NR2X; ++q) {
                             acc[b][q].re = fma(g, my[q].re, acc[b][q].re);
@@ -513,7 +527,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                 if (n < P.nr3a)
 #pragma unroll
                     for (int q = 0; q < NM3X; ++q)
-                        if (q < P.nm3) Zout[(n * P.lsa + lm) * P.nm3 + q] = acc[b][q];
+                        if (q < P.nm3) Zout[(n * P.lsa + lm) * P.nm3 + q] = {cst * acc[b][q].re, cst * acc[b][q].im};
             }
         } else if (isA2) {
 #pragma unroll
@@ -522,7 +536,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                 if (n < P.nb_a)
 #pragma unroll
                     for (int q = 0; q < NR2X; ++q)
-                        if (q < P.nr2m) Zout[P.oA2 + (n * P.nr2m + q) * P.lsm + lm] = acc[b][q];
+                        if (q < P.nr2m) Zout[P.oA2 + (n * P.nr2m + q) * P.lsm + lm] = {cst * acc[b][q].re, cst * acc[b][q].im};
             }
         }
     }
@@ -1362,6 +1376,10 @@ __global__ void __launch_bounds__(256) k_density_jvp(DevParams<double> P, const 
         if (isA2) { lm = (task - a2base) % P.lsm; n0 = ((task - a2base) / P.lsm) * NB2; }
         while ((l + 1) * (l + 1) <= lm) ++l;
         m = lm - l * l - l;
+        // m < 0 from the stored m >= 0 entries, as in k_density
+        const int lh = l * (l + 1) / 2 + (m < 0 ? -m : m);
+        const T cim = m < 0 ? (T)-1 : (T)1;
+        const T cst = (m < 0 && ((-m) & 1)) ? (T)-1 : (T)1;
         for (int c0 = 0; c0 < nn; c0 += CH) {
             const int nj = nn - c0 < CH ? nn - c0 : CH;
             __syncthreads();
@@ -1433,16 +1451,18 @@ __global__ void __launch_bounds__(256) k_density_jvp(DevParams<double> P, const 
                 *sE += sum;
             }
             if (isA) {
+                // unused columns and radial indices are read without a test, as in k_density
                 for (int j = 0; j < nj; ++j) {
-                    const cplx<T> y = yfull(sY + j * P.lha, l, m), yd = yfull(sYd + j * P.lha, l, m);
+                    const cplx<T> yv = sY[j * P.lha + lh], ydv = sYd[j * P.lha + lh];
+                    const cplx<T> y = {yv.re, cim * yv.im}, yd = {ydv.re, cim * ydv.im};
                     cplx<T> h[NM3X];
 #pragma unroll
-                    for (int q = 0; q < NM3X; ++q) h[q] = q < P.nm3 ? sH[j * P.nm3 + q] : cplx<T>{0, 0};
+                    for (int q = 0; q < NM3X; ++q) h[q] = sH[j * P.nm3 + q];
+                    const T* rr = sR + j * nfa + l * P.nr3a + n0;
+                    const T* rdr = sRd + j * nfa + l * P.nr3a + n0;
 #pragma unroll
                     for (int b = 0; b < NB; ++b) {
-                        const int n = n0 + b;
-                        const T r = n < P.nr3a ? sR[j * nfa + l * P.nr3a + n] : (T)0;
-                        const T rd = n < P.nr3a ? sRd[j * nfa + l * P.nr3a + n] : (T)0;
+                        const T r = rr[b], rd = rdr[b];
                         const cplx<T> ry = {rd * y.re + r * yd.re, rd * y.im + r * yd.im};
 #pragma unroll
                         for (int q = 0; q < NM3X; ++q) cfma(acc[b][q], ry, h[q]);
@@ -1450,17 +1470,16 @@ __global__ void __launch_bounds__(256) k_density_jvp(DevParams<double> P, const 
                 }
             } else if (isA2) {
                 for (int j = 0; j < nj; ++j) {
-                    const cplx<T> y = yfull(sYm + j * P.lhm, l, m);
+                    const cplx<T> yv = sYm[j * P.lhm + lh];
+                    const cplx<T> y = {yv.re, cim * yv.im};
+                    const T* mr = sM + j * nfm + l * P.nr2m;
                     cplx<T> my[NR2X];
 #pragma unroll
-                    for (int q = 0; q < NR2X; ++q) {
-                        const T mv = q < P.nr2m ? sM[j * nfm + l * P.nr2m + q] : (T)0;
-                        my[q] = {mv * y.re, mv * y.im};
-                    }
+                    for (int q = 0; q < NR2X; ++q) my[q] = {mr[q] * y.re, mr[q] * y.im};
+                    const T* gr = sGd + j * P.nb_a + n0;
 #pragma unroll
                     for (int b = 0; b < NB2; ++b) {
-                        const int n = n0 + b;
-                        const T gd = n < P.nb_a ? sGd[j * P.nb_a + n] : (T)0;
+                        const T gd = gr[b];
 #pragma unroll
                         for (int q = 0; q < NR2X; ++q) {
                             acc[b][q].re = fma(gd, my[q].re, acc[b][q].re);
@@ -1477,7 +1496,7 @@ __global__ void __launch_bounds__(256) k_density_jvp(DevParams<double> P, const 
                 if (n < P.nr3a)
 #pragma unroll
                     for (int q = 0; q < NM3X; ++q)
-                        if (q < P.nm3) Zdot[(n * P.lsa + lm) * P.nm3 + q] = acc[b][q];
+                        if (q < P.nm3) Zdot[(n * P.lsa + lm) * P.nm3 + q] = {cst * acc[b][q].re, cst * acc[b][q].im};
             }
         } else if (isA2) {
 #pragma unroll
@@ -1486,7 +1505,7 @@ __global__ void __launch_bounds__(256) k_density_jvp(DevParams<double> P, const 
                 if (n < P.nb_a)
 #pragma unroll
                     for (int q = 0; q < NR2X; ++q)
-                        if (q < P.nr2m) Zdot[P.oA2 + (n * P.nr2m + q) * P.lsm + lm] = acc[b][q];
+                        if (q < P.nr2m) Zdot[P.oA2 + (n * P.nr2m + q) * P.lsm + lm] = {cst * acc[b][q].re, cst * acc[b][q].im};
             }
         }
     }
