@@ -1105,8 +1105,15 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                             for (int q = 0; q < NTQ; ++q) Tq[q] = {0, 0};
 #pragma unroll
                             for (int n = 0; n < NRAX; ++n) {
-                                if (n >= P.nr3a) break;
-                                const cplx<T>* Wn = W + (n * P.lsa + lm) * P.nm3;
+                                // Q1 runs to the compile-time bound (past nr3a, R = R' = 0 and
+                                // row 0 is read), so that each (l, m) body is one straight block
+                                // the scheduler can interleave (lmax-6 FP32 0.963 -> 0.919 ms);
+                                // the general instance keeps the exit, as without it it spills
+                                if constexpr (!Q1) {
+                                    if (n >= P.nr3a) break;
+                                }
+                                const int nw = Q1 && n >= P.nr3a ? 0 : n;
+                                const cplx<T>* Wn = W + (nw * P.lsa + lm) * P.nm3;
                                 // the row's reads are issued together, outside the uniform
                                 // branches, so their LDS latency overlaps (a read inside each
                                 // branch waited for its own result)
@@ -1796,26 +1803,32 @@ static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, co
     // the Q1 instantiation exists only for the four-column class
     constexpr bool kQ1 = NM3X == 4;
     const bool q1 = kQ1 && P.nm3 == 4 && P.lsm3 == 4;
+    // Q1 visits every radial index up to its bound: a bound of 4 for nr3a <= 4
+    constexpr int kNR4 = NRAX < 4 ? NRAX : 4;
+    const bool r4 = P.nr3a <= kNR4;
     auto k3q = k_forces<T, L, L, NM3X, NBAX, NRAX, NR2X, kQ1>;
+    auto k3q4 = k_forces<T, L, L, NM3X, NBAX, kNR4, NR2X, kQ1>;
     auto k3g = k_forces<T, L, L, NM3X, NBAX, NRAX, NR2X, false>;
-    auto k3 = q1 ? k3q : k3g;
+    auto k3 = q1 ? (r4 ? k3q4 : k3q) : k3g;
     hipError_t e = hipSuccess;
     // the dynamic-LDS limit is raised once per kernel instantiation (to the workgroup
     // maximum), so that evaluations can be captured into graphs without further API calls
     static bool attributes_set = false;
-    static size_t limit[6] = {0, 0, 0, 0, 0, 0};
+    static size_t limit[7] = {0, 0, 0, 0, 0, 0, 0};
     if (!attributes_set) {
         int dev = 0, maxsh = 0;
         if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
         if ((e = hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev)) != hipSuccess)
             return e;
-        const void* ks[5] = {(const void*)k1, (const void*)k_adjoint<T, false>, (const void*)k3q,
-                             (const void*)k_adjoint<T, true>, (const void*)k3g};
-        for (int i = 0; i < 5; ++i) {
+        const void* ks[6] = {(const void*)k1, (const void*)k_adjoint<T, false>, (const void*)k3q,
+                             (const void*)k_adjoint<T, true>, (const void*)k3g, (const void*)k3q4};
+        for (int i = 0; i < 6; ++i) {
+            // limit[5] belongs to k_adjoint_pair below
+            const int li = i < 5 ? i : 6;
             hipFuncAttributes fa;
             if ((e = hipFuncGetAttributes(&fa, ks[i])) != hipSuccess) return e;
-            limit[i] = (size_t)maxsh > fa.sharedSizeBytes ? (size_t)maxsh - fa.sharedSizeBytes : 0;
-            if ((e = hipFuncSetAttribute(ks[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)limit[i])) !=
+            limit[li] = (size_t)maxsh > fa.sharedSizeBytes ? (size_t)maxsh - fa.sharedSizeBytes : 0;
+            if ((e = hipFuncSetAttribute(ks[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)limit[li])) !=
                 hipSuccess)
                 return e;
         }
@@ -1835,7 +1848,7 @@ static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, co
     if (pair && lds2p > limit[5]) return hipErrorInvalidValue;
     const size_t lds2 = (nd2 ? 3 : 2) * (size_t)(P.nz + 1) * sizeof(cplx<T>) + 2 * (kAdjointThreads / 64) * sizeof(double);
     const size_t lds3 = (size_t)P.nz * sizeof(cplx<T>);
-    if (lds1 > limit[0] || lds2 > limit[nd2 ? 3 : 1] || lds3 > limit[q1 ? 2 : 4]) return hipErrorInvalidValue;
+    if (lds1 > limit[0] || lds2 > limit[nd2 ? 3 : 1] || lds3 > limit[q1 ? (r4 ? 6 : 2) : 4]) return hipErrorInvalidValue;
     if (ev) (void)hipEventRecord(ev[0], s);
     hipLaunchKernelGGL(k1, dim3(natom), dim3(256), lds1, s, P, off, dlist, elist, dmomi, emomi, dmom,
                        emom, (cplx<T>*)B.Z, B.ehc, B.rho1, B.err);
