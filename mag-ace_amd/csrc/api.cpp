@@ -99,6 +99,8 @@ struct Context {
     };
     std::array<Slot, 2> slots;
     hipStream_t s_in = nullptr, s_out = nullptr;
+    // fork and join of an evaluation run as two halves (see run_device)
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool uploaded = false;
     // fitting: adjoint buffers, pair outputs of the fit forward, device coefficient structure
     FitBuffers fit;
@@ -150,6 +152,8 @@ Context::~Context() {
     }
     if (s_in) (void)hipStreamDestroy(s_in);
     if (s_out) (void)hipStreamDestroy(s_out);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
     for (auto& e : events)
         for (auto& x : e) (void)hipEventDestroy(x);
     if (stream) (void)hipStreamDestroy(stream);
@@ -308,6 +312,8 @@ int upload(Context& c) {
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(upload_legendre_constants());
     HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
     if (upload_table(c.t64.lut_a, M.spl.atomic)) return MAGACE_ERR_DEVICE;
     if (upload_table(c.t64.lut_m, M.spl.magnetic)) return MAGACE_ERR_DEVICE;
     if (upload_table(c.t32.lut_a, M.spl.atomic)) return MAGACE_ERR_DEVICE;
@@ -354,10 +360,53 @@ int ensure_atoms(Context& c, size_t natom) {
     return 0;
 }
 
+// Evaluations of at least kSplitAtoms atoms run as two halves of the atom range, the second on
+// the context's own stream (used otherwise only by acelib_), forked from the caller's stream
+// and joined back to it; no further stream is created, since the process's streams share
+// four hardware queues and a fifth one slowed acelib_'s copy pipeline.  Every
+// kernel is independent per atom, so the halves only overlap: the tail of one half's launch
+// and the next kernel of the other share the CUs, and an LDS-bound adjoint runs beside the
+// VALU-bound forces.  The second half uses its own rows of the per-atom device buffers; pair
+// arrays are indexed by absolute pair index.  Not while per-kernel times are measured, nor
+// while the caller's stream is being captured into a graph, nor inside acelib_'s chunk
+// pipeline, whose kernels run on that stream.  MAGACE_SPLIT_STREAMS=0 turns the split off
+// (read at every call).
+constexpr int kSplitAtoms = 2048;
+
+bool split_enabled() {
+    const char* e = std::getenv("MAGACE_SPLIT_STREAMS");
+    return !(e && e[0] == '0' && e[1] == 0);
+}
+
+// the device buffers of atoms [a, ...)
+Buffers buffers_from(const Buffers& B, size_t a, size_t zbytes, int nd) {
+    Buffers o = B;
+    o.Z = (char*)B.Z + a * zbytes;
+    o.W = (char*)B.W + a * zbytes;
+    o.ehc = B.ehc + a;
+    o.rho1 = B.rho1 + a * nd;
+    o.dE = B.dE + a * nd;
+    o.edf = B.edf + a;
+    o.rho = B.rho + a * nd;
+    o.cap_atoms = B.cap_atoms - a;
+    return o;
+}
+
+hipError_t launch(Context& c, int natom, const int* off, const double* dlist, const double* elist,
+                  const double* dmomi, const double* emomi, const double* dmom, const double* emom,
+                  double* eout, double* flist, double* tlist, double* tlist0, Buffers& B, hipStream_t s,
+                  hipEvent_t* ev) {
+    if (c.precision == MAGACE_FP64)
+        return Launch<double>::run(c.P64, natom, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist, tlist,
+                                   tlist0, B, s, c.shape_class, ev);
+    return Launch<float>::run(c.P32, natom, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist, tlist,
+                              tlist0, B, s, c.shape_class, ev);
+}
+
 int run_device(Context& c, int natom, const int* off, const double* dlist, const double* elist,
                const double* dmomi, const double* emomi, const double* dmom, const double* emom,
                double* eout, double* flist, double* tlist, double* tlist0, hipStream_t s,
-               bool clear_err = true) {
+               bool clear_err = true, bool allow_split = true) {
     if (int rc = upload(c)) return rc;
     if (int rc = ensure_atoms(c, (size_t)natom)) return rc;
     if (clear_err) HIPCHK(clear_flags(c.buf.err, s));
@@ -370,14 +419,27 @@ int run_device(Context& c, int natom, const int* off, const double* dlist, const
         }
         ev = c.events[c.events_used++].data();
     }
-    hipError_t e;
-    if (c.precision == MAGACE_FP64)
-        e = Launch<double>::run(c.P64, natom, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist, tlist,
-                                tlist0, c.buf, s, c.shape_class, ev);
-    else
-        e = Launch<float>::run(c.P32, natom, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist, tlist,
-                               tlist0, c.buf, s, c.shape_class, ev);
-    HIPCHK(e);
+    bool split = allow_split && s != c.stream && !c.profiling && natom >= kSplitAtoms && split_enabled();
+    if (split) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        HIPCHK(hipStreamIsCapturing(s, &cs));
+        split = cs == hipStreamCaptureStatusNone;
+    }
+    if (!split) {
+        HIPCHK(launch(c, natom, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist, tlist, tlist0, c.buf, s,
+                      ev));
+        return 0;
+    }
+    const int h = natom / 2;
+    const size_t zb = (size_t)c.model.z.nz * (c.precision == MAGACE_FP64 ? 16 : 8);
+    Buffers B1 = buffers_from(c.buf, (size_t)h, zb, c.model.pot.h.ndensity);
+    HIPCHK(hipEventRecord(c.ev_fork, s));
+    HIPCHK(hipStreamWaitEvent(c.stream, c.ev_fork, 0));
+    HIPCHK(launch(c, h, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist, tlist, tlist0, c.buf, s, nullptr));
+    HIPCHK(launch(c, natom - h, off + h, dlist, elist, dmomi + h, emomi + 3 * (size_t)h, dmom, emom, eout + h, flist,
+                  tlist, tlist0 + 3 * (size_t)h, B1, c.stream, nullptr));
+    HIPCHK(hipEventRecord(c.ev_join, c.stream));
+    HIPCHK(hipStreamWaitEvent(s, c.ev_join, 0));
     return 0;
 }
 
@@ -698,7 +760,7 @@ void acelib_(int* natom, int* nall, int* nei, int* nlist, int* occ, int* occlist
         if (run_device(c, (int)L.na, (const int*)(d + L.i_off), (const double*)(d + L.i_dl),
                        (const double*)(d + L.i_el), (const double*)(d + L.i_dmi), (const double*)(d + L.i_emi),
                        (const double*)(d + L.i_dm), (const double*)(d + L.i_em), (double*)(o + L.o_eout),
-                       (double*)(o + L.o_fl), (double*)(o + L.o_tl), (double*)(o + L.o_t0), s, false))
+                       (double*)(o + L.o_fl), (double*)(o + L.o_tl), (double*)(o + L.o_t0), s, false, false))
             return;
         if (fail(hipEventRecord(sl.ev_comp, s), "event") || fail(hipStreamWaitEvent(c.s_out, sl.ev_comp, 0), "event") ||
             fail(hipMemcpyAsync(sl.h_out, sl.d_out, L.out_bytes, hipMemcpyDeviceToHost, c.s_out), "copy") ||

@@ -156,3 +156,39 @@ def test_second_species_rejected_then_recovers(lists):
     got = capi.acelib(a)
     want = device_entry(a)
     assert relerr(got["flist"], want["flist"]) < 1e-12
+
+
+def test_split_halves_equal_one_sequence(tmp_path):
+    """From 2048 atoms an evaluation runs as two halves of the atom range on two streams
+    (MAGACE_SPLIT_STREAMS=0 keeps one sequence).  Only the order of the adjoint's LDS atomics
+    may change: FP64 to 1e-12, mixed to 1e-6, and acelib_ on host arrays agrees as well."""
+    pot = str(tmp_path / "Fe.pbe.in")
+    h = potgen.H500
+    with open(pot, "w") as f:
+        f.write(potgen.render(h, capi.enumerate_basis(h), 2032))
+    s = systems.make_system(12, 2032, spins="uniform")  # 3456 atoms
+    a = nbr.neilist(s["pos"], s["sdir"] * s["smag"][:, None], s["cell"], 5.0)
+    assert len(a["nstart"]) >= 2048
+    capi.init_file(pot)
+    old = os.environ.get("MAGACE_SPLIT_STREAMS")
+    try:
+        for prec, tol in ((capi.FP64, 1e-12), (capi.MIXED, 1e-6)):
+            capi.set_precision(prec)
+            say("split halves, precision", prec, len(a["nstart"]), "atoms")
+            os.environ["MAGACE_SPLIT_STREAMS"] = "0"
+            want = device_entry(a)
+            os.environ["MAGACE_SPLIT_STREAMS"] = "1"
+            got = device_entry(a)
+            again = device_entry(a)
+            host = capi.acelib(a)
+            for k in KEYS:
+                assert relerr(got[k], want[k]) < tol, (prec, k)
+                assert relerr(again[k], want[k]) < tol, (prec, k)
+                assert relerr(host[k], want[k]) < tol, (prec, k)
+    finally:
+        capi.set_precision(capi.FP64)
+        if old is None:
+            os.environ.pop("MAGACE_SPLIT_STREAMS", None)
+        else:
+            os.environ["MAGACE_SPLIT_STREAMS"] = old
+
