@@ -64,7 +64,10 @@ int magace_get_precision(void);
 /* acelib_ on device-resident arrays (same meaning as acelib_, FP64), with 0-based CSR
  * offsets[natom+1] instead of nstart/nstop.  Enqueued on `stream` (a hipStream_t; NULL is the
  * default stream, as in HIP); no host synchronisation.  magace_check_errors() synchronises the
- * stream and reports geometry errors raised by the kernels. */
+ * stream and reports geometry errors raised by the kernels.  From 2048 atoms the second half of
+ * the atoms runs on the library's own stream, forked from `stream` and joined back to it before
+ * the call's work on `stream` completes (not while `stream` is being captured into a graph;
+ * MAGACE_SPLIT_STREAMS=0 turns this off).  Calls on different streams must not overlap. */
 int magace_compute_device(int natom, int nei, const int *offsets, const double *dlist,
                           const double *elist, const double *dlist_momi, const double *elist_momi,
                           const double *dlist_mom, const double *elist_mom, double *eout,
