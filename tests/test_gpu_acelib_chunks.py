@@ -3,7 +3,9 @@ through pinned buffers so that copies overlap the kernels (MAGACE_ACELIB_CHUNK_P
 chunk size and is read at every call).  Chunking changes nothing but summation order inside
 the adjoint's LDS atomics: every chunk size must agree with the device-pointer entry point on
 the whole list to 1e-12 (FP64), including lists with atoms that have no neighbours and a
-list with no pairs at all.
+list with no pairs at all.  The device entry point's own partitions are checked the same way:
+evaluations of 2048+ atoms run as two halves on two streams, and the FP32 adjoint pairs atoms
+two per workgroup (odd counts included).
 """
 import os
 import sys
@@ -185,6 +187,76 @@ def test_split_halves_equal_one_sequence(tmp_path):
                 assert relerr(got[k], want[k]) < tol, (prec, k)
                 assert relerr(again[k], want[k]) < tol, (prec, k)
                 assert relerr(host[k], want[k]) < tol, (prec, k)
+    finally:
+        capi.set_precision(capi.FP64)
+        if old is None:
+            os.environ.pop("MAGACE_SPLIT_STREAMS", None)
+        else:
+            os.environ["MAGACE_SPLIT_STREAMS"] = old
+
+
+def first_atoms(a, n):
+    """The list of the first n atoms (their pairs only); each atom's results do not depend on
+    the others."""
+    b = dict(a)
+    npair = int(a["nstop"][n - 1])
+    b["nstart"], b["nstop"] = a["nstart"][:n].copy(), a["nstop"][:n].copy()
+    for k in ("dlist", "dlist_mom", "nlist", "occlist"):
+        b[k] = np.ascontiguousarray(a[k][:npair])
+    for k in ("elist", "elist_mom"):
+        b[k] = np.ascontiguousarray(a[k][:3 * npair])
+    b["dlist_momi"] = np.ascontiguousarray(a["dlist_momi"][:n])
+    b["elist_momi"] = np.ascontiguousarray(a["elist_momi"][:3 * n])
+    return b
+
+
+def compare_prefix(got, full, n, npair, tol, what):
+    assert relerr(got["eout"], full["eout"][:n]) < tol, (what, "eout")
+    assert relerr(got["tlist0"], full["tlist0"][:3 * n]) < tol, (what, "tlist0")
+    assert relerr(got["flist"], full["flist"][:3 * npair]) < tol, (what, "flist")
+    assert relerr(got["tlist"], full["tlist"][:3 * npair]) < tol, (what, "tlist")
+
+
+def test_odd_atom_counts(lists):
+    """The FP32 adjoint takes two atoms per workgroup; with an odd count the last atom is paired
+    with itself and only its first copy is written.  1023 and 1 atoms against the same atoms in
+    the 1024-atom evaluation (mixed to 1e-6; FP64 to 1e-12)."""
+    pot, a = lists
+    assert "dlist_momi" in a and len(a["dlist_momi"]) == len(a["nstart"])
+    capi.init_file(pot)
+    try:
+        for prec, tol in ((capi.MIXED, 1e-6), (capi.FP64, 1e-12)):
+            capi.set_precision(prec)
+            full = device_entry(a)
+            for n in (1023, 1):
+                say("odd atom count", n, "precision", prec)
+                b = first_atoms(a, n)
+                compare_prefix(device_entry(b), full, n, len(b["dlist"]), tol, (prec, n))
+    finally:
+        capi.set_precision(capi.FP64)
+
+
+def test_split_with_odd_half(tmp_path):
+    """3455 atoms: halves of 1727 and 1728 atoms, the first odd, against the same atoms in one
+    sequence."""
+    pot = str(tmp_path / "Fe.pbe.in")
+    h = potgen.H500
+    with open(pot, "w") as f:
+        f.write(potgen.render(h, capi.enumerate_basis(h), 2033))
+    s = systems.make_system(12, 2033, spins="uniform")
+    a = nbr.neilist(s["pos"], s["sdir"] * s["smag"][:, None], s["cell"], 5.0)
+    n = len(a["nstart"]) - 1
+    b = first_atoms(a, n)
+    capi.init_file(pot)
+    old = os.environ.get("MAGACE_SPLIT_STREAMS")
+    try:
+        for prec, tol in ((capi.MIXED, 1e-6), (capi.FP64, 1e-12)):
+            capi.set_precision(prec)
+            say("split with an odd half,", n, "atoms, precision", prec)
+            os.environ["MAGACE_SPLIT_STREAMS"] = "0"
+            full = device_entry(a)
+            os.environ["MAGACE_SPLIT_STREAMS"] = "1"
+            compare_prefix(device_entry(b), full, n, len(b["dlist"]), tol, prec)
     finally:
         capi.set_precision(capi.FP64)
         if old is None:
