@@ -1121,7 +1121,17 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                                 // hoisted values would not fit in registers)
                                 constexpr bool kHoist = NM3X <= 8;
                                 cplx<T> wv[kHoist ? NM3X : 1];
-                                if constexpr (kHoist) {
+                                if constexpr (Q1 && sizeof(T) == 4) {
+                                    // FP32 rows are 32 bytes, 16-byte aligned: two ds_read_b128
+                                    // for columns 0, 2, 3 instead of three 8-byte reads
+                                    const float4 w23 = *(const float4*)(Wn + 2);
+                                    wv[2] = {w23.x, w23.y};
+                                    wv[3] = {w23.z, w23.w};
+                                    if (sg == 0) {
+                                        const float4 w01 = *(const float4*)Wn;
+                                        wv[0] = {w01.x, w01.y};
+                                    }
+                                } else if constexpr (kHoist) {
 #pragma unroll
                                     for (int q = 0; q < NM3X; ++q)
                                         if (!Q1 || used(q, sg)) wv[q] = Wn[q < P.nm3 ? q : 0];
