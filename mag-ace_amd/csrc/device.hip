@@ -536,7 +536,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                 if (n < P.nb_a)
 #pragma unroll
                     for (int q = 0; q < NR2X; ++q)
-                        if (q < P.nr2m) Zout[P.oA2 + (n * P.nr2m + q) * P.lsm + lm] = {cst * acc[b][q].re, cst * acc[b][q].im};
+                        if (q < P.nr2m)
+                            Zout[P.oA2 + (n * P.nr2m + q) * P.lsm + lm] = {cst * acc[b][q].re, cst * acc[b][q].im};
             }
         }
     }
@@ -670,11 +671,11 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, con
     // w * Z[a] Z[b] Z[c] to a single target x in registers (four 16-byte records per step,
     // loads issued together; the 64 lanes of a wave read consecutive records).  Thread t
     // takes chunks t + kAdjointThreads j; the chunk -> thread mapping must match the record
-    // layout built by build_terms.  While the thread's next chunk has the same target the sum stays in
-    // registers; it goes to LDS with one atomic when the target changes (a plain store when
-    // the chunk is its target's only one).  Each block starts
-    // at a different j (rotated by its index), so that the blocks of an XCD stream different
-    // parts of the record table at any moment instead of all reading the same L2 lines.
+    // layout built by build_terms.  While the thread's next chunk has the same target the sum
+    // stays in registers; it goes to LDS with one atomic when the target changes (a plain
+    // store when the chunk is its target's only one).  Each block starts at a different j
+    // (rotated by its index), so that the blocks of an XCD stream different parts of the
+    // record table at any moment instead of all reading the same L2 lines.
     // rho = Re sum over A0 targets of Z[x] * dRho/dZ[x] (every monomial has exactly one A0
     // factor), summed in FP64 once per flush.
     double rho = 0.0, rho2 = 0.0;
@@ -1522,7 +1523,8 @@ __global__ void __launch_bounds__(256) k_density_jvp(DevParams<double> P, const 
                 if (n < P.nb_a)
 #pragma unroll
                     for (int q = 0; q < NR2X; ++q)
-                        if (q < P.nr2m) Zdot[P.oA2 + (n * P.nr2m + q) * P.lsm + lm] = {cst * acc[b][q].re, cst * acc[b][q].im};
+                        if (q < P.nr2m)
+                            Zdot[P.oA2 + (n * P.nr2m + q) * P.lsm + lm] = {cst * acc[b][q].re, cst * acc[b][q].im};
             }
         }
     }
@@ -1856,9 +1858,11 @@ static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, co
     const bool pair = sizeof(T) == 4 && !nd2 && P.crec16 != nullptr;
     const size_t lds2p = 2 * (size_t)(P.nz + 1) * 16 + 2 * (kAdjointThreads / 64) * sizeof(double);
     if (pair && lds2p > limit[5]) return hipErrorInvalidValue;
-    const size_t lds2 = (nd2 ? 3 : 2) * (size_t)(P.nz + 1) * sizeof(cplx<T>) + 2 * (kAdjointThreads / 64) * sizeof(double);
+    const size_t lds2 =
+        (nd2 ? 3 : 2) * (size_t)(P.nz + 1) * sizeof(cplx<T>) + 2 * (kAdjointThreads / 64) * sizeof(double);
     const size_t lds3 = (size_t)P.nz * sizeof(cplx<T>);
-    if (lds1 > limit[0] || lds2 > limit[nd2 ? 3 : 1] || lds3 > limit[q1 ? (r4 ? 6 : 2) : 4]) return hipErrorInvalidValue;
+    if (lds1 > limit[0] || lds2 > limit[nd2 ? 3 : 1] || lds3 > limit[q1 ? (r4 ? 6 : 2) : 4])
+        return hipErrorInvalidValue;
     if (ev) (void)hipEventRecord(ev[0], s);
     hipLaunchKernelGGL(k1, dim3(natom), dim3(256), lds1, s, P, off, dlist, elist, dmomi, emomi, dmom,
                        emom, (cplx<T>*)B.Z, B.ehc, B.rho1, B.err);
