@@ -445,15 +445,15 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
     // Z[a], Z[b], Z[c] of their k-th records from LDS (three reads, random addresses), and
     // LDS serves a wave in lane groups whose accesses must fall in distinct banks.  The
     // order inside each chunk is therefore chosen greedily, step by step, so that the
-    // factor indices of the lanes of a group spread over the bank classes: x mod 8, mod 16
-    // and mod 32 over groups of 8, 16 and 32 lanes (16-byte FP64 and 8-byte FP32 entries
-    // with 32 or 64 banks).  Padding (index nz) is a broadcast read.
+    // factor indices of each group of 16 lanes spread over the 16 classes x mod 16: a
+    // 16-byte entry covers 4 of the 64 four-byte banks, so one pass serves 16 lanes.  The
+    // FP64 and the two-atom FP32 adjoints read such entries.  Of the criteria tried (x mod 8
+    // over 8 lanes, mod 16 over 16, mod 32 over 32, all three summed, none), this one was the
+    // fastest: the lmax-6 adjoint 1.10 -> 1.07 ms (FP32, two atoms) and 1.81 -> 1.77 ms (FP64)
+    // against the sum of the three.  Padding (index nz) is a broadcast read.
     struct Spread {
-        int load[3][64];  // t = 0, 1, 2: groups of 8 << t lanes, classes x mod (8 << t)
-        static int slot(int lane, int v, int t) {
-            const int sh = 3 + t;
-            return (lane >> sh) * (1 << sh) + (v & ((1 << sh) - 1));
-        }
+        int load[64];  // lane group (lane >> 4), class x mod 16
+        static int slot(int lane, int v) { return (lane >> 4) * 16 + (v & 15); }
     };
     for (size_t g0 = 0; g0 < order.size(); g0 += 64) {
         const size_t g1 = std::min(order.size(), g0 + 64);
@@ -468,7 +468,7 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
         for (int L = 0; L < nl; ++L)
             for (int k = 0; k < order[g0 + L].cnt; ++k) left[L].push_back(k);
         for (int k = 0; k < len; ++k) {
-            Spread sp[3];
+            Spread sp[3];  // one per factor slot
             std::memset(sp, 0, sizeof sp);
             for (int L = 0; L < nl; ++L) {
                 auto& rem = left[L];
@@ -480,14 +480,14 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
                     int cost = 0;
                     for (int f = 0; f < 3; ++f) {
                         if (e.first[f] == one) continue;
-                        for (int t = 0; t < 3; ++t) cost += sp[f].load[t][Spread::slot(L, e.first[f], t)];
+                        cost += sp[f].load[Spread::slot(L, e.first[f])];
                     }
                     if (cost < bestc) { bestc = cost; best = i; }
                 }
                 const auto& e = by[ch.x][ch.off + rem[best]];
                 for (int f = 0; f < 3; ++f)
                     if (e.first[f] != one)
-                        for (int t = 0; t < 3; ++t) ++sp[f].load[t][Spread::slot(L, e.first[f], t)];
+                        ++sp[f].load[Spread::slot(L, e.first[f])];
                 const size_t r = base + (size_t)k * 64 + L;
                 T.cidx[4 * r] = (uint16_t)e.first[0];
                 T.cidx[4 * r + 1] = (uint16_t)e.first[1];
