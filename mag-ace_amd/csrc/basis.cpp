@@ -220,6 +220,101 @@ uint64_t pack4(std::array<int, 4> v) {
 
 }  // namespace
 
+// LDS passes of one read instruction for one 16-lane group: lanes that read the same entry are
+// served by one broadcast, distinct entries of one class x mod 16 (the same banks) take one
+// pass each, so the cost is the largest number of distinct entries in one class.
+struct BankCell {
+    int n = 0;
+    int val[16], mult[16];
+    void add(int v) {  // at most 16 distinct entries: one per lane of the group
+        for (int i = 0; i < n; ++i)
+            if (val[i] == v) { ++mult[i]; return; }
+        if (n == 16) std::abort();
+        val[n] = v;
+        mult[n++] = 1;
+    }
+    void remove(int v) {
+        for (int i = 0; i < n; ++i)
+            if (val[i] == v) {
+                if (--mult[i] == 0) { val[i] = val[n - 1]; mult[i] = mult[n - 1]; --n; }
+                return;
+            }
+    }
+    bool has(int v) const {
+        for (int i = 0; i < n; ++i)
+            if (val[i] == v) return true;
+        return false;
+    }
+    int load(int c) const {  // distinct entries in class c
+        int k = 0;
+        for (int i = 0; i < n; ++i) k += (val[i] & 15) == c;
+        return k;
+    }
+    int passes() const {
+        int cnt[16] = {0}, m = 0;
+        for (int i = 0; i < n; ++i) m = std::max(m, ++cnt[val[i] & 15]);
+        return m;
+    }
+};
+
+// Local search after the greedy record order of one storage group (len steps x nl lanes at
+// base): swaps two records of one lane between steps while that lowers the group's LDS passes
+// (BankCell, for every step, 16-lane group and factor slot; padding, index `one`, is a
+// broadcast and not counted).  The greedy's last steps have few records left to choose from;
+// the swaps repair them.  One sweep over the lanes: on the lmax-6 header it takes the modelled
+// passes per read from 1.53 to 1.41 (a second sweep: 1.405, for twice the init time, 0.8 s).
+static void refine_banks(TermList& T, size_t base, int len, int nl, int one) {
+    const int ng = (nl + 15) / 16;
+    std::vector<BankCell> cell((size_t)len * ng * 3);  // [step][group][factor]
+    auto at = [&](int k, int g, int f) -> BankCell& { return cell[((size_t)k * ng + g) * 3 + f]; };
+    for (int k = 0; k < len; ++k)
+        for (int L = 0; L < nl; ++L)
+            for (int f = 0; f < 3; ++f) {
+                const int v = T.cidx[4 * (base + (size_t)k * 64 + L) + f];
+                if (v != one) at(k, L >> 4, f).add(v);
+            }
+    {
+        for (int L = 0; L < nl; ++L) {
+            const int g = L >> 4;
+            for (int k1 = 0; k1 < len; ++k1)
+                for (int k2 = k1 + 1; k2 < len; ++k2) {
+                    const size_t r1 = base + (size_t)k1 * 64 + L, r2 = base + (size_t)k2 * 64 + L;
+                    int v1[3], v2[3];
+                    bool same = true;
+                    for (int f = 0; f < 3; ++f) {
+                        v1[f] = T.cidx[4 * r1 + f];
+                        v2[f] = T.cidx[4 * r2 + f];
+                        same = same && v1[f] == v2[f];
+                    }
+                    if (same) continue;
+                    auto move = [&](bool fwd) {  // r1 to step k2 and r2 to k1, or back
+                        for (int f = 0; f < 3; ++f) {
+                            BankCell &a = at(k1, g, f), &b = at(k2, g, f);
+                            const int x = fwd ? v1[f] : v2[f], y = fwd ? v2[f] : v1[f];
+                            // x leaves a for b, y leaves b for a; removals first, so that no
+                            // cell holds more than its 16 lanes' entries
+                            if (x != one) a.remove(x);
+                            if (y != one) b.remove(y);
+                            if (x != one) b.add(x);
+                            if (y != one) a.add(y);
+                        }
+                    };
+                    int delta = 0;
+                    for (int f = 0; f < 3; ++f) delta -= at(k1, g, f).passes() + at(k2, g, f).passes();
+                    move(true);
+                    for (int f = 0; f < 3; ++f) delta += at(k1, g, f).passes() + at(k2, g, f).passes();
+                    if (delta < 0) {
+                        for (int q = 0; q < 4; ++q) std::swap(T.cidx[4 * r1 + q], T.cidx[4 * r2 + q]);
+                        std::swap(T.cw[r1], T.cw[r2]);
+                        std::swap(T.rec_t[r1], T.rec_t[r2]);
+                    } else {
+                        move(false);
+                    }
+                }
+        }
+    }
+}
+
 TermList build_terms(const Potential& pot, const ZLayout& z) {
     const Header& h = pot.h;
     const Basis& B = pot.basis;
@@ -445,16 +540,12 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
     // Z[a], Z[b], Z[c] of their k-th records from LDS (three reads, random addresses), and
     // LDS serves a wave in lane groups whose accesses must fall in distinct banks.  The
     // order inside each chunk is therefore chosen greedily, step by step, so that the
-    // factor indices of each group of 16 lanes spread over the 16 classes x mod 16: a
-    // 16-byte entry covers 4 of the 64 four-byte banks, so one pass serves 16 lanes.  The
-    // FP64 and the two-atom FP32 adjoints read such entries.  Of the criteria tried (x mod 8
-    // over 8 lanes, mod 16 over 16, mod 32 over 32, all three summed, none), this one was the
-    // fastest: the lmax-6 adjoint 1.10 -> 1.07 ms (FP32, two atoms) and 1.81 -> 1.77 ms (FP64)
-    // against the sum of the three.  Padding (index nz) is a broadcast read.
-    struct Spread {
-        int load[64];  // lane group (lane >> 4), class x mod 16
-        static int slot(int lane, int v) { return (lane >> 4) * 16 + (v & 15); }
-    };
+    // distinct factor indices of each group of 16 lanes spread over the 16 classes x mod 16
+    // (BankCell): a 16-byte entry covers 4 of the 64 four-byte banks, so one pass serves 16
+    // lanes, and lanes reading the same entry share a broadcast.  The FP64 and the two-atom
+    // FP32 adjoints read such entries.  Of the lane groupings tried (x mod 8 over 8 lanes,
+    // mod 16 over 16, mod 32 over 32, all three summed, none), 16 was the fastest.  A local
+    // search (refine_banks) then repairs the last steps.  Padding (index nz) is a broadcast.
     for (size_t g0 = 0; g0 < order.size(); g0 += 64) {
         const size_t g1 = std::min(order.size(), g0 + 64);
         int len = 0;
@@ -468,8 +559,7 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
         for (int L = 0; L < nl; ++L)
             for (int k = 0; k < order[g0 + L].cnt; ++k) left[L].push_back(k);
         for (int k = 0; k < len; ++k) {
-            Spread sp[3];  // one per factor slot
-            std::memset(sp, 0, sizeof sp);
+            BankCell sp[3][4];  // [factor slot][16-lane group] of this step
             for (int L = 0; L < nl; ++L) {
                 auto& rem = left[L];
                 if (rem.empty()) continue;
@@ -480,14 +570,15 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
                     int cost = 0;
                     for (int f = 0; f < 3; ++f) {
                         if (e.first[f] == one) continue;
-                        cost += sp[f].load[Spread::slot(L, e.first[f])];
+                        const BankCell& bc = sp[f][L >> 4];
+                        if (!bc.has(e.first[f])) cost += bc.load(e.first[f] & 15);
                     }
                     if (cost < bestc) { bestc = cost; best = i; }
                 }
                 const auto& e = by[ch.x][ch.off + rem[best]];
                 for (int f = 0; f < 3; ++f)
                     if (e.first[f] != one)
-                        ++sp[f].load[Spread::slot(L, e.first[f])];
+                        sp[f][L >> 4].add(e.first[f]);
                 const size_t r = base + (size_t)k * 64 + L;
                 T.cidx[4 * r] = (uint16_t)e.first[0];
                 T.cidx[4 * r + 1] = (uint16_t)e.first[1];
@@ -497,6 +588,7 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
                 rem.erase(rem.begin() + best);
             }
         }
+        refine_banks(T, base, len, nl, one);
         // every chunk of the group records the group's length: the lanes of a wave step
         // through the same number of records (those past a chunk's end have weight 0).  Bit
         // 22 marks the only chunk of its target: its sum is the target's whole gradient and
