@@ -6,6 +6,7 @@ terms) against central differences of L(c) = sum_s gE_s E_s + sum_j gF_j . F_j, 
 exact up to rounding when E is linear in c (nfemb 0) and, Richardson-extrapolated, checked
 to 1e-7 for nonlinear embeddings with core repulsion and the inner cutoff active.
 """
+import os
 import sys
 
 import numpy as np
@@ -177,3 +178,38 @@ def test_config_batch_fit_step(tmp_path):
     (E.sum() + (F ** 2).sum()).backward()
     assert torch.isfinite(c.grad).all()
     say("  atoms", batch.natom, "pairs", batch.npairs, "|grad|", float(c.grad.norm()))
+
+
+def test_split_halves_leave_the_fit_unchanged(tmp_path):
+    """Batches of 2048+ atoms run the forward evaluation as two halves of the atom range on two
+    streams (MAGACE_SPLIT_STREAMS=0 keeps one sequence); the backward pass reads the per-atom
+    buffers both halves wrote.  Energies, forces and dL/dc equal one sequence to 1e-12."""
+    load(tmp_path, "H500", 2034)
+    batch = fit.Batch(structures(11, sizes=((10, 10, 10), (4, 4, 4))), 5.0)
+    assert batch.natom >= 2048
+    say("fit: two halves,", batch.natom, "atoms")
+    rng = np.random.default_rng(11)
+    c0 = capi.fit_get_coefficients()
+    gE = torch.tensor(rng.normal(size=batch.nstruc), dtype=torch.float64, device=DEV)
+    gF = torch.tensor(rng.normal(size=(batch.natom, 3)), dtype=torch.float64, device=DEV)
+    old = os.environ.get("MAGACE_SPLIT_STREAMS")
+    out = {}
+    try:
+        for flag in ("0", "1"):
+            os.environ["MAGACE_SPLIT_STREAMS"] = flag
+            capi.fit_set_coefficients(c0)
+            E = torch.zeros(batch.nstruc, dtype=torch.float64, device=DEV)
+            F = torch.zeros(batch.natom, 3, dtype=torch.float64, device=DEV)
+            batch.forward(E, F)
+            g = torch.zeros(len(c0), dtype=torch.float64, device=DEV)
+            batch.backward(gE, gF.contiguous(), g)
+            out[flag] = (E.cpu().numpy(), F.cpu().numpy(), g.cpu().numpy())
+    finally:
+        if old is None:
+            os.environ.pop("MAGACE_SPLIT_STREAMS", None)
+        else:
+            os.environ["MAGACE_SPLIT_STREAMS"] = old
+    for a, b, what in zip(out["1"], out["0"], ("E", "F", "dL/dc")):
+        scale = max(np.abs(b).max(), 1e-300)
+        assert np.abs(a - b).max() <= 1e-12 * scale, what
+
