@@ -213,9 +213,10 @@ __device__ __forceinline__ cplx<T> yfull(const cplx<T>* half, int l, int m) {
 // Shape class 0: at least 5 waves per SIMD in FP32 (the register budget drops to 102; measured
 // 0.91 -> 0.77 ms on the lmax-6 workload, 0.52 -> 0.46 on lmax 4) and 3 in FP64 (168 registers;
 // at 169 the occupancy halves and the kernel takes 1.85 instead of 1.39 ms).  Class 1 (27
-// magnetic columns) would spill.
+// magnetic columns): 3 waves per SIMD in FP32 (168 registers, a 12-byte spill; on the shape
+// of the example potential 0.75 -> 0.57 ms); its FP64 instance needs more than 256.
 template <typename T, int L, int CH, int NB, int NM3X, int NB2, int NR2X, bool SPLIT, bool TASKSPLIT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6 ? (sizeof(T) == 4 ? 5 : 3) : 1, 8))) k_density(DevParams<T> P, const int* __restrict__ off,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6 ? (sizeof(T) == 4 ? 5 : 3) : (sizeof(T) == 4 ? 3 : 1), 8))) k_density(DevParams<T> P, const int* __restrict__ off,
                                                 const double* __restrict__ dlist,
                                                 const double* __restrict__ elist,
                                                 const double* __restrict__ dmomi,
@@ -243,29 +244,34 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
     const int p0 = off[atom], nn = off[atom + 1] - p0;
     Zout += (size_t)atom * P.nz;
 
-    // central-atom magnetic functions: A0 and B1 (src/a_basis.F90:69-106, b_basis.F90:62-83)
-    if (tid == 0) {
-        const double mu = dmomi[atom];
+    // central-atom magnetic functions: A0 and B1 (src/a_basis.F90:69-106, b_basis.F90:62-83),
+    // one spline function per thread and the moment's Y_lm on the last wave, so that the block
+    // does not wait for one thread to evaluate them all
+    for (int k = tid; k < P.nb_m + nfm; k += nthr) {
         T w;
         bool unit;
-        const T* row = magnetic_row(P, mu, w, unit, err);
-        double rho1[2] = {0.0, 0.0};
-        for (int k = 0; k < P.nb_m + nfm; ++k) {
-            T v = unit ? (T)1 : (T)0, dv;
-            if (row) horner(row + 4 * k, w, (T)P.rs_m, v, dv);
-            if (k < P.nb_m) {
-                rho1[0] += P.c1[0][k] * ((double)v - 1.0);
-                rho1[1] += P.c1[1][k] * ((double)v - 1.0);
-                sMc[nfm + k] = v;
-            } else {
-                sMc[k - P.nb_m] = v;
-            }
-        }
-        for (int d = 0; d < P.nd; ++d) rho1_out[(size_t)atom * P.nd + d] = rho1[d];
+        const T* row = magnetic_row(P, dmomi[atom], w, unit, err);
+        T v = unit ? (T)1 : (T)0, dv;
+        if (row) horner(row + 4 * k, w, (T)P.rs_m, v, dv);
+        if (k < P.nb_m)
+            sMc[nfm + k] = v;
+        else
+            sMc[k - P.nb_m] = v;
+    }
+    if (tid == nthr - 64) {
         ylm_half<T, L>((T)emomi[3 * atom], (T)emomi[3 * atom + 1], (T)emomi[3 * atom + 2], P.l2m, sYc);
         *sEhc = 0.0;
     }
     __syncthreads();
+    if (tid == 0) {  // B1 part of rho, in the order of the sum over k
+        double rho1[2] = {0.0, 0.0};
+        for (int k = 0; k < P.nb_m; ++k) {
+            const double v = (double)sMc[nfm + k];
+            rho1[0] += P.c1[0][k] * (v - 1.0);
+            rho1[1] += P.c1[1][k] * (v - 1.0);
+        }
+        for (int d = 0; d < P.nd; ++d) rho1_out[(size_t)atom * P.nd + d] = rho1[d];
+    }
     for (int t = tid; t < P.nr2m * P.lsm; t += nthr) {
         const int np = t / P.lsm, lm = t % P.lsm;
         int l = 0;
