@@ -214,9 +214,10 @@ __device__ __forceinline__ cplx<T> yfull(const cplx<T>* half, int l, int m) {
 // 0.91 -> 0.77 ms on the lmax-6 workload, 0.52 -> 0.46 on lmax 4) and 3 in FP64 (168 registers;
 // at 169 the occupancy halves and the kernel takes 1.85 instead of 1.39 ms).  Class 1 (27
 // magnetic columns): 3 waves per SIMD in FP32 (168 registers, a 12-byte spill; on the shape
-// of the example potential 0.75 -> 0.57 ms); its FP64 instance needs more than 256.
+// of the example potential 0.75 -> 0.57 ms) and 2 in FP64 (256 registers with a 192-byte
+// spill instead of 302 registers at 1 wave: 2.06 -> 1.33 ms).
 template <typename T, int L, int CH, int NB, int NM3X, int NB2, int NR2X, bool SPLIT, bool TASKSPLIT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6 ? (sizeof(T) == 4 ? 5 : 3) : (sizeof(T) == 4 ? 3 : 1), 8))) k_density(DevParams<T> P, const int* __restrict__ off,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6 ? (sizeof(T) == 4 ? 5 : 3) : (sizeof(T) == 4 ? 3 : 2), 8))) k_density(DevParams<T> P, const int* __restrict__ off,
                                                 const double* __restrict__ dlist,
                                                 const double* __restrict__ elist,
                                                 const double* __restrict__ dmomi,
