@@ -1353,8 +1353,11 @@ __device__ __forceinline__ void ylm_half_jvp(double x, double y, double z, int l
 
 // Zdot = D_u Z and D_u e_hc, the pair vector of pair p = (i -> j) moving by u_i - u_j.
 // Same task layout as k_density; A0 does not depend on positions.
+// The wide class (27 magnetic columns) runs at 2 waves per SIMD: 256 registers with a spill
+// instead of 294 at 1 wave (fit step on the shape of the example potential 41.3 -> 37.3 ms).
 template <int L, int CH, int NB, int NM3X, int NB2, int NR2X>
-__global__ void __launch_bounds__(256) k_density_jvp(DevParams<double> P, const int* __restrict__ off,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM3X > 8 ? 2 : 1, 8)))
+k_density_jvp(DevParams<double> P, const int* __restrict__ off,
                                                     const int* __restrict__ nlist, const double* __restrict__ du,
                                                     const double* __restrict__ dlist,
                                                     const double* __restrict__ elist,
