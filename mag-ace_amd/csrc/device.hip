@@ -1358,13 +1358,13 @@ __device__ __forceinline__ void ylm_half_jvp(double x, double y, double z, int l
 template <int L, int CH, int NB, int NM3X, int NB2, int NR2X>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM3X > 8 ? 2 : 1, 8)))
 k_density_jvp(DevParams<double> P, const int* __restrict__ off,
-                                                    const int* __restrict__ nlist, const double* __restrict__ du,
-                                                    const double* __restrict__ dlist,
-                                                    const double* __restrict__ elist,
-                                                    const double* __restrict__ dmom,
-                                                    const double* __restrict__ emom,
-                                                    cplx<double>* __restrict__ Zdot,
-                                                    double* __restrict__ ehc_dot, int* err) {
+              const int* __restrict__ nlist, const double* __restrict__ du,
+              const double* __restrict__ dlist,
+              const double* __restrict__ elist,
+              const double* __restrict__ dmom,
+              const double* __restrict__ emom,
+              cplx<double>* __restrict__ Zdot,
+              double* __restrict__ ehc_dot, int* err) {
     using T = double;
     extern __shared__ __align__(16) unsigned char smem[];
     const int atom = blockIdx.x;
