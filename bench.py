@@ -98,7 +98,8 @@ def fit_leg(pkg, td, steps=5):
     """Secondary measurement: one batched fit step = energies and forces of 256 periodic bcc Fe
     configurations (14-1023 atoms, log-uniform sizes, strain +-3%, 0-2 vacancies; seed 2029)
     and the gradient of a loss in all 489 coefficients of the lmax-4 header (FP64): the
-    device neighbour build runs once, each step is magace_fit_forward + magace_fit_backward."""
+    device neighbour build runs once, each step is magace_fit_set_coefficients +
+    magace_fit_forward + magace_fit_backward."""
     import torch
     capi, potgen, systems, fit = pkg.capi, pkg.potgen, pkg.systems, pkg.fit
     h = dict(potgen.H500)
@@ -120,6 +121,9 @@ def fit_leg(pkg, td, steps=5):
     g = torch.zeros(len(c), dtype=torch.float64, device=dev)
 
     def step():
+        # as an optimiser step runs it: new coefficients (merged weights recomputed and
+        # uploaded), forward, backward
+        capi.fit_set_coefficients(c)
         batch.forward(E, F)
         batch.backward(gE, gF, g)
 
@@ -131,38 +135,88 @@ def fit_leg(pkg, td, steps=5):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     return {"workload": "batched fit step: 256 bcc Fe configurations (%d atoms, %d pairs), lmax-4 header, "
-                        "energies + forces + dL/dc (489 coefficients), FP64" % (batch.natom, batch.npairs),
+                        "coefficient upload + energies + forces + dL/dc (489 coefficients), FP64" % (batch.natom, batch.npairs),
             "configs": batch.nstruc, "atoms": batch.natom, "ms_per_step": 1e3 * dt,
             "atom_steps_per_s": batch.natom / dt, "configs_per_s": batch.nstruc / dt,
             "neighbour_build_s": t_nb}
 
 
-def cpu_baseline(pkg, pot, a, natom_sample, use_reference=True):
-    """Reference libace (oracle/_ref, serial Fortran) on the first natom_sample atoms of the
-    same workload; falls back to the oracle's C restatement (OpenMP) if the reference
-    binary is absent."""
+def _sub(a, lo, hi):
+    """acelib_ arrays of atoms [lo, hi) of a packed list (pair ranges re-based)."""
+    off = a["offsets"]
+    p0, p1 = int(off[lo]), int(off[hi])
+    st = (a["nstart"][lo:hi] - p0).astype(np.int32)
+    sp = (a["nstop"][lo:hi] - p0).astype(np.int32)
+    return dict(nstart=st, nstop=sp, nlist=a["nlist"][p0:p1], occ=a["occ"][lo:hi], occlist=a["occlist"][p0:p1],
+                dlist=a["dlist"][p0:p1], elist=a["elist"][3 * p0:3 * p1], dlist_momi=a["dlist_momi"][lo:hi],
+                elist_momi=a["elist_momi"][3 * lo:3 * hi], dlist_mom=a["dlist_mom"][p0:p1],
+                elist_mom=a["elist_mom"][3 * p0:3 * p1], nall=hi - lo)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(pot, a, atoms_per_proc=100, nproc=None, nreps=5, single_atoms=400):
+    """The reference libace (oracle/_ref: built from its Fortran sources, AMD flang -O2, serial)
+    on the host cores, as LAMMPS runs it under MPI: nproc independent processes, each calling
+    acelib_ on its own sub-range of atoms_per_proc atoms of the same workload (its pairs
+    prebuilt), all at once.  Each process makes 1 warm-up and nreps timed calls; the rate is
+    the sum over processes of atoms / median call time.  Beside it, one process on
+    single_atoms atoms alone (1 core).  nproc defaults to the host's CPU share: at most 16
+    cores.  Returns None if the reference binary is absent."""
+    import concurrent.futures as cf
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    if not os.path.exists(O.REF_DRIVER):
+        return None
+    ncpu = os.cpu_count() or 1
+    nproc = nproc or min(16, ncpu)
+    drv = O.RefDriver(pot)
+    t0 = time.perf_counter()
+    one = drv.run(_sub(a, 0, single_atoms), nreps=1 + nreps, timeout=900)
+    t1 = float(np.median(one["seconds"][1:]))
+    single = dict(value=single_atoms / t1, cores=1, atoms=single_atoms, median_s=t1,
+                  seconds=one["seconds"][1:].tolist())
+    subs = [_sub(a, k * atoms_per_proc, (k + 1) * atoms_per_proc) for k in range(nproc)]
+    with cf.ThreadPoolExecutor(max_workers=nproc) as ex:
+        outs = list(ex.map(lambda sb: drv.run(sb, nreps=1 + nreps, timeout=900), subs))
+    med = [float(np.median(o["seconds"][1:])) for o in outs]
+    rate = sum(atoms_per_proc / m for m in med)
+    log("cpu baseline: %.1f s wall" % (time.perf_counter() - t0))
+    return {"value": rate, "unit": "atom-steps/s", "cores": nproc, "kind": "reference",
+            "sample": "%d processes at once, each acelib_ on its own %d of the 16000 atoms (%d atoms, %d pairs "
+                      "in all), reference libace built from its Fortran sources with AMD flang -O2 (serial), "
+                      "1 warm-up + %d timed calls per process, rate = sum of atoms / median call time"
+                      % (nproc, atoms_per_proc, nproc * atoms_per_proc,
+                         int(a["offsets"][nproc * atoms_per_proc]), nreps),
+            "warmup": 1, "nreps": nreps, "host_cpus": ncpu, "cpu_model": cpu_model(),
+            "median_s_per_process": med, "single_core": single}
+
+
+def oracle_errors(pot, a, outs, natom_sample=2000):
+    """Per-output errors of the GPU outputs against the oracle (the C restatement of the
+    reference, FP64, 16 threads) on the first natom_sample atoms of the workload: max |dE_i|,
+    max |dF_p|, max |dT_p|, max |dT0_i| (eV, eV/A, eV/muB) per precision, and the FP64 path's
+    norm-wise relative error per output array."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     S = natom_sample
     P = int(a["offsets"][S])
-    sub = dict(nstart=a["nstart"][:S], nstop=a["nstop"][:S], nlist=a["nlist"][:P], occ=a["occ"][:S],
-               occlist=a["occlist"][:P], dlist=a["dlist"][:P], elist=a["elist"][:3 * P],
-               dlist_momi=a["dlist_momi"][:S], elist_momi=a["elist_momi"][:3 * S], dlist_mom=a["dlist_mom"][:P],
-               elist_mom=a["elist_mom"][:3 * P], nall=S)
-    if use_reference and os.path.exists(O.REF_DRIVER):
-        out = O.RefDriver(pot).run(sub, nreps=2, timeout=900)
-        t = float(np.median(out["seconds"]))
-        return dict(value=S / t, unit="atom-steps/s", cores=1, kind="reference",
-                    sample="%d of the 16000 atoms (%d pairs), reference libace built from its Fortran "
-                           "sources with AMD flang -O2, serial, median of 2 acelib_ calls" % (S, P))
-    nthr = min(16, os.cpu_count() or 1)
-    o = O.Oracle.load(pot)
-    o.acelib(sub, nthreads=nthr)
-    t0 = time.perf_counter()
-    o.acelib(sub, nthreads=nthr)
-    t = time.perf_counter() - t0
-    return dict(value=S / t, unit="atom-steps/s", cores=nthr, kind="port",
-                sample="%d of the 16000 atoms, C restatement of the reference (OpenMP)" % S)
+    ref = O.Oracle.load(pot).acelib(_sub(a, 0, S), nthreads=min(16, os.cpu_count() or 1))
+    cut = dict(eout=S, flist=3 * P, tlist=3 * P, tlist0=3 * S)
+    res = {"sample": "first %d atoms (%d pairs) of the 16000" % (S, P)}
+    for prec, o in outs.items():
+        res[prec] = {k: float(np.abs(o[k][:n] - ref[k]).max()) for k, n in cut.items()}
+        res[prec + "_rel"] = {k: float(np.abs(o[k][:n] - ref[k]).max() / max(np.abs(ref[k]).max(), 1e-300))
+                              for k, n in cut.items()}
+    return res
 
 
 def main():
@@ -172,7 +226,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--ncell", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=400)
+    ap.add_argument("--cpu-sample", type=int, default=100,
+                    help="atoms per reference process in the CPU baseline")
     ap.add_argument("--no-md", action="store_true",
                     help="skip the secondary legs (128000-atom spin-lattice MD, 256-configuration fit step)")
     ap.add_argument("--no-host-arrays", action="store_true",
@@ -316,19 +371,18 @@ def main():
         mdres = md_leg(pkg, td)
         log("md (128000 atoms):", {k: v for k, v in mdres.items() if k != "workload"})
 
-    cpu = cpu_all = None
+    cpu = orc = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(pkg, pot, a, min(args.cpu_sample, N))
+            cpu = cpu_baseline(pot, a, atoms_per_proc=min(args.cpu_sample, N // 16))
             log("cpu baseline:", cpu)
         except Exception as e:  # the headline line must still be printed
             log("cpu baseline failed:", e)
         try:
-            # the same algorithm on all host cores: the oracle's C restatement, OpenMP
-            cpu_all = cpu_baseline(pkg, pot, a, min(4 * args.cpu_sample, N), use_reference=False)
-            log("cpu, all cores:", cpu_all)
+            orc = oracle_errors(pot, a, {"mixed": o32, "fp64": o64}, min(2000, N))
+            log("errors vs the oracle:", orc)
         except Exception as e:
-            log("cpu all-cores baseline failed:", e)
+            log("oracle comparison failed:", e)
 
     line = {
         "metric": "atom-steps/s E+F+torques, 16k Fe atoms: mixed vs FP64 @ force err <=1e-4 eV/A",
@@ -359,6 +413,7 @@ def main():
                      "traffic": mixed_traffic(kernels[dom]) if args.ncell == 20 else None,
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"},
         "cpu_baseline": cpu,
+        "error_vs_oracle": orc,
         "neighbor_build_ms": nb_ms,
         "acelib_host_arrays": None if host_ms is None else {
             "ms_per_call": host_ms, "atom_steps_per_s": N / (1e-3 * host_ms),
@@ -369,9 +424,7 @@ def main():
     }
     if cpu:
         line["gpu_over_cpu"] = res["mixed"]["value"] / cpu["value"]
-    if cpu_all:
-        line["cpu_all_cores"] = cpu_all
-        line["gpu_over_cpu_all_cores"] = res["mixed"]["value"] / cpu_all["value"]
+        line["gpu_over_cpu_single_core"] = res["mixed"]["value"] / cpu["single_core"]["value"]
     print(json.dumps(line))
 
 

@@ -145,10 +145,12 @@ int magace_md_create(int natom, const double *cell, double rlist, double mass, d
                      const double *smag, void **md);
 int magace_md_run(void *md, int nsteps, int use_graph);
 /* Copies the state back (any output may be NULL): x, v, sdir, f (forces), dEdm [3*natom],
- * eatom [natom]; epot = sum eatom and ekin (eV); counters = {steps, rebuilds, graph
- * captures, pairs}. */
+ * eatom [natom]; epot = sum eatom and ekin (eV); virial[9] = W_ab (row-major) = sum over
+ * pairs of u_a F_b with u = x_i - x_j and F = dE_i/du (eV), the tally of
+ * lammps/pair_spin_ace.cpp:635-636; epot and virial are reduced on the device in FP64 in a
+ * fixed order; counters = {steps, rebuilds, graph captures, pairs}. */
 int magace_md_read(void *md, double *x, double *v, double *sdir, double *f, double *dEdm,
-                   double *eatom, double *epot, double *ekin, long long *counters);
+                   double *eatom, double *epot, double *ekin, double *virial, long long *counters);
 int magace_md_destroy(void *md);
 
 /* Fitting (FP64).  Coefficients are ordered as the potential file's B functions: c1 (order
@@ -194,8 +196,9 @@ int magace_model_info(long long *info, int n);
 int magace_model_terms(unsigned short *idx, double *w, int cap);
 /* The transposed gradient table the adjoint kernel streams (dRho/dZ[x] gathered per target
  * x, the gather form of src/forces.F90:84-349): chunks[2c] = first record, chunks[2c+1] =
- * x | count << 16 | sole << 22 (count = the length of the chunk's group of 64, sole = the
- * target's only chunk); record r holds the
+ * x | count << 12 | sole << 18 | slot << 19 (unsigned; count = the length of the chunk's
+ * group of 64, sole = the target's only chunk, slot = the partial-sum slot of the chunk's run
+ * of consecutive same-target chunks on one adjoint thread); record r holds the
  * factors idx[4r..4r+2] and the weight w[r] (density 1); a chunk contributes
  * sum_j w Z[a_j] Z[b_j] Z[c_j] over its count records, 64 apart.  Copies when the caps
  * suffice; returns the record count (chunk count in *nchunks).  Host-side introspection. */

@@ -71,7 +71,7 @@ def lib():
     L.magace_md_create.argtypes = [C.c_int, _D, C.c_double, C.c_double, C.c_double, C.c_int, _D, _D, _D, _D,
                                    C.POINTER(C.c_void_p)]
     L.magace_md_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
-    L.magace_md_read.argtypes = [C.c_void_p] + [C.c_void_p] * 8 + [C.c_void_p]
+    L.magace_md_read.argtypes = [C.c_void_p] + [C.c_void_p] * 9 + [C.c_void_p]
     L.magace_md_destroy.argtypes = [C.c_void_p]
     L.magace_packstructures.argtypes = [C.c_int, _I, _I, _D, C.c_double, _D, _D, C.c_longlong,
                                         _I, _I, _I, _I, _D, _D, _D, _D, _D, _D, _I, _IP]

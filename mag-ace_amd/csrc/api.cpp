@@ -28,6 +28,7 @@
 namespace magace {
 
 static_assert(TermList::kLanes == kAdjointThreads, "the chunk order must be built for the adjoint's workgroup size");
+static_assert(TermList::kGroup == kCombGroup, "combine groups must match the kernel's");
 
 namespace {
 
@@ -81,6 +82,8 @@ struct Context {
     double* cw2_64 = nullptr;  // density-2 record weights (ndensity 2)
     float* cw2_32 = nullptr;
     int2* chunks = nullptr;
+    int2* comb = nullptr;
+    int2* comb2 = nullptr;
     uint32_t* fold = nullptr;
     Buffers buf;
     std::vector<int> h_off;
@@ -102,6 +105,7 @@ struct Context {
     // fork and join of an evaluation run as two halves (see run_device)
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool uploaded = false;
+    int device = -1;  // the device the tables were uploaded to
     // fitting: adjoint buffers, pair outputs of the fit forward, device coefficient structure
     FitBuffers fit;
     size_t fit_cap_pairs = 0;
@@ -116,6 +120,10 @@ struct Context {
 };
 
 std::unique_ptr<Context> g_ctx;
+
+// generation of the state baked into captured evaluations (context_generation)
+std::atomic<unsigned long long> g_generation{1};
+void bump_generation() { g_generation.fetch_add(1); }
 
 #define HIPCHK(expr)                                                                      \
     do {                                                                                  \
@@ -135,8 +143,10 @@ void dfree(T*& p) {
 
 Context::~Context() {
     dfree(t64.lut_a); dfree(t64.lut_m); dfree(t32.lut_a); dfree(t32.lut_m);
-    dfree(crec64); dfree(crec32); dfree(crec16); dfree(chunks); dfree(fold); dfree(cw2_64); dfree(cw2_32);
+    dfree(crec64); dfree(crec32); dfree(crec16); dfree(chunks); dfree(comb); dfree(comb2); dfree(fold); dfree(cw2_64);
+    dfree(cw2_32);
     dfree(buf.Z); dfree(buf.W); dfree(buf.ehc); dfree(buf.rho1); dfree(buf.dE); dfree(buf.edf); dfree(buf.rho);
+    dfree(buf.Pt);
     dfree(buf.err);
     dfree(fit.Zdot); dfree(fit.ehc_dot); dfree(fit.alpha); dfree(fit.beta); dfree(fit.M); dfree(fit.idx);
     dfree(fit.ub_ptr); dfree(fit.ub_t); dfree(fit.ub_u);
@@ -177,6 +187,9 @@ void fill_params(DevParams<T>& P, const Model& M) {
     P.lha = (h.l3a + 1) * (h.l3a + 2) / 2;
     P.lhm = (h.l2m + 1) * (h.l2m + 2) / 2;
     P.nchunks = (int)(M.terms.chunks.size() / 2);
+    P.ncomb = (int)(M.terms.comb.size() / 2);
+    P.ncomb2 = (int)(M.terms.comb2.size() / 2);
+    P.nslot = M.terms.nslot;
     P.nfemb = h.nfemb;
     P.nd = h.ndensity;
     for (int d = 0; d < 2; ++d) {
@@ -231,6 +244,11 @@ int build_context(Potential&& pot, std::unique_ptr<Context>& out) {
     M.terms = build_terms(M.pot, M.z);
     if (M.terms.ncontrib() >= (size_t(1) << 27)) {
         set_error(MAGACE_ERR_INPUT, "gradient record table too large (2^27 records)");
+        return MAGACE_ERR_INPUT;
+    }
+    // chunk headers hold the target in 12 bits and the run's slot in 13
+    if (M.z.nz >= 4096 || M.terms.nslot >= 8192) {
+        set_error(MAGACE_ERR_INPUT, "gradient table too large (feature vector >= 4096 or >= 8192 partial sums)");
         return MAGACE_ERR_INPUT;
     }
     M.spl = build_splines(M.pot);
@@ -300,6 +318,7 @@ int refresh_weights(Context& c) {
     }
     set_c1(c.P64, c.model);
     set_c1(c.P32, c.model);
+    bump_generation();  // c1 travels by value in the kernel arguments
     if (c.uploaded && nr) return upload_records(c);
     return 0;
 }
@@ -323,6 +342,12 @@ int upload(Context& c) {
     HIPCHK(hipMalloc((void**)&c.crec32, nc * sizeof(uint4) + 64));
     HIPCHK(hipMalloc((void**)&c.crec16, nc * sizeof(uint4) + 64));
     HIPCHK(hipMalloc((void**)&c.chunks, nch * sizeof(int2) + 64));
+    const size_t ncomb = M.terms.comb.size() / 2;
+    HIPCHK(hipMalloc((void**)&c.comb, ncomb * sizeof(int2) + 64));
+    if (ncomb) HIPCHK(hipMemcpy(c.comb, M.terms.comb.data(), ncomb * sizeof(int2), hipMemcpyHostToDevice));
+    const size_t ncomb2 = M.terms.comb2.size() / 2;
+    HIPCHK(hipMalloc((void**)&c.comb2, ncomb2 * sizeof(int2) + 64));
+    if (ncomb2) HIPCHK(hipMemcpy(c.comb2, M.terms.comb2.data(), ncomb2 * sizeof(int2), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc((void**)&c.fold, M.terms.fold.size() * sizeof(uint32_t) + 64));
     HIPCHK(hipMemcpy(c.fold, M.terms.fold.data(), M.terms.fold.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     if (nc) {
@@ -337,8 +362,12 @@ int upload(Context& c) {
     c.P32.crec = c.crec32;
     c.P32.crec16 = c.crec16;
     c.P64.chunks = c.P32.chunks = c.chunks;
+    c.P64.comb = c.P32.comb = c.comb;
+    c.P64.comb2 = c.P32.comb2 = c.comb2;
     c.P64.fold = c.P32.fold = c.fold;
     c.uploaded = true;
+    c.device = dev;
+    bump_generation();
     return 0;
 }
 
@@ -356,7 +385,10 @@ int ensure_atoms(Context& c, size_t natom) {
     HIPCHK(hipMalloc((void**)&c.buf.edf, cap * sizeof(double)));
     dfree(c.buf.rho);
     HIPCHK(hipMalloc((void**)&c.buf.rho, cap * nd * sizeof(double)));
+    dfree(c.buf.Pt);
+    HIPCHK(hipMalloc(&c.buf.Pt, cap * partials_bytes(c.model.terms.nslot, (int)nd) + 256));
     c.buf.cap_atoms = cap;
+    bump_generation();  // captured evaluations refer to the old buffers
     return 0;
 }
 
@@ -379,8 +411,9 @@ bool split_enabled() {
 }
 
 // the device buffers of atoms [a, ...)
-Buffers buffers_from(const Buffers& B, size_t a, size_t zbytes, int nd) {
+Buffers buffers_from(const Buffers& B, size_t a, size_t zbytes, int nd, int nslot) {
     Buffers o = B;
+    o.Pt = (char*)B.Pt + a * partials_bytes(nslot, nd);
     o.Z = (char*)B.Z + a * zbytes;
     o.W = (char*)B.W + a * zbytes;
     o.ehc = B.ehc + a;
@@ -392,26 +425,31 @@ Buffers buffers_from(const Buffers& B, size_t a, size_t zbytes, int nd) {
     return o;
 }
 
-hipError_t launch(Context& c, int natom, const int* off, const double* dlist, const double* elist,
+hipError_t launch(Context& c, int natom, int atom_base, const int* off, const double* dlist, const double* elist,
                   const double* dmomi, const double* emomi, const double* dmom, const double* emom,
                   double* eout, double* flist, double* tlist, double* tlist0, Buffers& B, hipStream_t s,
                   hipEvent_t* ev) {
     if (c.precision == MAGACE_FP64)
-        return Launch<double>::run(c.P64, natom, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist, tlist,
-                                   tlist0, B, s, c.shape_class, ev);
-    return Launch<float>::run(c.P32, natom, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist, tlist,
-                              tlist0, B, s, c.shape_class, ev);
+        return Launch<double>::run(c.P64, natom, atom_base, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist,
+                                   tlist, tlist0, B, s, c.shape_class, ev);
+    return Launch<float>::run(c.P32, natom, atom_base, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist,
+                              tlist, tlist0, B, s, c.shape_class, ev);
 }
 
 int run_device(Context& c, int natom, const int* off, const double* dlist, const double* elist,
                const double* dmomi, const double* emomi, const double* dmom, const double* emom,
                double* eout, double* flist, double* tlist, double* tlist0, hipStream_t s,
-               bool clear_err = true, bool allow_split = true) {
-    if (int rc = upload(c)) return rc;
-    if (int rc = ensure_atoms(c, (size_t)natom)) return rc;
+               bool clear_err = true, bool allow_split = true, Buffers* ext = nullptr, int atom_base = 0) {
+    // any evaluation overwrites the per-atom buffers a fit forward left for its backward
+    c.fit_natom = -1;
+    if (!ext) {
+        if (int rc = upload(c)) return rc;
+        if (int rc = ensure_atoms(c, (size_t)natom)) return rc;
+    }
+    Buffers& buf = ext ? *ext : c.buf;
     if (clear_err) HIPCHK(clear_flags(c.buf.err, s));
     hipEvent_t* ev = nullptr;
-    if (c.profiling) {
+    if (c.profiling && !ext) {
         if (c.events_used == c.events.size()) {
             std::array<hipEvent_t, 4> q;
             for (auto& x : q) HIPCHK(hipEventCreate(&x));
@@ -426,18 +464,21 @@ int run_device(Context& c, int natom, const int* off, const double* dlist, const
         split = cs == hipStreamCaptureStatusNone;
     }
     if (!split) {
-        HIPCHK(launch(c, natom, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist, tlist, tlist0, c.buf, s,
-                      ev));
+        HIPCHK(launch(c, natom, atom_base, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist, tlist, tlist0,
+                      buf, s, ev));
         return 0;
     }
-    const int h = natom / 2;
+    // an even split keeps the two-atom adjoint's atom pairs (and so every result bit) as in
+    // one sequence
+    const int h = (natom / 2) & ~1;
     const size_t zb = (size_t)c.model.z.nz * (c.precision == MAGACE_FP64 ? 16 : 8);
-    Buffers B1 = buffers_from(c.buf, (size_t)h, zb, c.model.pot.h.ndensity);
+    Buffers B1 = buffers_from(buf, (size_t)h, zb, c.model.pot.h.ndensity, c.model.terms.nslot);
     HIPCHK(hipEventRecord(c.ev_fork, s));
     HIPCHK(hipStreamWaitEvent(c.stream, c.ev_fork, 0));
-    HIPCHK(launch(c, h, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist, tlist, tlist0, c.buf, s, nullptr));
-    HIPCHK(launch(c, natom - h, off + h, dlist, elist, dmomi + h, emomi + 3 * (size_t)h, dmom, emom, eout + h, flist,
-                  tlist, tlist0 + 3 * (size_t)h, B1, c.stream, nullptr));
+    HIPCHK(launch(c, h, atom_base, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist, tlist, tlist0, buf, s,
+                  nullptr));
+    HIPCHK(launch(c, natom - h, atom_base + h, off + h, dlist, elist, dmomi + h, emomi + 3 * (size_t)h, dmom, emom,
+                  eout + h, flist, tlist, tlist0 + 3 * (size_t)h, B1, c.stream, nullptr));
     HIPCHK(hipEventRecord(c.ev_join, c.stream));
     HIPCHK(hipStreamWaitEvent(s, c.ev_join, 0));
     return 0;
@@ -495,6 +536,7 @@ std::vector<int> chunk_plan(const std::vector<int>& off, int cmax) {
         const int a0 = i;
         ++i;
         while (i < n && off[i + 1] - off[a0] <= target) ++i;
+        if (i < n && (i & 1)) ++i;  // chunks start at even atoms (the adjoint's atom pairs)
         cut.push_back(i);
     }
     return cut;
@@ -516,6 +558,18 @@ int check_kernel_errors(Context& c, hipStream_t s) {
 }
 
 }  // namespace
+
+unsigned long long context_generation() { return g_generation.load(); }
+
+int context_prepare(int natom) {
+    if (!g_ctx) {
+        set_error(MAGACE_ERR_STATE, "library not initialised");
+        return MAGACE_ERR_STATE;
+    }
+    if (int rc = upload(*g_ctx)) return rc;
+    return ensure_atoms(*g_ctx, (size_t)std::max(natom, 0));
+}
+
 }  // namespace magace
 
 using namespace magace;
@@ -546,6 +600,7 @@ int magace_init_file(const char* path, const char* element, double rcut, double 
     if (int rc = build_context(std::move(pot), ctx)) return rc;
     ctx->precision = prec;
     g_ctx = std::move(ctx);
+    bump_generation();
     return 0;
 }
 
@@ -573,6 +628,7 @@ void acelibini_(int* nele, double* rcut, double* rcut_magnetic, double* smear, i
 
 int magace_finalize(void) {
     g_ctx.reset();
+    bump_generation();
     return 0;
 }
 
@@ -585,6 +641,7 @@ int magace_set_precision(int mode) {
         set_error(MAGACE_ERR_STATE, "library not initialised");
         return MAGACE_ERR_STATE;
     }
+    if (g_ctx->precision != mode) bump_generation();
     g_ctx->precision = mode;
     return 0;
 }
@@ -760,7 +817,8 @@ void acelib_(int* natom, int* nall, int* nei, int* nlist, int* occ, int* occlist
         if (run_device(c, (int)L.na, (const int*)(d + L.i_off), (const double*)(d + L.i_dl),
                        (const double*)(d + L.i_el), (const double*)(d + L.i_dmi), (const double*)(d + L.i_emi),
                        (const double*)(d + L.i_dm), (const double*)(d + L.i_em), (double*)(o + L.o_eout),
-                       (double*)(o + L.o_fl), (double*)(o + L.o_tl), (double*)(o + L.o_t0), s, false, false))
+                       (double*)(o + L.o_fl), (double*)(o + L.o_tl), (double*)(o + L.o_t0), s, false, false, nullptr,
+                       cut[k]))
             return;
         if (fail(hipEventRecord(sl.ev_comp, s), "event") || fail(hipStreamWaitEvent(c.s_out, sl.ev_comp, 0), "event") ||
             fail(hipMemcpyAsync(sl.h_out, sl.d_out, L.out_bytes, hipMemcpyDeviceToHost, c.s_out), "copy") ||

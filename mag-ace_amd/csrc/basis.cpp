@@ -607,8 +607,40 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
             const Ch& ch = order[g0 + L];
             const bool sole = rowchunks[ch.x].size() == 1;
             T.chunks.push_back((int32_t)(base + L));
-            T.chunks.push_back(ch.x | (len << 16) | (sole ? 1 << 22 : 0));
+            T.chunks.push_back(ch.x | (len << 12) | (sole ? 1 << 18 : 0));
         }
+    }
+    // partial-sum slots of the runs of targets with more than one chunk (TermList::comb)
+    {
+        const size_t L = TermList::kLanes, nc = T.chunks.size() / 2;
+        std::vector<std::vector<std::vector<size_t>>> runs(z.nz);  // target -> runs -> chunks
+        for (size_t lane = 0; lane < L && lane < nc; ++lane)
+            for (size_t c = lane; c < nc; c += L) {
+                const int x = T.chunks[2 * c + 1] & 0xfff;
+                if (rowchunks[x].size() == 1) continue;
+                const bool cont = c >= L && (T.chunks[2 * (c - L) + 1] & 0xfff) == x;
+                if (!cont) runs[x].emplace_back();
+                runs[x].back().push_back(c);
+            }
+        int slot = 0;
+        const int G = TermList::kGroup;
+        for (int x = 0; x < z.nz; ++x) {
+            if (runs[x].empty()) continue;
+            const int n = (int)runs[x].size(), ng = (n + G - 1) / G;
+            for (int g = 0; g < ng; ++g) {
+                T.comb.push_back(slot + g * G);
+                T.comb.push_back(x | (std::min(G, n - g * G) << 12) | (ng == 1 ? 1 << 16 : 0));
+            }
+            if (ng > 1) {
+                T.comb2.push_back(slot);
+                T.comb2.push_back(x | (ng << 12));
+            }
+            for (const auto& r : runs[x]) {
+                for (size_t c : r) T.chunks[2 * c + 1] = (int32_t)((uint32_t)T.chunks[2 * c + 1] | ((uint32_t)slot << 19));
+                ++slot;
+            }
+        }
+        T.nslot = slot;
     }
     if (nd > 1) {
         const size_t nr = T.rec_t.size();

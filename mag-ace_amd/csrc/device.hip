@@ -8,10 +8,10 @@
 //              (l'm', n-block) columns of A2 and accumulate in registers.  Writes
 //              Z = [A | A2 | A0] per atom, the core sum and the B1 part of rho.
 //   k_adjoint  (prepB + forceweights, src/b_basis.F90:62-307, src/forces.F90:84-349)
-//              one workgroup per atom.  Z and dZ in LDS; each thread walks monomials
-//              w*Z[a]Z[b]Z[c]Z[d] of the merged term list, accumulating rho and scattering
-//              the holomorphic gradient with LDS atomics; then E_i, dE/drho and the scaled
-//              gradient W = dE/dZ are written.
+//              one workgroup per atom (two in FP32).  Z and dZ in LDS; each thread sums
+//              chunks of gradient records w*Z[a]Z[b]Z[c] per target (the transposed
+//              monomial table), partial sums meet in a fixed order (no atomics); then rho,
+//              E_i, dE/drho and the scaled gradient W = dE/dZ are written.
 //   k_forces   (forcecompute + torquecompute, src/forces.F90:418-623, 688-961) one lane per
 //              pair.  Splines and Y_lm are recomputed in registers (nothing per pair is
 //              stored between launches); F_p = dE_i/du_p and T_p = dE_i/dm_j are
@@ -567,9 +567,6 @@ template <> __device__ __forceinline__ double rec_weight<double>(uint4 r) {
 }
 template <> __device__ __forceinline__ float rec_weight<float>(uint4 r) { return __uint_as_float(r.z); }
 
-__device__ __forceinline__ void lds_add(double* p, double v) { atomicAdd(p, v); }
-__device__ __forceinline__ void lds_add(float* p, float v) { atomicAdd(p, v); }
-
 template <typename T>
 __device__ __forceinline__ double block_sum(double v, double* red) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -646,11 +643,62 @@ __device__ inline void atom_energy(const DevParams<T>& P, int nd, int atom, cons
     edfout[atom] = dfc * Fs + 1.0;
 }
 
+__device__ __forceinline__ float4 vadd(float4 a, float4 b) {
+    return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> vadd(cplx<T> a, cplx<T> b) {
+    return {a.re + b.re, a.im + b.im};
+}
+
+// The adjoint's partial sums of one atom (or atom pair), part[slot], added up per target in a
+// fixed order.  First pass: a thread sums one group of <= kCombGroup consecutive slots of a
+// target, its loads issued together, into sW[x] when the group is the target's only one and
+// into the group's first slot otherwise.  Second pass: a thread sums the group results of one
+// target with several groups.  Two round trips to L2 instead of one per slot.
+template <typename P_t, typename V>
+__device__ __forceinline__ void combine_partials(const P_t& P, V* __restrict__ part, V* __restrict__ sW) {
+    const int nthr = (int)blockDim.x, tid = (int)threadIdx.x;
+    for (int k = tid; k < P.ncomb; k += nthr) {
+        const int2 e = P.comb[k];
+        const int x = e.y & 0xfff, n = (e.y >> 12) & 15;
+        V v[kCombGroup];
+#pragma unroll
+        for (int u = 0; u < kCombGroup; ++u)
+            if (u < n) v[u] = part[e.x + u];
+        V sum = v[0];
+#pragma unroll
+        for (int u = 1; u < kCombGroup; ++u)
+            if (u < n) sum = vadd(sum, v[u]);
+        if ((e.y >> 16) & 1)
+            sW[x] = sum;
+        else
+            part[e.x] = sum;
+    }
+    __syncthreads();
+    for (int k = tid; k < P.ncomb2; k += nthr) {
+        const int2 e = P.comb2[k];
+        const int x = e.y & 0xfff, ng = e.y >> 12;
+        V sum = part[e.x];
+        for (int g0 = 1; g0 < ng; g0 += kCombGroup) {
+            V v[kCombGroup];
+#pragma unroll
+            for (int u = 0; u < kCombGroup; ++u)
+                if (g0 + u < ng) v[u] = part[e.x + (g0 + u) * kCombGroup];
+#pragma unroll
+            for (int u = 0; u < kCombGroup; ++u)
+                if (g0 + u < ng) sum = vadd(sum, v[u]);
+        }
+        sW[x] = sum;
+    }
+}
+
 // ND2: two densities (rho_d = sum_k c_{k,d} B_k, E = fc sum_d F(rho_d) + e_hc,
 // src/forces.F90:976-1089): a second weight per record, a second gradient array in LDS
 template <typename T, bool ND2>
-__global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, const cplx<T>* __restrict__ Zin,
-                                                cplx<T>* __restrict__ Wout,
+__global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int atom_base,
+                                                const cplx<T>* __restrict__ Zin,
+                                                cplx<T>* __restrict__ Wout, char* __restrict__ Pt,
                                                 const double* __restrict__ ehc,
                                                 const double* __restrict__ rho1,
                                                 double* __restrict__ eout, double* __restrict__ dEout,
@@ -678,20 +726,26 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, con
     // w * Z[a] Z[b] Z[c] to a single target x in registers (four 16-byte records per step,
     // loads issued together; the 64 lanes of a wave read consecutive records).  Thread t
     // takes chunks t + kAdjointThreads j; the chunk -> thread mapping must match the record
-    // layout built by build_terms.  While the thread's next chunk has the same target the sum
-    // stays in registers; it goes to LDS with one atomic when the target changes (a plain
-    // store when the chunk is its target's only one).  Each block starts at a different j
-    // (rotated by its index), so that the blocks of an XCD stream different parts of the
-    // record table at any moment instead of all reading the same L2 lines.
-    // rho = Re sum over A0 targets of Z[x] * dRho/dZ[x] (every monomial has exactly one A0
-    // factor), summed in FP64 once per flush.
-    double rho = 0.0, rho2 = 0.0;
+    // layout built by build_terms.  While the thread's next chunk (round j + 1) has the same
+    // target the sum stays in registers; at the end of such a run it is stored: into sW[x]
+    // when the chunk is its target's only one, otherwise into the run's slot of the atom's
+    // partial sums in global memory, which the pass after the walk adds up per target in slot
+    // order.  No atomics: the evaluation is bitwise repeatable.  The walk starts at round rot,
+    // a function of the atom's index in the whole system, so that the blocks of an XCD stream
+    // different parts of the record table at any moment instead of all reading the same L2
+    // lines; a run that crosses from round rot - 1 into round rot is met in two parts, and the
+    // first part's sum is held in registers until the walk ends with the second.
+    cplx<T>* part = (cplx<T>*)(Pt + (size_t)atom * partials_bytes(P.nslot, P.nd));
+    cplx<T>* part2 = part + P.nslot;
     const int nthr = (int)blockDim.x;
     const int nj = (P.nchunks + nthr - 1) / nthr;
-    const int rot = nj > 0 ? (int)(blockIdx.x % (unsigned)nj) : 0;
-    cplx<T> acc = {0, 0}, acc2 = {0, 0};
+    const int rot = nj > 0 ? (int)((unsigned)(atom_base + atom) % (unsigned)nj) : 0;
+    cplx<T> acc = {0, 0}, acc2 = {0, 0}, head = {0, 0}, head2 = {0, 0};
     int j = rot;
     int2 ch = tid + j * nthr < P.nchunks ? P.chunks[tid + j * nthr] : make_int2(0, 0);
+    bool split = false;
+    if (rot > 0 && tid + rot * nthr < P.nchunks) split = ((P.chunks[tid + (rot - 1) * nthr].y ^ ch.y) & 0xfff) == 0;
+    bool hold = split;
     for (int j0 = 0; j0 < nj; ++j0) {
         const int c = tid + j * nthr;
         const int jn = j + 1 < nj ? j + 1 : 0;
@@ -699,8 +753,8 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, con
         const bool has_next = j0 + 1 < nj && cn < P.nchunks;
         const int2 chn = has_next ? P.chunks[cn] : make_int2(0, 0);
         if (c < P.nchunks) {
-            const int x = ch.y & 0xffff, cnt = (ch.y >> 16) & 63;
-            const bool sole = (ch.y >> 22) & 1;  // the target's only chunk
+            const int x = ch.y & 0xfff, cnt = (ch.y >> 12) & 63;
+            const bool sole = (ch.y >> 18) & 1;  // the target's only chunk
             const uint4* __restrict__ rec = P.crec + ch.x;  // records of this lane, stride 64
             const T* __restrict__ w2p = ND2 ? P.cw2 + ch.x : nullptr;
             for (int k = 0; k < cnt; k += 4) {
@@ -727,20 +781,25 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, con
                     }
                 }
             }
-            if (!(has_next && (chn.y & 0xffff) == x)) {
-                if (x >= P.oA0) {
-                    rho += (double)re_mul(sZ[x], acc);
-                    if (ND2) rho2 += (double)re_mul(sZ[x], acc2);
-                }
-                if (sole) {  // nothing else contributes to x: a plain store
-                    sW[x] = acc;
-                    if (ND2) sW2[x] = acc2;
+            if (!(jn == j + 1 && has_next && ((chn.y ^ ch.y) & 0xfff) == 0)) {
+                if (hold) {  // first part of the run split by the start of the walk
+                    head = acc;
+                    head2 = acc2;
+                    hold = false;
                 } else {
-                    lds_add(&sW[x].re, acc.re);
-                    lds_add(&sW[x].im, acc.im);
-                    if (ND2) {
-                        lds_add(&sW2[x].re, acc2.re);
-                        lds_add(&sW2[x].im, acc2.im);
+                    if (split && j0 + 1 == nj) {  // its second part: the whole run
+                        acc.re += head.re;
+                        acc.im += head.im;
+                        acc2.re += head2.re;
+                        acc2.im += head2.im;
+                    }
+                    if (sole) {
+                        sW[x] = acc;
+                        if (ND2) sW2[x] = acc2;
+                    } else {
+                        const int slot = (int)((unsigned)ch.y >> 19);
+                        part[slot] = acc;
+                        if (ND2) part2[slot] = acc2;
                     }
                 }
                 acc = {0, 0};
@@ -749,6 +808,20 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, con
         }
         ch = chn;
         j = jn;
+    }
+    __syncthreads();  // every run stored: sole targets in sW, the others in their slots
+    combine_partials(P, part, sW);
+    if (ND2) {
+        __syncthreads();
+        combine_partials(P, part2, sW2);
+    }
+    __syncthreads();
+    // rho = Re sum over A0 targets of Z[x] dRho/dZ[x] (every monomial has exactly one A0
+    // factor), summed in FP64 in a fixed order
+    double rho = 0.0, rho2 = 0.0;
+    for (int x = P.oA0 + tid; x < P.nz; x += nthr) {
+        rho += (double)re_mul(sZ[x], sW[x]);
+        if (ND2) rho2 += (double)re_mul(sZ[x], sW2[x]);
     }
     const double tot = block_sum<T>(rho, red);
     double tot2 = 0.0;
@@ -789,9 +862,10 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, con
 // each record (byte offsets of 16-byte entries, FP32 weight) is loaded and decoded once for
 // both atoms.  Same chunk walk, carry, sole stores and rotation as k_adjoint; an odd last
 // atom is paired with itself and its second copy is not written.
-__global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<float> P, const uint4* __restrict__ crec16,
+__global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<float> P, int atom_base,
+                                                     const uint4* __restrict__ crec16,
                                                      const cplx<float>* __restrict__ Zin,
-                                                     cplx<float>* __restrict__ Wout,
+                                                     cplx<float>* __restrict__ Wout, char* __restrict__ Pt,
                                                      const double* __restrict__ ehc,
                                                      const double* __restrict__ rho1, double* __restrict__ eout,
                                                      double* __restrict__ dEout, double* __restrict__ edfout,
@@ -816,13 +890,17 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<floa
         sW[P.nz] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     __syncthreads();
-    double rho0 = 0.0, rho1s = 0.0;
+    // the walk, runs, slots and rotation of k_adjoint; a slot holds both atoms' sums
+    float4* part = (float4*)(Pt + (size_t)blockIdx.x * partials_bytes(P.nslot, 1));
     const int nthr = (int)blockDim.x;
     const int nj = (P.nchunks + nthr - 1) / nthr;
-    const int rot = nj > 0 ? (int)(blockIdx.x % (unsigned)nj) : 0;
-    cplx<T> acc0 = {0, 0}, acc1 = {0, 0};
+    const int rot = nj > 0 ? (int)((unsigned)((atom_base >> 1) + (int)blockIdx.x) % (unsigned)nj) : 0;
+    cplx<T> acc0 = {0, 0}, acc1 = {0, 0}, head0 = {0, 0}, head1 = {0, 0};
     int j = rot;
     int2 ch = tid + j * nthr < P.nchunks ? P.chunks[tid + j * nthr] : make_int2(0, 0);
+    bool split = false;
+    if (rot > 0 && tid + rot * nthr < P.nchunks) split = ((P.chunks[tid + (rot - 1) * nthr].y ^ ch.y) & 0xfff) == 0;
+    bool hold = split;
     const char* zb = (const char*)sZ;
     for (int j0 = 0; j0 < nj; ++j0) {
         const int c = tid + j * nthr;
@@ -831,8 +909,8 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<floa
         const bool has_next = j0 + 1 < nj && cn < P.nchunks;
         const int2 chn = has_next ? P.chunks[cn] : make_int2(0, 0);
         if (c < P.nchunks) {
-            const int x = ch.y & 0xffff, cnt = (ch.y >> 16) & 63;
-            const bool sole = (ch.y >> 22) & 1;
+            const int x = ch.y & 0xfff, cnt = (ch.y >> 12) & 63;
+            const bool sole = (ch.y >> 18) & 1;
             const uint4* __restrict__ rec = crec16 + ch.x;
             for (int k = 0; k < cnt; k += 4) {
                 uint4 r[4];
@@ -852,20 +930,23 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<floa
                     acc1.im = fma(w, p1.im, acc1.im);
                 }
             }
-            if (!(has_next && (chn.y & 0xffff) == x)) {
-                if (x >= P.oA0) {
-                    const float4 z = sZ[x];
-                    rho0 += (double)re_mul(cplx<T>{z.x, z.y}, acc0);
-                    rho1s += (double)re_mul(cplx<T>{z.z, z.w}, acc1);
-                }
-                if (sole) {
-                    sW[x] = make_float4(acc0.re, acc0.im, acc1.re, acc1.im);
+            if (!(jn == j + 1 && has_next && ((chn.y ^ ch.y) & 0xfff) == 0)) {
+                if (hold) {
+                    head0 = acc0;
+                    head1 = acc1;
+                    hold = false;
                 } else {
-                    float* wx = (float*)&sW[x];
-                    lds_add(wx, acc0.re);
-                    lds_add(wx + 1, acc0.im);
-                    lds_add(wx + 2, acc1.re);
-                    lds_add(wx + 3, acc1.im);
+                    if (split && j0 + 1 == nj) {
+                        acc0.re += head0.re;
+                        acc0.im += head0.im;
+                        acc1.re += head1.re;
+                        acc1.im += head1.im;
+                    }
+                    const float4 v = make_float4(acc0.re, acc0.im, acc1.re, acc1.im);
+                    if (sole)
+                        sW[x] = v;
+                    else
+                        part[(unsigned)ch.y >> 19] = v;
                 }
                 acc0 = {0, 0};
                 acc1 = {0, 0};
@@ -873,6 +954,15 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<floa
         }
         ch = chn;
         j = jn;
+    }
+    __syncthreads();
+    combine_partials(P, part, sW);
+    __syncthreads();
+    double rho0 = 0.0, rho1s = 0.0;
+    for (int x = P.oA0 + tid; x < P.nz; x += nthr) {
+        const float4 z = sZ[x], w = sW[x];
+        rho0 += (double)re_mul(cplx<T>{z.x, z.y}, cplx<T>{w.x, w.y});
+        rho1s += (double)re_mul(cplx<T>{z.z, z.w}, cplx<T>{w.z, w.w});
     }
     const double tot0 = block_sum<T>(rho0, red);
     __syncthreads();
@@ -1811,7 +1901,7 @@ int supported_nm3_class(int nm3, int nr2m, int nb_a, int l3a, int l2m, int nr3a)
 }
 
 template <typename T, int L, int CH, bool SPLIT, bool TASKSPLIT, int NB, int NM3X, int NB2, int NR2X, int NRAX, int NBAX>
-static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, const double* dlist,
+static hipError_t run_class(const DevParams<T>& P, int natom, int atom_base, const int* off, const double* dlist,
                             const double* elist, const double* dmomi, const double* emomi,
                             const double* dmom, const double* emom, double* eout, double* flist,
                             double* tlist, double* tlist0, Buffers& B, hipStream_t s, hipEvent_t* ev) {
@@ -1880,17 +1970,17 @@ static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, co
     if (ev) (void)hipEventRecord(ev[1], s);
     if constexpr (sizeof(T) == 4) {
         if (pair)
-            hipLaunchKernelGGL(k_adjoint_pair, dim3((natom + 1) / 2), dim3(kAdjointThreads), lds2p, s, P, P.crec16,
-                               (const cplx<T>*)B.Z, (cplx<T>*)B.W, B.ehc, B.rho1, eout, B.dE, B.edf, B.rho,
-                               natom);
+            hipLaunchKernelGGL(k_adjoint_pair, dim3((natom + 1) / 2), dim3(kAdjointThreads), lds2p, s, P, atom_base,
+                               P.crec16, (const cplx<T>*)B.Z, (cplx<T>*)B.W, (char*)B.Pt, B.ehc, B.rho1, eout, B.dE,
+                               B.edf, B.rho, natom);
     }
     if (pair) {
     } else if (nd2)
-        hipLaunchKernelGGL((k_adjoint<T, true>), dim3(natom), dim3(kAdjointThreads), lds2, s, P, (const cplx<T>*)B.Z,
-                           (cplx<T>*)B.W, B.ehc, B.rho1, eout, B.dE, B.edf, B.rho);
+        hipLaunchKernelGGL((k_adjoint<T, true>), dim3(natom), dim3(kAdjointThreads), lds2, s, P, atom_base,
+                           (const cplx<T>*)B.Z, (cplx<T>*)B.W, (char*)B.Pt, B.ehc, B.rho1, eout, B.dE, B.edf, B.rho);
     else
-        hipLaunchKernelGGL((k_adjoint<T, false>), dim3(natom), dim3(kAdjointThreads), lds2, s, P, (const cplx<T>*)B.Z,
-                           (cplx<T>*)B.W, B.ehc, B.rho1, eout, B.dE, B.edf, B.rho);
+        hipLaunchKernelGGL((k_adjoint<T, false>), dim3(natom), dim3(kAdjointThreads), lds2, s, P, atom_base,
+                           (const cplx<T>*)B.Z, (cplx<T>*)B.W, (char*)B.Pt, B.ehc, B.rho1, eout, B.dE, B.edf, B.rho);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(k3, dim3(natom), dim3(64), lds3, s, P, off, dlist, elist, dmomi, emomi, dmom, emom,
@@ -1900,7 +1990,7 @@ static hipError_t run_class(const DevParams<T>& P, int natom, const int* off, co
 }
 
 template <typename T>
-hipError_t Launch<T>::run(const DevParams<T>& P, int natom, const int* off, const double* dlist,
+hipError_t Launch<T>::run(const DevParams<T>& P, int natom, int atom_base, const int* off, const double* dlist,
                           const double* elist, const double* dmomi, const double* emomi,
                           const double* dmom, const double* emom, double* eout, double* flist,
                           double* tlist, double* tlist0, Buffers& B, hipStream_t s, int cls,
@@ -1914,10 +2004,12 @@ hipError_t Launch<T>::run(const DevParams<T>& P, int natom, const int* off, cons
     // 0.93 -> 0.81 ms; with lmax 6 the runtime split costs FP64 3%).
     const bool heavy = P.nb_a + P.nr3a * (P.l3a + 1) >= 40;
 #define MAGACE_RUN(CH, SPLIT, TS)                                                                          \
-    (cls == 0 ? run_class<T, 6, CH, SPLIT, TS, 4, 4, 4, 2, 8, 10>(P, natom, off, dlist, elist, dmomi, emomi,  \
-                                                                  dmom, emom, eout, flist, tlist, tlist0, B, s, ev) \
-              : run_class<T, 2, CH, SPLIT, TS, 1, 27, 2, 5, 4, 16>(P, natom, off, dlist, elist, dmomi, emomi,  \
-                                                                   dmom, emom, eout, flist, tlist, tlist0, B, s, ev))
+    (cls == 0 ? run_class<T, 6, CH, SPLIT, TS, 4, 4, 4, 2, 8, 10>(P, natom, atom_base, off, dlist, elist, dmomi,  \
+                                                                  emomi, dmom, emom, eout, flist, tlist, tlist0, B, \
+                                                                  s, ev)                                            \
+              : run_class<T, 2, CH, SPLIT, TS, 1, 27, 2, 5, 4, 16>(P, natom, atom_base, off, dlist, elist, dmomi, \
+                                                                   emomi, dmom, emom, eout, flist, tlist, tlist0, B, \
+                                                                   s, ev))
     if (heavy) return MAGACE_RUN(32, true, false);
     if (sizeof(T) == 8) return MAGACE_RUN(32, true, true);
     return MAGACE_RUN(64, false, true);

@@ -15,6 +15,8 @@ struct cplx {
 // threads of an adjoint workgroup; chunk c of the gradient records runs on thread
 // c % kAdjointThreads, and the chunk order built with the records assumes this count
 constexpr int kAdjointThreads = 512;
+// slots per group of the adjoint's first combine pass (TermList::kGroup)
+constexpr int kCombGroup = 8;
 
 // Everything a kernel needs about the potential, passed by value as a kernel argument.
 template <typename T>
@@ -32,8 +34,14 @@ struct DevParams {
     const uint4* crec = nullptr;
     const uint4* crec16 = nullptr;   // FP32 only: the records with offsets of 16-byte entries (two
                                      // atoms' Z interleaved, k_adjoint_pair)
-    const int2* chunks = nullptr;    // (start, target | count << 16 | sole << 22), count = its group's length;
-                                     // chunk c runs on thread c % kAdjointThreads
+    const int2* chunks = nullptr;    // (start, target | count << 12 | sole << 18 | slot << 19), count = its
+                                     // group's length; chunk c runs on thread c % kAdjointThreads
+    // partial-sum combine (TermList::comb, comb2): groups of <= kCombGroup slots
+    // (first slot, target | slots << 12 | only group << 16) and targets with several groups
+    // (first slot, target | groups << 12)
+    const int2* comb = nullptr;
+    const int2* comb2 = nullptr;
+    int ncomb = 0, ncomb2 = 0, nslot = 0;  // groups, such targets, partial-sum slots per atom and density
     const uint32_t* fold = nullptr;  // conjugate-pair folding of W (see TermList::fold)
     int nchunks = 0;
     int nfemb = 0;
@@ -59,17 +67,24 @@ struct Buffers {
     double* dE = nullptr;    // fcut * F'(rho), [natom][nd]
     double* edf = nullptr;   // d f_in/d e_hc * F + 1
     double* rho = nullptr;   // rho_i (including the B1 part), [natom][nd]
+    void* Pt = nullptr;      // adjoint partial sums, [natom][nd][nslot] x 16 bytes (partials_bytes)
     int* err = nullptr;
     size_t cap_atoms = 0;
 };
 
+// bytes of Buffers::Pt per atom
+__host__ __device__ inline size_t partials_bytes(int nslot, int nd) { return (size_t)nslot * 16 * (size_t)nd; }
+
+// atom_base: the index of the launch's first atom in the whole system (even), which fixes the
+// adjoint's walk per atom, so that results do not depend on how the atoms are split into
+// launches
 template <typename T>
 struct Launch {
-    static hipError_t run(const DevParams<T>& P, int natom, const int* offsets, const double* dlist,
-                          const double* elist, const double* dmomi, const double* emomi,
-                          const double* dmom, const double* emom, double* eout, double* flist,
-                          double* tlist, double* tlist0, Buffers& B, hipStream_t s, int nm3_class,
-                          hipEvent_t* ev);
+    static hipError_t run(const DevParams<T>& P, int natom, int atom_base, const int* offsets,
+                          const double* dlist, const double* elist, const double* dmomi,
+                          const double* emomi, const double* dmom, const double* emom, double* eout,
+                          double* flist, double* tlist, double* tlist0, Buffers& B, hipStream_t s,
+                          int nm3_class, hipEvent_t* ev);
 };
 
 // fitting (FP64): forward-mode density, per-atom adjoint scalars, monomial accumulation

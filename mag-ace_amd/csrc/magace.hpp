@@ -74,10 +74,24 @@ struct TermList {
     std::vector<double> w;       // merged weights, density-major: w[d * nterms + t]
     std::vector<uint16_t> cidx;  // 4 per contribution: the three other factors + padding
     std::vector<double> cw;      // weight per contribution, density-major (d * ncontrib + r)
-    std::vector<int32_t> chunks; // 2 per chunk: first record, target | (count << 16) | (sole << 22),
-                                 // count = the length of its group of 64 chunks (padded with
-                                 // zero weights), sole = the target's only chunk; records of a
+    std::vector<int32_t> chunks; // 2 per chunk: first record, target | (count << 12) | (sole << 18)
+                                 // | (slot << 19), count = the length of its group of 64 chunks
+                                 // (padded with zero weights), sole = the target's only chunk,
+                                 // slot = the partial-sum slot of its run (below); records of a
                                  // chunk are 64 apart (wave-interleaved layout)
+    // Runs: chunk c runs on lane c % kLanes in round c / kLanes; a run is a maximal sequence of
+    // a lane's chunks in consecutive rounds with one target.  The adjoint flushes one sum per
+    // run: a sole chunk's sum is its target's gradient and is stored; the other runs store
+    // their sums into slots, numbered so that a target's slots are consecutive.  Two passes
+    // then add them up in a fixed order (the evaluation is bitwise repeatable): the first sums
+    // groups of up to kGroup consecutive slots of a target (loads issued together) and writes
+    // the result into the group's first slot, or straight into the gradient when the target has
+    // one group; the second sums the group results of targets with several groups.
+    // comb: 2 per group: first slot, target | (slots << 12) | (the target's only group << 16);
+    // comb2: 2 per target with several groups: first slot, target | (groups << 12).
+    static constexpr int kGroup = 8;
+    std::vector<int32_t> comb, comb2;
+    int nslot = 0;
     // folding of the gradient onto representatives of conjugate pairs (x, xbar):
     // partner | sign << 16 (1 = odd m+m') | representative << 17 | self-conjugate << 18
     std::vector<uint32_t> fold;
@@ -117,6 +131,14 @@ struct Model {
 // api.cpp: the library's last-error state (magace_last_error / magace_error_string)
 void report_error(int code, const std::string& msg);
 void reset_error();
+// api.cpp: a counter that changes whenever state baked into a captured evaluation changes
+// (a new or finalised potential, reallocated per-atom buffers, the precision mode, new
+// coefficients); a graph captured at one value must not be replayed at another
+unsigned long long context_generation();
+// api.cpp: uploads the potential's tables and sizes the per-atom buffers for natom atoms
+// (no evaluation), so that a following capture records no allocation; MAGACE_ERR_STATE
+// without a potential
+int context_prepare(int natom);
 // potential.cpp
 bool parse_potential(const std::string& path, const std::string& element, int xc, double smear,
                      Potential& pot, std::string& err);

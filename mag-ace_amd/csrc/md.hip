@@ -21,9 +21,11 @@
 // Neighbour lists are rebuilt every `rebuild_every` steps with cutoff r_list = r_c + skin
 // (the build records each pair's lattice shift, so the pair vectors are recomputed from the
 // moved positions in between; pairs beyond r_c contribute exact zeros).  The steps between
-// two rebuilds are captured once into a HIP graph and replayed; the graph stays valid as
-// long as the pair buffers do not have to grow.  Force and field assembly is a gather over
-// each atom's own pairs and their reverse partners, so results are deterministic.
+// two rebuilds are captured once into a HIP graph and replayed.  The graph bakes in the pair
+// buffers and the evaluator's state at capture time (its per-atom buffers, tables, precision
+// and coefficients), so it is captured again when the pair buffers grow or when that state
+// changed since (context_generation).  Force and field assembly is a gather over each atom's
+// own pairs and their reverse partners, so results are deterministic.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -168,6 +170,50 @@ __global__ void k_md_kick(int n, double dt, double inv_m, double* __restrict__ v
     v[i] += 0.5 * dt * inv_m * f[i];
 }
 
+// Energy and virial of the last evaluation, FP64, in a fixed summation order (bitwise
+// repeatable): E = sum_i E_i and W_ab = sum_p u_a F_b with u = x_i - x_j = r e_p, the tally
+// of lammps/pair_spin_ace.cpp:635-636 (ev_tally_xyz with the pair's F_p and u_p).  Block b
+// sums a fixed strided slice of atoms and pairs and reduces through LDS in a fixed tree;
+// k_md_tally_final sums the block partials in order.
+constexpr int kTallyBlocks = 512, kTallyThreads = 256, kTally = 10;  // E, W_xx W_xy ... W_zz
+
+__global__ void __launch_bounds__(kTallyThreads) k_md_tally(int n, const int* __restrict__ off,
+                                                            const double* __restrict__ eout,
+                                                            const double* __restrict__ dl,
+                                                            const double* __restrict__ el,
+                                                            const double* __restrict__ fl,
+                                                            double* __restrict__ part) {
+    __shared__ double red[kTally][kTallyThreads];
+    const int t = threadIdx.x, stride = kTallyBlocks * kTallyThreads;
+    const int g = blockIdx.x * kTallyThreads + t;
+    double v[kTally] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = g; i < n; i += stride) v[0] += eout[i];
+    const int np = off[n];
+    for (int p = g; p < np; p += stride) {
+        const double r = dl[p];
+        const double u[3] = {r * el[3 * (size_t)p], r * el[3 * (size_t)p + 1], r * el[3 * (size_t)p + 2]};
+        const double F[3] = {fl[3 * (size_t)p], fl[3 * (size_t)p + 1], fl[3 * (size_t)p + 2]};
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) v[1 + 3 * a + b] += u[a] * F[b];
+    }
+    for (int k = 0; k < kTally; ++k) red[k][t] = v[k];
+    __syncthreads();
+    for (int h = kTallyThreads / 2; h > 0; h >>= 1) {
+        if (t < h)
+            for (int k = 0; k < kTally; ++k) red[k][t] += red[k][t + h];
+        __syncthreads();
+    }
+    if (t < kTally) part[(size_t)blockIdx.x * kTally + t] = red[t][0];
+}
+
+__global__ void k_md_tally_final(const double* __restrict__ part, double* __restrict__ out) {
+    const int k = threadIdx.x;
+    if (k >= kTally) return;
+    double s = 0.0;
+    for (int b = 0; b < kTallyBlocks; ++b) s += part[(size_t)b * kTally + k];
+    out[k] = s;
+}
+
 struct Md {
     int n = 0;
     Cell cell{};
@@ -183,8 +229,10 @@ struct Md {
     int npairs = 0;
     int *nl = nullptr, *sh = nullptr, *owner = nullptr, *rev = nullptr;
     double *dl = nullptr, *el = nullptr, *dm = nullptr, *em = nullptr, *fl = nullptr, *tl = nullptr;
+    double* tally = nullptr;  // [kTallyBlocks][kTally] block partials, then [kTally] totals
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    unsigned long long gen = 0;  // context_generation() the graph was captured at
     long long step = 0;
     int rebuilds = 0, captures = 0;
     ~Md();
@@ -206,7 +254,7 @@ void drop_graph(Md& M) {
 Md::~Md() {
     drop_graph(*this);
     dfree(x); dfree(v); dfree(s); dfree(mu); dfree(m); dfree(f); dfree(G); dfree(Gprev); dfree(eout); dfree(t0);
-    dfree(dmi); dfree(emi); dfree(off);
+    dfree(dmi); dfree(emi); dfree(off); dfree(tally);
     dfree(nl); dfree(sh); dfree(owner); dfree(rev); dfree(dl); dfree(el); dfree(dm); dfree(em); dfree(fl); dfree(tl);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -290,6 +338,10 @@ int one_step(Md& M) {
 
 int capture(Md& M) {
     drop_graph(M);
+    // tables uploaded and per-atom buffers sized before the capture, so that no allocation
+    // is recorded; an error here (no potential) is returned before capturing
+    if (int rc = context_prepare(M.n)) return rc;
+    const unsigned long long gen = context_generation();
     MDCHK(hipStreamBeginCapture(M.stream, hipStreamCaptureModeRelaxed));
     int rc = 0;
     for (int k = 0; k < M.every && !rc; ++k) rc = one_step(M);
@@ -302,6 +354,7 @@ int capture(Md& M) {
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
     M.graph = g;
     MDCHK(hipGraphInstantiate(&M.exec, M.graph, nullptr, nullptr, 0));
+    M.gen = gen;
     ++M.captures;
     return 0;
 }
@@ -336,6 +389,7 @@ int magace_md_create(int natom, const double* cell, double rlist, double mass, d
     double** d1[] = {&M->mu, &M->eout, &M->dmi};
     for (double** p : d1) MDCHK(hipMalloc((void**)p, n * sizeof(double)));
     MDCHK(hipMalloc((void**)&M->off, (n + 1) * sizeof(int)));
+    MDCHK(hipMalloc((void**)&M->tally, (kTallyBlocks + 1) * kTally * sizeof(double)));
     std::vector<double> sd(3 * n);
     for (size_t i = 0; i < n; ++i) {
         const double a = sdir[3 * i], b = sdir[3 * i + 1], c = sdir[3 * i + 2];
@@ -374,6 +428,7 @@ int magace_md_run(void* handle, int nsteps, int use_graph) {
         const int left_in_block = M.every - (int)(M.step % M.every);
         const int k = std::min(left_in_block, nsteps - done);
         if (use_graph && k == M.every) {
+            if (M.exec && M.gen != context_generation()) drop_graph(M);  // stale evaluator state
             if (!M.exec)
                 if (int rc = capture(M)) return rc;
             MDCHK(hipGraphLaunch(M.exec, M.stream));
@@ -388,13 +443,19 @@ int magace_md_run(void* handle, int nsteps, int use_graph) {
 }
 
 int magace_md_read(void* handle, double* x, double* v, double* sdir, double* f, double* dEdm, double* eatom,
-                   double* epot, double* ekin, long long* counters) {
+                   double* epot, double* ekin, double* virial, long long* counters) {
     reset_error();
     if (!handle) {
         report_error(MAGACE_ERR_INPUT, "magace_md_read: null handle");
         return MAGACE_ERR_INPUT;
     }
     Md& M = *(Md*)handle;
+    // energy and virial of the current state, reduced on the device in FP64
+    k_md_tally<<<kTallyBlocks, kTallyThreads, 0, M.stream>>>(M.n, M.off, M.eout, M.dl, M.el, M.fl, M.tally);
+    k_md_tally_final<<<1, 64, 0, M.stream>>>(M.tally, M.tally + (size_t)kTallyBlocks * kTally);
+    MDCHK(hipGetLastError());
+    double tot[kTally];
+    MDCHK(hipMemcpyAsync(tot, M.tally + (size_t)kTallyBlocks * kTally, sizeof tot, hipMemcpyDeviceToHost, M.stream));
     MDCHK(hipStreamSynchronize(M.stream));
     const size_t n = (size_t)M.n;
     std::vector<double> hv(3 * n), he(n);
@@ -406,10 +467,11 @@ int magace_md_read(void* handle, double* x, double* v, double* sdir, double* f, 
     if (f) MDCHK(hipMemcpy(f, M.f, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
     if (dEdm) MDCHK(hipMemcpy(dEdm, M.G, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
     if (eatom) std::copy(he.begin(), he.end(), eatom);
-    double ep = 0, ek = 0;
-    for (size_t i = 0; i < n; ++i) ep += he[i];
+    double ek = 0;
     for (size_t i = 0; i < 3 * n; ++i) ek += hv[i] * hv[i];
-    if (epot) *epot = ep;
+    if (epot) *epot = tot[0];
+    if (virial)
+        for (int k = 0; k < 9; ++k) virial[k] = tot[1 + k];
     if (ekin) *ekin = 0.5 * M.mass * kMvv2e * ek;
     if (counters) {
         counters[0] = M.step;

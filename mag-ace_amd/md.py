@@ -43,15 +43,16 @@ class SpinLatticeMD:
         n = self.n
         x, v, s, f, g = (np.zeros(3 * n) for _ in range(5))
         e = np.zeros(n)
+        vir = np.zeros(9)
         ep, ek = C.c_double(), C.c_double()
         cnt = np.zeros(4, np.int64)
         rc = capi.lib().magace_md_read(self.h, x.ctypes.data, v.ctypes.data, s.ctypes.data, f.ctypes.data,
                                        g.ctypes.data, e.ctypes.data, C.addressof(ep), C.addressof(ek),
-                                       cnt.ctypes.data)
+                                       vir.ctypes.data, cnt.ctypes.data)
         capi._check(rc)
         return dict(x=x.reshape(n, 3), v=v.reshape(n, 3), sdir=s.reshape(n, 3), f=f.reshape(n, 3),
-                    dEdm=g.reshape(n, 3), eatom=e, epot=ep.value, ekin=ek.value, steps=int(cnt[0]),
-                    rebuilds=int(cnt[1]), captures=int(cnt[2]), pairs=int(cnt[3]))
+                    dEdm=g.reshape(n, 3), eatom=e, epot=ep.value, ekin=ek.value, virial=vir.reshape(3, 3),
+                    steps=int(cnt[0]), rebuilds=int(cnt[1]), captures=int(cnt[2]), pairs=int(cnt[3]))
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:

@@ -45,11 +45,23 @@ def _f(x: float) -> str:
 
 
 def render(header: dict, basis: dict, seed: int, element: str = "Fe", xc: str = "pbe",
-           smear: float = 0.2, c_scale: float = 1e-2) -> str:
+           smear: float = 0.2, c_scale: float = 1e-2, values: dict | None = None) -> str:
     """Text of a potential file.  ``basis[k]`` = (nvar_k, width) int array of index tuples
-    in the enumeration order of the reference (b1index/b2index/b3index/b4index columns)."""
+    in the enumeration order of the reference (b1index/b2index/b3index/b4index columns).
+
+    ``values`` (one density only) replaces the random draws with given parameters:
+    ``crad_a`` [nr3_a][l3a+1][nb_a], ``crad_m`` [nr2_m][l2m+1][nb_m] and ``c`` {order: array
+    in enumeration order}; the header's p1a/p1b/core_pre/core_lambda give par and core."""
     h = dict(BASE, **header)
     rng = np.random.default_rng(seed)
+    if values is not None:
+        if int(h.get("ndensity", 1)) != 1:
+            raise ValueError("explicit values are supported for one density")
+        ca, cm = np.asarray(values["crad_a"]), np.asarray(values["crad_m"])
+        cv = {k: np.asarray(values["c"][k]).ravel() for k in (1, 2, 3, 4)}
+        for k in (1, 2, 3, 4):
+            if len(cv[k]) != len(np.asarray(basis.get(k, np.zeros((0, 1))))):
+                raise ValueError("coefficient count of order %d does not match the basis" % k)
     L = []
     L.append(" Parameters (last)")
     L.append(" created        2026           1           1           0           0           0")
@@ -76,22 +88,22 @@ def render(header: dict, basis: dict, seed: int, element: str = "Fe", xc: str = 
     for nr in range(1, h["nr3_a"] + 1):
         for l in range(0, h["l3a"] + 1):
             for nb in range(1, nb_a + 1):
+                v = ca[nr - 1, l, nb - 1] if values is not None else rng.normal() / np.sqrt(nb_a)
                 L.append(" rad_atomic 0 0")
-                L.append(" %s %s %d %d %d %s T" % (element, element, nb, nr, l,
-                                                   _f(rng.normal() / np.sqrt(nb_a))))
+                L.append(" %s %s %d %d %d %s T" % (element, element, nb, nr, l, _f(v)))
     for nr in range(1, h["nr2_m"] + 1):
         for l in range(0, h["l2m"] + 1):
             for nb in range(1, nb_m + 1):
+                v = cm[nr - 1, l, nb - 1] if values is not None else rng.normal() / np.sqrt(nb_m)
                 L.append(" rad_magnetic 0 0")
-                L.append(" %s %s %d %d %d %s T" % (element, element, nb, nr, l,
-                                                   _f(rng.normal() / np.sqrt(nb_m))))
+                L.append(" %s %s %d %d %d %s T" % (element, element, nb, nr, l, _f(v)))
     for order in (1, 2, 3, 4):
         idx = np.asarray(basis.get(order, np.zeros((0, 1), np.int32)))
         els = " ".join([element] * order)
-        for row in idx:
+        for k, row in enumerate(idx):
+            v = cv[order][k] if values is not None else rng.normal(0.0, c_scale)
             L.append(" clu %d 1" % order)
-            L.append(" %s %s %s T" % (els, " ".join(str(int(v)) for v in row),
-                                      _f(rng.normal(0.0, c_scale))))
+            L.append(" %s %s %s T" % (els, " ".join(str(int(x)) for x in row), _f(v)))
     # further densities (ndensity 2): their par and clu records follow, so that files with one
     # density are unchanged
     for nd in range(2, int(h.get("ndensity", 1)) + 1):

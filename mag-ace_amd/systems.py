@@ -162,11 +162,11 @@ def assemble(a, eout, flist, tlist, tlist0):
     F = flist.reshape(-1, 3)
     T = tlist.reshape(-1, 3)
     f = np.zeros((n, 3))
-    fm = np.zeros((n, 3))
-    np.add.at(f, a["pair_i"], -F)
-    np.add.at(f, a["pair_j"], F)
-    fm -= tlist0.reshape(-1, 3)
-    np.add.at(fm, a["pair_j"], -T)
+    fm = -np.asarray(tlist0, dtype=np.float64).reshape(-1, 3).copy()
+    for k in range(3):
+        f[:, k] += np.bincount(a["pair_j"], weights=F[:, k], minlength=n)
+        f[:, k] -= np.bincount(a["pair_i"], weights=F[:, k], minlength=n)
+        fm[:, k] -= np.bincount(a["pair_j"], weights=T[:, k], minlength=n)
     u = a["elist"].reshape(-1, 3) * a["dlist"][:, None]
     vir = u.T @ F
     return float(np.sum(eout)), f, fm, vir

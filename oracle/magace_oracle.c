@@ -1559,7 +1559,7 @@ int orc_bvectors(orc_pot *p, int natom, int nei, const int *nstart, const int *n
     double *tl = xcalloc(3 * (size_t)nei + 1, 8), *t0 = xcalloc(3 * (size_t)natom, 8);
     job J = {natom, nstart, nstop, dlist, elist, dlist_momi, elist_momi, dlist_mom, elist_mom,
              eout, fl, tl, t0, bout};
-    int rc = run(p, &J, 1);
+    int rc = run(p, &J, 0);  /* atoms write their own rows of bout: OpenMP over atoms */
     free(eout); free(fl); free(tl); free(t0);
     return rc;
 }

@@ -63,3 +63,34 @@ def per_atom_errors(a, got, ref):
     e1, f1, m1, _ = s.assemble(a, got["eout"], got["flist"], got["tlist"], got["tlist0"])
     e0, f0, m0, _ = s.assemble(a, ref["eout"], ref["flist"], ref["tlist"], ref["tlist0"])
     return (np.abs(got["eout"] - ref["eout"]).max(), np.abs(f1 - f0).max(), np.abs(m1 - m0).max())
+
+
+FLOAT_KEYS = ("lambda_a", "lambda_m", "cut_a", "cut_m", "dcut_a", "dcut_m", "ecut", "decut", "p1a", "p1b",
+              "core_pre", "core_lambda")
+
+
+def fe_example(tmpdir):
+    """The reference's fitted example potential (example/Fe.pbe.in), regenerated from the
+    parameters stored in golden/fe_example.npz (tests/golden/make_fe_example.py), and the
+    fixture itself.  Returns (path, fixture)."""
+    z = np.load(os.path.join(GOLDEN, "fe_example.npz"))
+    h = {str(k): (float(v) if str(k) in FLOAT_KEYS else int(round(v)))
+         for k, v in zip(z["header_keys"], z["header_vals"])}
+    basis = PKG.capi.enumerate_basis(h)
+    vals = dict(crad_a=z["crad_a"], crad_m=z["crad_m"], c={k: z["c%d" % k] for k in (1, 2, 3, 4)})
+    text = PKG.potgen.render(h, basis, 0, values=vals)
+    assert hashlib.sha256(text.encode()).hexdigest() == str(z["pot_sha256"]), "regenerated potential differs"
+    path = os.path.join(str(tmpdir), "Fe.pbe.in")
+    with open(path, "w") as f:
+        f.write(text)
+    return path, z
+
+
+FE_CASES = ("notebook", "nc16", "nc128")
+
+
+def fe_case(z, name, rcut=4.5):
+    """acelib_ arrays and reference outputs of one system of the example-potential fixture."""
+    a = PKG.systems.pack_acelib(z[name + "_pos"], z[name + "_cell"], z[name + "_sdir"], z[name + "_smag"], rcut)
+    ref = {k: z["%s_%s" % (name, k)] for k in ("eout", "flist", "tlist", "tlist0")}
+    return a, ref

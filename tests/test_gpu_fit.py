@@ -213,3 +213,88 @@ def test_split_halves_leave_the_fit_unchanged(tmp_path):
         scale = max(np.abs(b).max(), 1e-300)
         assert np.abs(a - b).max() <= 1e-12 * scale, what
 
+
+
+def test_backward_refused_after_another_evaluation(tmp_path):
+    """The backward pass reads the Z and W its forward left on the device.  Any other
+    evaluation in between (here a mixed-precision magace_compute_device on the same atoms)
+    overwrites them, so magace_fit_backward must refuse instead of returning a wrong dL/dc."""
+    say("fit: backward after an intervening evaluation")
+    load(tmp_path, "H500", 2045)
+    batch = fit.Batch(structures(2), 5.0)
+    E = torch.zeros(batch.nstruc, dtype=torch.float64, device=DEV)
+    F = torch.zeros(batch.natom, 3, dtype=torch.float64, device=DEV)
+    batch.forward(E, F)
+    L = batch.L
+    N, P = batch.natom, batch.npairs
+    outs = [torch.zeros(n, dtype=torch.float64, device=DEV) for n in (N, 3 * P, 3 * P, 3 * N)]
+    capi.set_precision(capi.MIXED)
+    try:
+        capi.compute_device(N, P, L["offsets"].data_ptr(), L["dlist"].data_ptr(), L["elist"].data_ptr(),
+                            L["dlist_momi"].data_ptr(), L["elist_momi"].data_ptr(), L["dlist_mom"].data_ptr(),
+                            L["elist_mom"].data_ptr(), *[o.data_ptr() for o in outs],
+                            stream=torch.cuda.current_stream().cuda_stream)
+    finally:
+        capi.set_precision(capi.FP64)
+    gF = torch.zeros(batch.natom, 3, dtype=torch.float64, device=DEV)
+    g = torch.zeros(capi.fit_ncoef(), dtype=torch.float64, device=DEV)
+    with pytest.raises(capi.MagaceError, match="must follow"):
+        batch.backward(None, gF, g)
+    # a new forward makes the backward valid again
+    batch.forward(E, F)
+    batch.backward(None, gF, g)
+
+
+def test_full_batch_loss_gradient(tmp_path):
+    """The batched-fit workload (256 configurations, lmax-4 header, linear embedding) with
+    L = sum_s (E_s - E_s^ref)^2 + sum_j |F_j - F_j^ref|^2 against FP64 targets from a teacher
+    potential (the same coefficients perturbed by 30%).  The energy part of dL/dc
+    (sum_s 2 dE_s sum_{i in s} B_i) equals the oracle's B vectors on all 256 configurations
+    (<= 1e-9 relative); the full gradient along a random direction equals a central
+    difference of L (L is quadratic in c for a linear embedding, so one central difference is
+    exact up to rounding; <= 1e-8 relative)."""
+    import oracle as O
+
+    say("fit: full 256-configuration batch, L = dE^2 + dF^2")
+    pot = load(tmp_path, "H500", 2029)
+    structs = systems.config_batch()
+    batch = fit.Batch(structs, 5.0)
+    c0 = capi.fit_get_coefficients()
+    rng = np.random.default_rng(2029)
+    c_t = c0 * (1.0 + 0.3 * rng.normal(size=len(c0)))
+    E_ref = torch.zeros(batch.nstruc, dtype=torch.float64, device=DEV)
+    F_ref = torch.zeros(batch.natom, 3, dtype=torch.float64, device=DEV)
+    capi.fit_set_coefficients(c_t)
+    batch.forward(E_ref, F_ref)
+    c = torch.tensor(c0, dtype=torch.float64, device=DEV, requires_grad=True)
+
+    def L(cc):
+        E, F = fit.EnergyForces.apply(cc, batch)
+        return ((E - E_ref) ** 2).sum() + ((F - F_ref) ** 2).sum(), E
+
+    l0, E0 = L(c)
+    l0.backward()
+    g = c.grad.detach().cpu().numpy()
+    say("  atoms %d pairs %d  L %.6e  |dL/dc| %.3e" % (batch.natom, batch.npairs, float(l0), np.linalg.norm(g)))
+    # energy part alone, against the oracle's per-atom B vectors
+    gE = (2.0 * (E0.detach() - E_ref)).contiguous()
+    gF = torch.zeros(batch.natom, 3, dtype=torch.float64, device=DEV)
+    ge = gradient(batch, c0, gE, gF)
+    o = O.Oracle.load(pot)
+    want = np.zeros_like(ge)
+    w = gE.cpu().numpy()
+    for s, ws in zip(structs, w):
+        a = nbr.neilist(s["pos"], s["sdir"] * s["smag"][:, None], s["cell"], 5.0)
+        want += ws * o.bvectors(a).sum(axis=0)
+    err = np.abs(ge - want).max() / np.abs(want).max()
+    say("  energy part vs oracle B vectors: rel err %.2e" % err)
+    assert err < 1e-9
+    d = rng.normal(size=len(c0))
+    h = 1e-3 * np.abs(c0).mean()
+    with torch.no_grad():
+        dd = torch.tensor(d, dtype=torch.float64, device=DEV)
+        fd = (float(L(c + h * dd)[0]) - float(L(c - h * dd)[0])) / (2 * h)
+    an = float(g @ d)
+    say("  dL/dc . d: analytic %.10e  central difference %.10e" % (an, fd))
+    assert abs(an - fd) <= 1e-8 * max(1e-12, abs(fd))
+    capi.fit_set_coefficients(c0)

@@ -129,3 +129,41 @@ def test_mixed_within_targets_at_full_size(setup):
     df, dg = np.abs(f32 - f64).max(), np.abs(g32 - g64).max()
     say("  max |dF| %.3e eV/A  max |d dE/dm| %.3e eV/muB  max |dE_i| %.3e eV" % (df, dg, np.abs(e32 - e64).max()))
     assert df < 1e-4 and dg < 1e-4
+
+
+def test_virial_matches_strain_differences(setup):
+    """The virial W_ab = sum_p u_a F_b (u = x_i - x_j, F = dE_i/du; the tally of
+    lammps/pair_spin_ace.cpp:635-636) from the GPU pair outputs equals the derivative of the
+    total energy under a homogeneous deformation u -> (1 + eps) u of every pair vector: central
+    differences with eps = +-1e-5 for all nine components (FP64, <= 1e-6 of max |W|).  The mixed
+    path's virial is within 1e-5 of the FP64 one (relative to max |W|)."""
+    pot, s = setup
+    say("full size: virial vs homogeneous-strain differences (16000 atoms)")
+    capi.init_file(pot)
+    capi.set_precision(capi.FP64)
+    a = nbr.neilist(s["pos"], s["sdir"] * s["smag"][:, None], s["cell"], 5.0)
+    out = capi.acelib(a)
+    _, _, _, vir = systems.assemble(a, out["eout"], out["flist"], out["tlist"], out["tlist0"])
+    u = a["elist"].reshape(-1, 3) * a["dlist"][:, None]
+    eps = 1e-5
+    scale = np.abs(vir).max()
+    for ia in range(3):
+        for ib in range(3):
+            e = []
+            for sg in (1, -1):
+                Fd = np.eye(3)
+                Fd[ia, ib] += sg * eps
+                up = u @ Fd.T
+                d = np.linalg.norm(up, axis=1)
+                b = dict(a, dlist=d, elist=(up / d[:, None]).ravel())
+                e.append(capi.acelib(b)["eout"].sum())
+            dE = (e[0] - e[1]) / (2 * eps)
+            say("  W[%d,%d] = %.8f   dE/d eps_%d%d = %.8f" % (ib, ia, vir[ib, ia], ia, ib, dE))
+            assert abs(dE - vir[ib, ia]) <= 1e-6 * max(1.0, scale)
+    capi.set_precision(capi.MIXED)
+    o32 = capi.acelib(a)
+    capi.set_precision(capi.FP64)
+    _, _, _, v32 = systems.assemble(a, o32["eout"], o32["flist"], o32["tlist"], o32["tlist0"])
+    err = np.abs(v32 - vir).max() / scale
+    say("  mixed virial rel err %.2e (max |W| %.3e eV)" % (err, scale))
+    assert err <= 1e-5

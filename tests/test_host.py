@@ -177,10 +177,28 @@ def test_gradient_records_equal_the_monomial_derivative(hname, tmp_path):
     F = Z[idx]
     for s in range(4):
         np.add.at(want, idx[:, s], w * np.prod(np.delete(F, s, axis=1), axis=1))
-    first, x, cnt, sole = ch[:, 0], ch[:, 1] & 0xFFFF, (ch[:, 1] >> 16) & 63, (ch[:, 1] >> 22) & 1
+    y = ch[:, 1].astype(np.int64) & 0xFFFFFFFF
+    first, x, cnt, sole, slot = ch[:, 0], y & 0xFFF, (y >> 12) & 63, (y >> 18) & 1, y >> 19
     assert np.all(x < nz) and np.all(cnt % 4 == 0) and np.all((cnt >= 4) & (cnt <= 32))
     nchunk_of = np.bincount(x, minlength=nz)
     assert np.array_equal(sole == 1, nchunk_of[x] == 1)  # flagged exactly when the target has one chunk
+    # runs (a lane's chunks in consecutive rounds with one target) of targets with several
+    # chunks: one slot per run, and a target's slots consecutive
+    L = 512
+    runs = {}
+    for c in range(len(ch)):
+        if sole[c]:
+            continue
+        cont = c >= L and x[c - L] == x[c]
+        if cont:
+            assert slot[c] == slot[c - L]
+        else:
+            assert slot[c] not in runs
+            runs[slot[c]] = x[c]
+    slots = np.array(sorted(runs))
+    assert np.array_equal(slots, np.arange(len(slots)))
+    tg = np.array([runs[k] for k in slots])
+    assert np.all(np.diff(tg) >= 0)  # consecutive per target
     for g0 in range(0, len(ch), 64):
         assert np.all(cnt[g0:g0 + 64] == cnt[g0]) and np.all(first[g0:g0 + 64] == first[g0] + np.arange(len(ch[g0:g0 + 64])))
     got = np.zeros(nz + 1, complex)
