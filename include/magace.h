@@ -74,6 +74,21 @@ int magace_compute_device(int natom, int nei, const int *offsets, const double *
                           double *flist, double *tlist, double *tlist0, void *stream);
 int magace_check_errors(void *stream);
 
+/* magace_compute_device without allocating device memory: no counterpart in the reference.
+ * For callers that cannot allocate (code under graph capture, tight serving loops).  The
+ * per-atom intermediates live in a fixed 128 MiB device array of the library instead of
+ * buffers grown with hipMalloc, so natom is limited (about 2490 atoms with the lmax-6 header
+ * in mixed precision, 1440 in FP64; 7000 and 4100 with the lmax-4 header); larger natom is
+ * rejected with MAGACE_ERR_INPUT and a message giving the limit.  The potential's tables must
+ * already be on the current device: magace_init_file/acelibini_ upload them when a GPU is
+ * visible (otherwise run magace_compute_device once).  All calls on one GPU share the array:
+ * calls that overlap on one GPU (different streams, threads) race. */
+int magace_compute_device_noalloc(int natom, int nei, const int *offsets, const double *dlist,
+                                  const double *elist, const double *dlist_momi,
+                                  const double *elist_momi, const double *dlist_mom,
+                                  const double *elist_mom, double *eout, double *flist,
+                                  double *tlist, double *tlist0, void *stream);
+
 /* Caller side of the boundary: PairSpinACE::compute (lammps/pair_spin_ace.cpp:340-669) on
  * LAMMPS-shaped arrays.  x[nall][3], sp[nall][4] (unit direction, magnitude), type[nall],
  * map[type] -> element index (must be 0), full neighbour list ilist/numneigh with the

@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libmagace.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MAGACE_ARCH", "gfx950")
 
-SOURCES = ["device.hip", "neighbors.hip", "md.hip", "api.cpp", "basis.cpp", "potential.cpp", "splines.cpp", "pair_host.cpp",
+SOURCES = ["device.hip", "neighbors.hip", "md.hip", "noalloc.hip", "api.cpp", "basis.cpp", "potential.cpp", "splines.cpp", "pair_host.cpp",
            "host_pool.cpp"]
 # neighbour lists reproduce the reference's FP64 values bit for bit: no contraction
 EXTRA = {"neighbors.hip": ["-ffp-contract=off"]}

@@ -28,7 +28,7 @@ EXPORTS = ("acelibini_", "acelib_", "magace_last_error", "magace_error_string", 
            "magace_neighbors_device", "magace_packstructures", "magace_md_create", "magace_md_run",
            "magace_md_read", "magace_md_destroy", "magace_fit_ncoef", "magace_fit_get_coefficients",
            "magace_fit_set_coefficients", "magace_fit_forward", "magace_fit_backward", "magace_write_potential",
-           "magace_model_records")
+           "magace_model_records", "magace_compute_device_noalloc")
 
 ERR_CAPACITY = 4
 
@@ -56,6 +56,7 @@ def lib():
     L.acelib_.argtypes = [_IP, _IP, _IP, _I, _I, _I, _I, _I, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]
     L.acelib_.restype = None
     L.magace_compute_device.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 12
+    L.magace_compute_device_noalloc.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 12
     L.magace_check_errors.argtypes = [C.c_void_p]
     L.magace_basis_counts.argtypes = [_I]
     L.magace_enumerate.argtypes = [_I, C.c_int, C.c_void_p, C.c_int]
@@ -144,6 +145,15 @@ def compute_device(natom, nei, offsets, dlist, elist, dmomi, emomi, dmom, emom, 
     """acelib_ on device pointers (ints).  Enqueued on `stream` (int handle, 0 = library stream)."""
     rc = lib().magace_compute_device(natom, nei, offsets, dlist, elist, dmomi, emomi, dmom, emom, eout,
                                      flist, tlist, tlist0, stream or None)
+    _check(rc)
+
+
+def compute_device_noalloc(natom, nei, offsets, dlist, elist, dmomi, emomi, dmom, emom, eout, flist, tlist,
+                           tlist0, stream=0):
+    """compute_device with the per-atom intermediates in the library's fixed device scratch:
+    no device memory is allocated (a size beyond the scratch raises MagaceError)."""
+    rc = lib().magace_compute_device_noalloc(natom, nei, offsets, dlist, elist, dmomi, emomi, dmom, emom, eout,
+                                             flist, tlist, tlist0, stream or None)
     _check(rc)
 
 

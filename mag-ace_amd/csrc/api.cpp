@@ -330,6 +330,10 @@ int upload(Context& c) {
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(upload_legendre_constants());
+    if (scratch_setup() != 0) {
+        set_error(MAGACE_ERR_DEVICE, "hipGetSymbolAddress failed for the fixed evaluation scratch");
+        return MAGACE_ERR_DEVICE;
+    }
     HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
@@ -557,6 +561,30 @@ int check_kernel_errors(Context& c, hipStream_t s) {
     return 0;
 }
 
+// Per-atom buffers of the allocation-free entry point, carved from the fixed device scratch:
+// the adjoint's partial sums (pbytes per atom), Z and W rows of zbytes each, then ehc, rho1,
+// dE, edf, rho; every region 256-byte aligned.
+// Returns the number of atoms that fit.
+size_t carve_scratch(char* base, size_t bytes, size_t zbytes, int nd, size_t pbytes, Buffers* B) {
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t per_atom = 2 * zbytes + pbytes + sizeof(double) * (2 + 3 * (size_t)nd);
+    size_t cap = bytes / per_atom;
+    while (cap > 0 && up(cap * zbytes) * 2 + up(cap * pbytes) + up(cap * 8) * 2 + up(cap * nd * 8) * 3 > bytes) --cap;
+    if (B && cap > 0) {
+        size_t o = 0;
+        B->Pt = base + o; o += up(cap * pbytes);
+        B->Z = base + o; o += up(cap * zbytes);
+        B->W = base + o; o += up(cap * zbytes);
+        B->ehc = (double*)(base + o); o += up(cap * 8);
+        B->rho1 = (double*)(base + o); o += up(cap * nd * 8);
+        B->dE = (double*)(base + o); o += up(cap * nd * 8);
+        B->edf = (double*)(base + o); o += up(cap * 8);
+        B->rho = (double*)(base + o); o += up(cap * nd * 8);
+        B->cap_atoms = cap;
+    }
+    return cap;
+}
+
 }  // namespace
 
 unsigned long long context_generation() { return g_generation.load(); }
@@ -601,6 +629,12 @@ int magace_init_file(const char* path, const char* element, double rcut, double 
     ctx->precision = prec;
     g_ctx = std::move(ctx);
     bump_generation();
+    // With a GPU visible the tables go to the current device now, so that the
+    // allocation-free entry point finds them (and its scratch address) ready; without one
+    // the host model stays usable and the upload waits for the first evaluation.
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0) return upload(*g_ctx);
+    (void)hipGetLastError();
     return 0;
 }
 
@@ -667,6 +701,49 @@ int magace_compute_device(int natom, int nei, const int* offsets, const double* 
                             eout, flist, tlist, tlist0, s))
         return rc;
     return 0;
+}
+
+int magace_compute_device_noalloc(int natom, int nei, const int* offsets, const double* dlist,
+                                  const double* elist, const double* dlist_momi, const double* elist_momi,
+                                  const double* dlist_mom, const double* elist_mom, double* eout,
+                                  double* flist, double* tlist, double* tlist0, void* stream) {
+    clear_error();
+    if (!g_ctx) {
+        set_error(MAGACE_ERR_STATE, "library not initialised");
+        return MAGACE_ERR_STATE;
+    }
+    Context& c = *g_ctx;
+    if (natom < 0 || nei < 0) {
+        set_error(MAGACE_ERR_INPUT, "negative sizes");
+        return MAGACE_ERR_INPUT;
+    }
+    if (!c.uploaded) {
+        set_error(MAGACE_ERR_STATE, "magace_compute_device_noalloc: the potential's tables are not on the device "
+                                    "(initialise with a GPU visible, or run one magace_compute_device first)");
+        return MAGACE_ERR_STATE;
+    }
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    void* base = scratch_device_ptr(dev);
+    if (dev != c.device || !base) {
+        set_error(MAGACE_ERR_STATE, "magace_compute_device_noalloc: the current device is not the one the "
+                                    "potential was loaded on");
+        return MAGACE_ERR_STATE;
+    }
+    const size_t zb = (size_t)c.model.z.nz * (c.precision == MAGACE_FP64 ? 16 : 8);
+    Buffers B = c.buf;  // err flag of the context (allocated at set-up)
+    const int nd = c.model.pot.h.ndensity;
+    const size_t cap = carve_scratch((char*)base, scratch_bytes(), zb, nd, partials_bytes(c.model.terms.nslot, nd), &B);
+    if ((size_t)natom > cap) {
+        set_error(MAGACE_ERR_INPUT, "magace_compute_device_noalloc: " + std::to_string(natom) +
+                                        " atoms exceed the fixed scratch (" + std::to_string(scratch_bytes() >> 20) +
+                                        " MiB: at most " + std::to_string(cap) +
+                                        " atoms for this potential and precision)");
+        return MAGACE_ERR_INPUT;
+    }
+    if (natom == 0) return 0;
+    return run_device(c, natom, offsets, dlist, elist, dlist_momi, elist_momi, dlist_mom, elist_mom, eout, flist,
+                      tlist, tlist0, (hipStream_t)stream, true, true, &B);
 }
 
 int magace_check_errors(void* stream) {

@@ -139,6 +139,10 @@ unsigned long long context_generation();
 // (no evaluation), so that a following capture records no allocation; MAGACE_ERR_STATE
 // without a potential
 int context_prepare(int natom);
+// noalloc.hip: the fixed device scratch of magace_compute_device_noalloc
+int scratch_setup();                 // looks up the scratch address on the current device
+void* scratch_device_ptr(int dev);   // nullptr until scratch_setup ran on that device
+size_t scratch_bytes();
 // potential.cpp
 bool parse_potential(const std::string& path, const std::string& element, int xc, double smear,
                      Potential& pot, std::string& err);
