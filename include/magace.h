@@ -61,6 +61,25 @@ int magace_finalize(void);
 int magace_set_precision(int mode);
 int magace_get_precision(void);
 
+/* Kernel tuning (no counterpart in the reference).  The density kernel has variants of its
+ * neighbour staging and of the split of its column tasks over idle threads; the choice
+ * changes timing and the order of floating-point sums (so results in the last bits), never
+ * the arithmetic.  Defaults come from a table compiled into the library per potential header
+ * (measured by mag-ace_amd/tune.py), else from a built-in rule.
+ * v[4] = {FP64 variant, FP64 split, mixed variant, mixed split}; variant -1 = built-in rule,
+ * 0..2; split 0 = automatic, 1..8.
+ * magace_tune times MAGACE_TUNE_CANDIDATES (variant, split) candidates on the caller's
+ * device-resident system (magace_compute_device arguments, reps evaluations each, on `stream`)
+ * for the current precision, writes the average milliseconds per evaluation into ms[] (-1:
+ * candidate not available for this potential), keeps the fastest and returns it in best[2]. */
+#define MAGACE_TUNE_CANDIDATES 9
+int magace_set_tuning(const int *v, int n);
+int magace_get_tuning(int *v, int n);
+int magace_tune(int natom, int nei, const int *offsets, const double *dlist, const double *elist,
+                const double *dlist_momi, const double *elist_momi, const double *dlist_mom,
+                const double *elist_mom, double *eout, double *flist, double *tlist, double *tlist0,
+                int reps, double *ms, int *best, void *stream);
+
 /* acelib_ on device-resident arrays (same meaning as acelib_, FP64), with 0-based CSR
  * offsets[natom+1] instead of nstart/nstop.  Enqueued on `stream` (a hipStream_t; NULL is the
  * default stream, as in HIP); no host synchronisation.  magace_check_errors() synchronises the

@@ -28,7 +28,8 @@ EXPORTS = ("acelibini_", "acelib_", "magace_last_error", "magace_error_string", 
            "magace_neighbors_device", "magace_packstructures", "magace_md_create", "magace_md_run",
            "magace_md_read", "magace_md_destroy", "magace_fit_ncoef", "magace_fit_get_coefficients",
            "magace_fit_set_coefficients", "magace_fit_forward", "magace_fit_backward", "magace_write_potential",
-           "magace_model_records", "magace_compute_device_noalloc")
+           "magace_model_records", "magace_compute_device_noalloc", "magace_set_tuning", "magace_get_tuning",
+           "magace_tune")
 
 ERR_CAPACITY = 4
 
@@ -57,6 +58,9 @@ def lib():
     L.acelib_.restype = None
     L.magace_compute_device.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 12
     L.magace_compute_device_noalloc.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 12
+    L.magace_set_tuning.argtypes = [_I, C.c_int]
+    L.magace_get_tuning.argtypes = [_I, C.c_int]
+    L.magace_tune.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 11 + [C.c_int, _D, _I, C.c_void_p]
     L.magace_check_errors.argtypes = [C.c_void_p]
     L.magace_basis_counts.argtypes = [_I]
     L.magace_enumerate.argtypes = [_I, C.c_int, C.c_void_p, C.c_int]
@@ -155,6 +159,32 @@ def compute_device_noalloc(natom, nei, offsets, dlist, elist, dmomi, emomi, dmom
     rc = lib().magace_compute_device_noalloc(natom, nei, offsets, dlist, elist, dmomi, emomi, dmom, emom, eout,
                                              flist, tlist, tlist0, stream or None)
     _check(rc)
+
+
+TUNE_CANDIDATES = ((0, 0), (1, 1), (1, 2), (1, 3), (1, 4), (2, 1), (2, 2), (2, 3), (2, 4))
+
+
+def set_tuning(fp64=(-1, 0), mixed=(-1, 0)):
+    """Density kernel variant and neighbour split per precision (see magace_set_tuning)."""
+    _check(lib().magace_set_tuning(np.array([*fp64, *mixed], np.int32), 4))
+
+
+def get_tuning():
+    v = np.zeros(4, np.int32)
+    _check(lib().magace_get_tuning(v, 4))
+    return (int(v[0]), int(v[1])), (int(v[2]), int(v[3]))
+
+
+def tune(natom, nei, offsets, dlist, elist, dmomi, emomi, dmom, emom, eout, flist, tlist, tlist0, reps=5,
+         stream=0):
+    """Times the density variants on device arrays for the current precision and keeps the
+    fastest.  Returns (best (variant, split), {candidate: ms per evaluation or None})."""
+    ms = np.zeros(len(TUNE_CANDIDATES))
+    best = np.zeros(2, np.int32)
+    rc = lib().magace_tune(natom, nei, offsets, dlist, elist, dmomi, emomi, dmom, emom, eout, flist, tlist, tlist0,
+                           int(reps), ms, best, stream or None)
+    _check(rc)
+    return (int(best[0]), int(best[1])), {c: (float(t) if t >= 0 else None) for c, t in zip(TUNE_CANDIDATES, ms)}
 
 
 def check_errors(stream=0):

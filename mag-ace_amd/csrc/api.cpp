@@ -72,6 +72,10 @@ struct Context {
     Model model;
     int precision = MAGACE_FP64;
     int shape_class = 0;
+    // density staging variant and neighbour split per precision (index 0 FP64, 1 mixed):
+    // -1/0 = the built-in rule, otherwise a tuned choice (tuning table or magace_tune)
+    int dvar[2] = {-1, -1};
+    int dsplit[2] = {0, 0};
     DevParams<double> P64;
     DevParams<float> P32;
     DevTables<double> t64;
@@ -120,6 +124,35 @@ struct Context {
 };
 
 std::unique_ptr<Context> g_ctx;
+
+// Tuned density variants per potential header, measured on MI355X by mag-ace_amd/tune.py
+// (which writes tuning_table.inc and the record tuning.json); headers not listed use the
+// built-in rule.
+struct TuneEntry {
+    int hdr[12];  // nb_a nb_m nr2_m l2m nr3_a nr3_m l3a l3m nr4_a nr4_m l4a l4m
+    int fp64[2];  // variant, neighbour split
+    int mixed[2];
+};
+const TuneEntry kTuning[] = {
+#include "tuning_table.inc"
+    {{-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}, {-1, 0}, {-1, 0}},  // end
+};
+
+void apply_tuning_table(Context& c) {
+    const Header& h = c.model.pot.h;
+    const int key[12] = {h.nb_a, h.nb_m, h.nr2_m, h.l2m, h.nr3_a, h.nr3_m, h.l3a, h.l3m,
+                         h.nr4_a, h.nr4_m, h.l4a, h.l4m};
+    for (const TuneEntry& e : kTuning) {
+        if (e.hdr[0] < 0) break;
+        if (std::equal(key, key + 12, e.hdr)) {
+            c.dvar[0] = e.fp64[0]; c.dsplit[0] = e.fp64[1];
+            c.dvar[1] = e.mixed[0]; c.dsplit[1] = e.mixed[1];
+            break;
+        }
+    }
+    c.P64.sa = c.dsplit[0];
+    c.P32.sa = c.dsplit[1];
+}
 
 // generation of the state baked into captured evaluations (context_generation)
 std::atomic<unsigned long long> g_generation{1};
@@ -184,6 +217,12 @@ void fill_params(DevParams<T>& P, const Model& M) {
     P.nr3a = h.nr3_a; P.nr3m = h.nr3_m; P.l3a = h.l3a; P.l3m = h.l3m;
     P.lsa = M.z.lsa; P.lsm = M.z.lsm; P.lsm3 = M.z.lsm3; P.nm3 = M.z.nm3;
     P.oA2 = M.z.oA2; P.oA0 = M.z.oA0; P.nz = M.z.nz;
+    for (int q = 0; q < M.z.nm3 && q < 32; ++q) {
+        const int lq = q % M.z.lsm3;
+        int lp = 0;
+        while ((lp + 1) * (lp + 1) <= lq) ++lp;
+        P.mpq[q] = (signed char)(lq - lp * lp - lp);
+    }
     P.lha = (h.l3a + 1) * (h.l3a + 2) / 2;
     P.lhm = (h.l2m + 1) * (h.l2m + 2) / 2;
     P.nchunks = (int)(M.terms.chunks.size() / 2);
@@ -237,8 +276,8 @@ int build_context(Potential&& pot, std::unique_ptr<Context>& out) {
         set_error(MAGACE_ERR_INPUT, "potential shape outside the compiled kernel classes");
         return MAGACE_ERR_INPUT;
     }
-    if (h.nb_m > 32) {
-        set_error(MAGACE_ERR_INPUT, "nradbase_magnetic > 32");
+    if (h.nb_m > 32 || M.z.nm3 > 32) {
+        set_error(MAGACE_ERR_INPUT, "nradbase_magnetic > 32 or more than 32 magnetic columns");
         return MAGACE_ERR_INPUT;
     }
     M.terms = build_terms(M.pot, M.z);
@@ -254,6 +293,7 @@ int build_context(Potential&& pot, std::unique_ptr<Context>& out) {
     M.spl = build_splines(M.pot);
     fill_params(ctx->P64, M);
     fill_params(ctx->P32, M);
+    apply_tuning_table(*ctx);
     out = std::move(ctx);
     return 0;
 }
@@ -435,9 +475,9 @@ hipError_t launch(Context& c, int natom, int atom_base, const int* off, const do
                   hipEvent_t* ev) {
     if (c.precision == MAGACE_FP64)
         return Launch<double>::run(c.P64, natom, atom_base, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist,
-                                   tlist, tlist0, B, s, c.shape_class, ev);
+                                   tlist, tlist0, B, s, c.shape_class, ev, c.dvar[0]);
     return Launch<float>::run(c.P32, natom, atom_base, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist,
-                              tlist, tlist0, B, s, c.shape_class, ev);
+                              tlist, tlist0, B, s, c.shape_class, ev, c.dvar[1]);
 }
 
 int run_device(Context& c, int natom, const int* off, const double* dlist, const double* elist,
@@ -681,6 +721,116 @@ int magace_set_precision(int mode) {
 }
 
 int magace_get_precision(void) { return g_ctx ? g_ctx->precision : -1; }
+
+int magace_set_tuning(const int* v, int n) {
+    clear_error();
+    if (!g_ctx) {
+        set_error(MAGACE_ERR_STATE, "library not initialised");
+        return MAGACE_ERR_STATE;
+    }
+    if (!v || n != 4) {
+        set_error(MAGACE_ERR_INPUT, "magace_set_tuning: 4 values (FP64 variant, split, mixed variant, split)");
+        return MAGACE_ERR_INPUT;
+    }
+    for (int p = 0; p < 2; ++p)
+        if (v[2 * p] < -1 || v[2 * p] >= kDensityVariants || v[2 * p + 1] < 0 || v[2 * p + 1] > 8) {
+            set_error(MAGACE_ERR_INPUT, "magace_set_tuning: variant in -1..2, split in 0..8");
+            return MAGACE_ERR_INPUT;
+        }
+    Context& c = *g_ctx;
+    for (int p = 0; p < 2; ++p) {
+        c.dvar[p] = v[2 * p];
+        c.dsplit[p] = v[2 * p + 1];
+    }
+    c.P64.sa = c.dsplit[0];
+    c.P32.sa = c.dsplit[1];
+    bump_generation();  // captured evaluations hold the old variant
+    return 0;
+}
+
+int magace_get_tuning(int* v, int n) {
+    clear_error();
+    if (!g_ctx) {
+        set_error(MAGACE_ERR_STATE, "library not initialised");
+        return MAGACE_ERR_STATE;
+    }
+    if (!v || n != 4) {
+        set_error(MAGACE_ERR_INPUT, "magace_get_tuning: 4 values");
+        return MAGACE_ERR_INPUT;
+    }
+    for (int p = 0; p < 2; ++p) {
+        v[2 * p] = g_ctx->dvar[p];
+        v[2 * p + 1] = g_ctx->dsplit[p];
+    }
+    return 0;
+}
+
+int magace_tune(int natom, int nei, const int* offsets, const double* dlist, const double* elist,
+                const double* dlist_momi, const double* elist_momi, const double* dlist_mom,
+                const double* elist_mom, double* eout, double* flist, double* tlist, double* tlist0, int reps,
+                double* ms, int* best, void* stream) {
+    clear_error();
+    if (!g_ctx) {
+        set_error(MAGACE_ERR_STATE, "library not initialised");
+        return MAGACE_ERR_STATE;
+    }
+    if (natom <= 0 || nei < 0 || reps <= 0 || !ms || !best) {
+        set_error(MAGACE_ERR_INPUT, "magace_tune: invalid arguments");
+        return MAGACE_ERR_INPUT;
+    }
+    Context& c = *g_ctx;
+    hipStream_t s = (hipStream_t)stream;
+    const int p = c.precision == MAGACE_FP64 ? 0 : 1;
+    // (variant, neighbour split) candidates, MAGACE_TUNE_CANDIDATES of them
+    static const int cand[MAGACE_TUNE_CANDIDATES][2] = {{0, 0}, {1, 1}, {1, 2}, {1, 3}, {1, 4},
+                                                        {2, 1}, {2, 2}, {2, 3}, {2, 4}};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int bi = -1;
+    for (int k = 0; k < MAGACE_TUNE_CANDIDATES; ++k) {
+        c.dvar[p] = cand[k][0];
+        c.dsplit[p] = cand[k][1];
+        (p ? c.P32.sa : c.P64.sa) = cand[k][1];
+        ms[k] = -1.0;
+        // a variant whose LDS does not fit this potential fails at launch and is skipped
+        int rc = run_device(c, natom, offsets, dlist, elist, dlist_momi, elist_momi, dlist_mom, elist_mom, eout,
+                            flist, tlist, tlist0, s);
+        if (rc == 0) rc = check_kernel_errors(c, s);
+        if (rc) {
+            clear_error();
+            (void)hipGetLastError();
+            continue;
+        }
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps && rc == 0; ++r)
+            rc = run_device(c, natom, offsets, dlist, elist, dlist_momi, elist_momi, dlist_mom, elist_mom, eout, flist,
+                            tlist, tlist0, s);
+        (void)hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess || rc) {
+            clear_error();
+            (void)hipGetLastError();
+            continue;
+        }
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        ms[k] = t / reps;
+        if (bi < 0 || ms[k] < ms[bi]) bi = k;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (bi < 0) {
+        set_error(MAGACE_ERR_DEVICE, "magace_tune: no variant ran");
+        return MAGACE_ERR_DEVICE;
+    }
+    c.dvar[p] = cand[bi][0];
+    c.dsplit[p] = cand[bi][1];
+    (p ? c.P32.sa : c.P64.sa) = cand[bi][1];
+    best[0] = cand[bi][0];
+    best[1] = cand[bi][1];
+    bump_generation();
+    return 0;
+}
 
 int magace_compute_device(int natom, int nei, const int* offsets, const double* dlist,
                           const double* elist, const double* dlist_momi, const double* elist_momi,

@@ -216,8 +216,52 @@ __device__ __forceinline__ cplx<T> yfull(const cplx<T>* half, int l, int m) {
 // magnetic columns): 3 waves per SIMD in FP32 (168 registers, a 12-byte spill; on the shape
 // of the example potential 0.75 -> 0.57 ms) and 2 in FP64 (256 registers with a 192-byte
 // spill instead of 302 registers at 1 wave: 2.06 -> 1.33 ms).
+// Stores of the m >= 0 density tasks (k_density, k_density_jvp): A[n][l,m][q] for the task's
+// radial block and, for m > 0, the conjugate partners
+// A[n][l,-m][qbar] = (-1)^(m+m') conj(A[n][l,m][q]) with q = n' lsm3 + l'^2 + l' + m' and
+// qbar = n' lsm3 + l'^2 + l' - m'; A2[n][n'][l',m'] and A2[n][n'][l',-m'] = (-1)^m' conj(.).
+template <typename T, int NB, int NM3X, int NACC, int NQ>
+__device__ __forceinline__ void store_a_half(const DevParams<T>& P, const cplx<T> (&acc)[NACC][NQ], int n0, int l,
+                                             int m, cplx<T>* __restrict__ Z) {
+    const int lm = l * l + l + m;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int n = n0 + b;
+        if (n < P.nr3a)
+#pragma unroll
+            for (int q = 0; q < NM3X; ++q)
+                if (q < P.nm3) {
+                    Z[(n * P.lsa + lm) * P.nm3 + q] = acc[b][q];
+                    if (m > 0) {
+                        // m' of column q (a kernel argument: q is a compile-time index); its
+                        // partner is column q - 2 m'
+                        const int mp = P.mpq[q];
+                        const T sg = ((m + mp) & 1) ? (T)-1 : (T)1;
+                        Z[(n * P.lsa + lm - 2 * m) * P.nm3 + q - 2 * mp] = {sg * acc[b][q].re, -sg * acc[b][q].im};
+                    }
+                }
+    }
+}
+template <typename T, int NB2, int NR2X, int NACC, int NQ>
+__device__ __forceinline__ void store_a2_half(const DevParams<T>& P, const cplx<T> (&acc)[NACC][NQ], int n0, int m,
+                                              int lm, cplx<T>* __restrict__ Z) {
+    const T s = (m & 1) ? (T)-1 : (T)1;
+#pragma unroll
+    for (int b = 0; b < NB2; ++b) {
+        const int n = n0 + b;
+        if (n < P.nb_a)
+#pragma unroll
+            for (int q = 0; q < NR2X; ++q)
+                if (q < P.nr2m) {
+                    cplx<T>* z = Z + P.oA2 + (n * P.nr2m + q) * P.lsm;
+                    z[lm] = acc[b][q];
+                    if (m > 0) z[lm - 2 * m] = {s * acc[b][q].re, -s * acc[b][q].im};
+                }
+    }
+}
+
 template <typename T, int L, int CH, int NB, int NM3X, int NB2, int NR2X, bool SPLIT, bool TASKSPLIT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6 ? (sizeof(T) == 4 ? 5 : 3) : (sizeof(T) == 4 ? 3 : 2), 8))) k_density(DevParams<T> P, const int* __restrict__ off,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6 && NM3X <= 4 ? (sizeof(T) == 4 ? 5 : 3) : (sizeof(T) == 4 ? 3 : 2), 8))) k_density(DevParams<T> P, const int* __restrict__ off,
                                                 const double* __restrict__ dlist,
                                                 const double* __restrict__ elist,
                                                 const double* __restrict__ dmomi,
@@ -283,19 +327,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
         Zout[P.oA0 + np * P.lsm + lm] = {mv * y.re, mv * y.im};
     }
 
+    // Tasks cover m >= 0 only (P.lha and P.lhm entries per radial block): since R, M and g are
+    // real and Y_{l,-m} = (-1)^m conj(Y_lm),
+    //   A[n][l,-m][n',l',-m'] = (-1)^(m+m') conj(A[n][l,m][n',l',m'])  and
+    //   A2[n][n'][l',-m'] = (-1)^m' conj(A2[n][n'][l',m']),
+    // which are written at the store from the m > 0 sums (m = 0 tasks hold all their columns).
     const int nblkA = (P.nr3a + NB - 1) / NB, nblkA2 = (P.nb_a + NB2 - 1) / NB2;
-    const int nTA = P.lsa * nblkA, nTA2 = P.lsm * nblkA2;
-    // TASKSPLIT (light headers): when the A and A2 column tasks leave most threads idle
-    // (lmax-4 headers: 75 tasks for 256 threads), each A task's neighbours are split over SA
-    // threads (neighbour j on part j mod SA); the parts meet in LDS at the end, in the staging
-    // area, which is free by then.  With SA = 2 (lmax 6) the extra barriers and scratch
-    // traffic cost more than they gain, so SA < 3 falls back to 1.
+    const int nTA = P.lha * nblkA, nTA2 = P.lhm * nblkA2;
+    // TASKSPLIT: when the A and A2 column tasks leave threads idle, each A task's neighbours
+    // are split over SA threads (neighbour j on part j mod SA); the parts meet in LDS at the
+    // end, in the staging area, which is free by then.  SA is P.sa when that is set (a tuned
+    // choice), otherwise the largest that fits (at most 4), and 1 when that is below 3 (the
+    // extra barrier and scratch traffic of SA = 2 cost more than it gained with all-m tasks).
     int SA = 1;
     if constexpr (TASKSPLIT) if (nTA + nTA2 <= nthr) {
-        SA = min(4, (nthr - nTA2) / max(nTA, 1));
+        SA = min(P.sa > 0 ? P.sa : 4, (nthr - nTA2) / max(nTA, 1));
         const size_t room = (size_t)((const char*)sEhc - (const char*)smem);
         while (SA > 1 && (size_t)(SA - 1) * nTA * NB * NM3X * sizeof(cplx<T>) > room) --SA;
-        if (SA < 3) SA = 1;
+        if (P.sa <= 0 && SA < 3) SA = 1;
     }
     // A2 tasks start at a wave boundary when that still fits in one round: a wave holding both
     // kinds would run both loops one after the other
@@ -314,16 +363,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
         const int sa = kSplit ? SA : 1;
         const bool isA = task < sa * nTA, isA2 = task >= a2base && task < a2base + nTA2;
         const int ta = kSplit && isA ? task % nTA : task, part = kSplit && isA ? task / nTA : 0;
-        int lm = 0, l = 0, m = 0, n0 = 0;
-        if (isA) { lm = ta % P.lsa; n0 = (ta / P.lsa) * NB; }
-        if (isA2) { lm = (task - a2base) % P.lsm; n0 = ((task - a2base) / P.lsm) * NB2; }
-        while ((l + 1) * (l + 1) <= lm) ++l;
-        m = lm - l * l - l;
-        // Y_{l,-|m|} = (-1)^|m| conj(Y_{l,|m|}): the loops read the stored m >= 0 entry and
-        // flip the sign of its imaginary part; (-1)^|m| is applied once, at the store
-        const int lh = l * (l + 1) / 2 + (m < 0 ? -m : m);
-        const T cim = m < 0 ? (T)-1 : (T)1;
-        const T cst = (m < 0 && ((-m) & 1)) ? (T)-1 : (T)1;
+        int lh = 0, l = 0, n0 = 0;
+        if (isA) { lh = ta % P.lha; n0 = (ta / P.lha) * NB; }
+        if (isA2) { lh = (task - a2base) % P.lhm; n0 = ((task - a2base) / P.lhm) * NB2; }
+        while ((l + 1) * (l + 2) / 2 <= lh) ++l;
+        const int m = lh - l * (l + 1) / 2;  // >= 0
+        const int lm = l * l + l + m;
 
         for (int c0 = 0; c0 < nn; c0 += CH) {
             const int nj = nn - c0 < CH ? nn - c0 : CH;
@@ -464,8 +509,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                 // and radial indices past nr3a are read without a test: those reads stay inside
                 // the staging area, and their sums land in slots that are never stored.
                 for (int j = part; j < nj; j += sa) {
-                    const cplx<T> yv = sY[j * P.lha + lh];
-                    const cplx<T> y = {yv.re, cim * yv.im};
+                    const cplx<T> y = sY[j * P.lha + lh];
                     cplx<T> h[NM3X];
 #pragma unroll
                     for (int q = 0; q < NM3X; ++q) h[q] = sH[j * P.nm3 + q];
@@ -488,8 +532,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                 // A2[n][n'][l'm'] += g_n M_{l',n'} Y_{l'm'}   (src/a_basis.F90:193-201); unused
                 // slots as above
                 for (int j = 0; j < nj; ++j) {
-                    const cplx<T> yv = sYm[j * P.lhm + lh];
-                    const cplx<T> y = {yv.re, cim * yv.im};
+                    const cplx<T> y = sYm[j * P.lhm + lh];
                     const T* mr = sM + j * nfm + l * P.nr2m;
                     cplx<T> my[NR2X];
 #pragma unroll
@@ -528,24 +571,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                         }
         }
         if (isA && part == 0) {
-#pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                const int n = n0 + b;
-                if (n < P.nr3a)
-#pragma unroll
-                    for (int q = 0; q < NM3X; ++q)
-                        if (q < P.nm3) Zout[(n * P.lsa + lm) * P.nm3 + q] = {cst * acc[b][q].re, cst * acc[b][q].im};
-            }
+            store_a_half<T, NB, NM3X>(P, acc, n0, l, m, Zout);
         } else if (isA2) {
-#pragma unroll
-            for (int b = 0; b < NB2; ++b) {
-                const int n = n0 + b;
-                if (n < P.nb_a)
-#pragma unroll
-                    for (int q = 0; q < NR2X; ++q)
-                        if (q < P.nr2m)
-                            Zout[P.oA2 + (n * P.nr2m + q) * P.lsm + lm] = {cst * acc[b][q].re, cst * acc[b][q].im};
-            }
+            store_a2_half<T, NB2, NR2X>(P, acc, n0, m, lm, Zout);
         }
     }
     __syncthreads();
@@ -1478,7 +1506,8 @@ k_density_jvp(DevParams<double> P, const int* __restrict__ off,
     if (tid == 0) *sE = 0.0;
 
     const int nblkA = (P.nr3a + NB - 1) / NB, nblkA2 = (P.nb_a + NB2 - 1) / NB2;
-    const int nTA = P.lsa * nblkA, nTA2 = P.lsm * nblkA2;
+    // tasks cover m >= 0 only; the m < 0 entries are conjugates (see k_density)
+    const int nTA = P.lha * nblkA, nTA2 = P.lhm * nblkA2;
     // A2 tasks start at a wave boundary when that fits in one round (no wave runs both loops)
     const int a2base = ((nTA + 63) & ~63) + nTA2 <= nthr ? (nTA + 63) & ~63 : nTA;
     for (int round = 0; round * nthr < a2base + nTA2; ++round) {
@@ -1489,15 +1518,12 @@ k_density_jvp(DevParams<double> P, const int* __restrict__ off,
 #pragma unroll
             for (int b = 0; b < (NM3X > NR2X ? NM3X : NR2X); ++b) acc[a][b] = {0, 0};
         const bool isA = task < nTA, isA2 = task >= a2base && task < a2base + nTA2;
-        int lm = 0, l = 0, m = 0, n0 = 0;
-        if (isA) { lm = task % P.lsa; n0 = (task / P.lsa) * NB; }
-        if (isA2) { lm = (task - a2base) % P.lsm; n0 = ((task - a2base) / P.lsm) * NB2; }
-        while ((l + 1) * (l + 1) <= lm) ++l;
-        m = lm - l * l - l;
-        // m < 0 from the stored m >= 0 entries, as in k_density
-        const int lh = l * (l + 1) / 2 + (m < 0 ? -m : m);
-        const T cim = m < 0 ? (T)-1 : (T)1;
-        const T cst = (m < 0 && ((-m) & 1)) ? (T)-1 : (T)1;
+        int lh = 0, l = 0, n0 = 0;
+        if (isA) { lh = task % P.lha; n0 = (task / P.lha) * NB; }
+        if (isA2) { lh = (task - a2base) % P.lhm; n0 = ((task - a2base) / P.lhm) * NB2; }
+        while ((l + 1) * (l + 2) / 2 <= lh) ++l;
+        const int m = lh - l * (l + 1) / 2;  // >= 0
+        const int lm = l * l + l + m;
         for (int c0 = 0; c0 < nn; c0 += CH) {
             const int nj = nn - c0 < CH ? nn - c0 : CH;
             __syncthreads();
@@ -1571,8 +1597,7 @@ k_density_jvp(DevParams<double> P, const int* __restrict__ off,
             if (isA) {
                 // unused columns and radial indices are read without a test, as in k_density
                 for (int j = 0; j < nj; ++j) {
-                    const cplx<T> yv = sY[j * P.lha + lh], ydv = sYd[j * P.lha + lh];
-                    const cplx<T> y = {yv.re, cim * yv.im}, yd = {ydv.re, cim * ydv.im};
+                    const cplx<T> y = sY[j * P.lha + lh], yd = sYd[j * P.lha + lh];
                     cplx<T> h[NM3X];
 #pragma unroll
                     for (int q = 0; q < NM3X; ++q) h[q] = sH[j * P.nm3 + q];
@@ -1588,8 +1613,7 @@ k_density_jvp(DevParams<double> P, const int* __restrict__ off,
                 }
             } else if (isA2) {
                 for (int j = 0; j < nj; ++j) {
-                    const cplx<T> yv = sYm[j * P.lhm + lh];
-                    const cplx<T> y = {yv.re, cim * yv.im};
+                    const cplx<T> y = sYm[j * P.lhm + lh];
                     const T* mr = sM + j * nfm + l * P.nr2m;
                     cplx<T> my[NR2X];
 #pragma unroll
@@ -1607,26 +1631,10 @@ k_density_jvp(DevParams<double> P, const int* __restrict__ off,
                 }
             }
         }
-        if (isA) {
-#pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                const int n = n0 + b;
-                if (n < P.nr3a)
-#pragma unroll
-                    for (int q = 0; q < NM3X; ++q)
-                        if (q < P.nm3) Zdot[(n * P.lsa + lm) * P.nm3 + q] = {cst * acc[b][q].re, cst * acc[b][q].im};
-            }
-        } else if (isA2) {
-#pragma unroll
-            for (int b = 0; b < NB2; ++b) {
-                const int n = n0 + b;
-                if (n < P.nb_a)
-#pragma unroll
-                    for (int q = 0; q < NR2X; ++q)
-                        if (q < P.nr2m)
-                            Zdot[P.oA2 + (n * P.nr2m + q) * P.lsm + lm] = {cst * acc[b][q].re, cst * acc[b][q].im};
-            }
-        }
+        if (isA)
+            store_a_half<T, NB, NM3X>(P, acc, n0, l, m, Zdot);
+        else if (isA2)
+            store_a2_half<T, NB2, NR2X>(P, acc, n0, m, lm, Zdot);
     }
     __syncthreads();
     if (tid == 0) ehc_dot[atom] = *sE;
@@ -1847,10 +1855,12 @@ hipError_t fit_backward(const DevParams<double>& P, int cls, int natom, const in
                         const double* dmom, const double* emom, const int* atom_struct, const double* gE,
                         Buffers& B, FitBuffers& F, int nterms, int nb234, double* grad, hipStream_t s) {
     if (natom <= 0) return hipSuccess;
-    hipError_t e = cls == 0 ? run_jvp_class<6, 32, 4, 4, 4, 2>(P, natom, off, nlist, du, dlist, elist, dmom, emom,
-                                                               (cplx<double>*)F.Zdot, F.ehc_dot, B.err, s)
-                            : run_jvp_class<2, 32, 1, 27, 2, 5>(P, natom, off, nlist, du, dlist, elist, dmom, emom,
-                                                                (cplx<double>*)F.Zdot, F.ehc_dot, B.err, s);
+    hipError_t e = cls == 0   ? run_jvp_class<6, 32, 4, 4, 4, 2>(P, natom, off, nlist, du, dlist, elist, dmom, emom,
+                                                                 (cplx<double>*)F.Zdot, F.ehc_dot, B.err, s)
+                   : cls == 1 ? run_jvp_class<2, 32, 1, 27, 2, 5>(P, natom, off, nlist, du, dlist, elist, dmom, emom,
+                                                                  (cplx<double>*)F.Zdot, F.ehc_dot, B.err, s)
+                              : run_jvp_class<6, 32, 4, 9, 4, 5>(P, natom, off, nlist, du, dlist, elist, dmom, emom,
+                                                                 (cplx<double>*)F.Zdot, F.ehc_dot, B.err, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_fit_atom, dim3(natom), dim3(256), 0, s, P, (const cplx<double>*)B.W,
                        (const cplx<double>*)F.Zdot, B.rho, B.ehc, F.ehc_dot, atom_struct, gE, F.alpha, F.beta);
@@ -1893,10 +1903,12 @@ hipError_t clear_flags(int* p, hipStream_t s) {
 
 // shape classes: 0 = small magnetic A block (nm3 <= 4, nr2m <= 2, nb_a <= 10, nr3a <= 8, l <= 6)
 //                1 = wide magnetic A block  (nm3 <= 27, nr2m <= 5, nb_a <= 16, nr3a <= 4, l <= 2)
+//                2 = medium magnetic A block at l <= 6 (nm3 <= 9, nr2m <= 5, nb_a <= 16, nr3a <= 8)
 int supported_nm3_class(int nm3, int nr2m, int nb_a, int l3a, int l2m, int nr3a) {
     if (nb_a > 16) return -1;
     if (nm3 <= 4 && nr2m <= 2 && l3a <= 6 && l2m <= 6 && nr3a <= 8 && nb_a <= 10) return 0;
     if (nm3 <= 27 && nr2m <= 5 && l3a <= 2 && l2m <= 2 && nr3a <= 4) return 1;
+    if (nm3 <= 9 && nr2m <= 5 && l3a <= 6 && l2m <= 6 && nr3a <= 8) return 2;
     return -1;
 }
 
@@ -1994,7 +2006,7 @@ hipError_t Launch<T>::run(const DevParams<T>& P, int natom, int atom_base, const
                           const double* elist, const double* dmomi, const double* emomi,
                           const double* dmom, const double* emom, double* eout, double* flist,
                           double* tlist, double* tlist0, Buffers& B, hipStream_t s, int cls,
-                          hipEvent_t* ev) {
+                          hipEvent_t* ev, int dvar) {
     if (natom <= 0) return hipSuccess;
     // Density staging: with many radial functions per pair (lmax-6 headers) or in FP64 the
     // per-pair staging is spread over the whole block, 32 neighbours per round; light FP32
@@ -2004,14 +2016,19 @@ hipError_t Launch<T>::run(const DevParams<T>& P, int natom, int atom_base, const
     // 0.93 -> 0.81 ms; with lmax 6 the runtime split costs FP64 3%).
     const bool heavy = P.nb_a + P.nr3a * (P.l3a + 1) >= 40;
 #define MAGACE_RUN(CH, SPLIT, TS)                                                                          \
-    (cls == 0 ? run_class<T, 6, CH, SPLIT, TS, 4, 4, 4, 2, 8, 10>(P, natom, atom_base, off, dlist, elist, dmomi,  \
-                                                                  emomi, dmom, emom, eout, flist, tlist, tlist0, B, \
-                                                                  s, ev)                                            \
-              : run_class<T, 2, CH, SPLIT, TS, 1, 27, 2, 5, 4, 16>(P, natom, atom_base, off, dlist, elist, dmomi, \
-                                                                   emomi, dmom, emom, eout, flist, tlist, tlist0, B, \
-                                                                   s, ev))
-    if (heavy) return MAGACE_RUN(32, true, false);
-    if (sizeof(T) == 8) return MAGACE_RUN(32, true, true);
+    (cls == 0   ? run_class<T, 6, CH, SPLIT, TS, 4, 4, 4, 2, 8, 10>(P, natom, atom_base, off, dlist, elist, dmomi,  \
+                                                                    emomi, dmom, emom, eout, flist, tlist, tlist0, B, \
+                                                                    s, ev)                                            \
+     : cls == 1 ? run_class<T, 2, CH, SPLIT, TS, 1, 27, 2, 5, 4, 16>(P, natom, atom_base, off, dlist, elist, dmomi, \
+                                                                     emomi, dmom, emom, eout, flist, tlist, tlist0, B, \
+                                                                     s, ev)                                           \
+                : run_class<T, 6, CH, SPLIT, TS, 4, 9, 4, 5, 8, 16>(P, natom, atom_base, off, dlist, elist, dmomi, \
+                                                                    emomi, dmom, emom, eout, flist, tlist, tlist0, B, \
+                                                                    s, ev))
+    // a density variant chosen by the tuner (kDensityVariants) replaces the rule above
+    if (dvar < 0) dvar = heavy ? 0 : (sizeof(T) == 8 ? 1 : 2);
+    if (dvar == 0) return MAGACE_RUN(32, true, false);
+    if (dvar == 1) return MAGACE_RUN(32, true, true);
     return MAGACE_RUN(64, false, true);
 #undef MAGACE_RUN
 }

@@ -28,6 +28,8 @@ struct DevParams {
     int nb_a = 0, nb_m = 0, nr2m = 0, l2m = 0, nr3a = 0, nr3m = 0, l3a = 0, l3m = 0;
     int lsa = 0, lsm = 0, lsm3 = 0, nm3 = 0, oA2 = 0, oA0 = 0, nz = 0;
     int lha = 0, lhm = 0;  // (l+1)(l+2)/2 for l3a, l2m
+    signed char mpq[32] = {0};  // m' of the magnetic column q of A (q < nm3 <= 32)
+    int sa = 0;                 // k_density: neighbour split of the A tasks (0 = automatic)
     // gradient contributions, one 16-byte record each: x = a | b << 16, y = c as byte offsets
     // of the factors in the adjoint's LDS copy of Z (index * sizeof(cplx<T>)), (z, w) = weight
     // (FP64 bits as lo/hi words, or FP32 bits in z)
@@ -78,13 +80,19 @@ __host__ __device__ inline size_t partials_bytes(int nslot, int nd) { return (si
 // atom_base: the index of the launch's first atom in the whole system (even), which fixes the
 // adjoint's walk per atom, so that results do not depend on how the atoms are split into
 // launches
+// Density staging variants (Launch::run's dvar; -1 = the built-in rule): 0 = 32 neighbours per
+// round, each pair's staging spread over 8 threads, no neighbour split of the A tasks; 1 = the
+// same with the split (DevParams::sa); 2 = 64 neighbours per round, one thread per pair and
+// side, with the split.
+constexpr int kDensityVariants = 3;
+
 template <typename T>
 struct Launch {
     static hipError_t run(const DevParams<T>& P, int natom, int atom_base, const int* offsets,
                           const double* dlist, const double* elist, const double* dmomi,
                           const double* emomi, const double* dmom, const double* emom, double* eout,
                           double* flist, double* tlist, double* tlist0, Buffers& B, hipStream_t s,
-                          int nm3_class, hipEvent_t* ev);
+                          int nm3_class, hipEvent_t* ev, int dvar = -1);
 };
 
 // fitting (FP64): forward-mode density, per-atom adjoint scalars, monomial accumulation

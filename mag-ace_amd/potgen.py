@@ -35,6 +35,13 @@ HFE = dict(BASE, nb_a=16, nb_m=16, nr2_m=5, l2m=2, nr3_a=4, nr3_m=3, l3a=1, l3m=
 #: lmax-4 header with two densities, E = F(rho_1) + F(rho_2) (src/forces.F90:1066-1085), a
 #: nonlinear embedding on the second density
 H500ND2 = dict(H500, ndensity=2, nfemb=1, p1a=1.0, p1b=1.0, p1a_2=0.7, p1b_2=0.6)
+#: lmax 6 with l3max_magnetic 2 (nine magnetic columns of A): "12 4 2 6 / 4 1 6 2 / 2 1 2 2"
+H6W = dict(BASE, nb_a=12, nb_m=4, nr2_m=2, l2m=6, nr3_a=4, nr3_m=1, l3a=6, l3m=2,
+           nr4_a=2, nr4_m=1, l4a=2, l4m=2)
+#: sixteen atomic and six magnetic radial base functions, five magnetic radial functions,
+#: l3max_magnetic 2 at lmax 4: "16 6 5 4 / 6 1 4 2 / 2 1 2 2"
+H4W = dict(BASE, nb_a=16, nb_m=6, nr2_m=5, l2m=4, nr3_a=6, nr3_m=1, l3a=4, l3m=2,
+           nr4_a=2, nr4_m=1, l4a=2, l4m=2)
 #: tiny header for fast unit tests
 HTINY = dict(BASE, nb_a=4, nb_m=2, nr2_m=2, l2m=2, nr3_a=2, nr3_m=1, l3a=2, l3m=1,
              nr4_a=2, nr4_m=1, l4a=2, l4m=1)

@@ -10,7 +10,7 @@ PKG = __graft_entry__.load_package()
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = os.path.join(HERE, "golden")
 CASES = ("tiny_16", "h500_128", "h2000_16", "hfe_16", "edge_h500core", "nd2_16", "radt46_16", "radt58_16",
-         "m0_16")
+         "m0_16", "h6w_16", "h4w_16")
 
 HEADERS = {"HTINY": PKG.potgen.HTINY, "H500": PKG.potgen.H500, "H2000": PKG.potgen.H2000,
            "HFE": PKG.potgen.HFE, "H500ND2": PKG.potgen.H500ND2,
@@ -18,7 +18,8 @@ HEADERS = {"HTINY": PKG.potgen.HTINY, "H500": PKG.potgen.H500, "H2000": PKG.potg
            "H500R58": dict(PKG.potgen.H500, radtype_a=5, radtype_m=8),
            "H500CORE": dict(PKG.potgen.H500, core_pre=25.0, core_lambda=0.35, ecut=14.0, decut=8.0,
                             nfemb=2, p1a=1.0, p1b=0.9),
-           "H500M0": dict(PKG.potgen.H500, nr3_m=2, l3m=0, l4m=0)}
+           "H500M0": dict(PKG.potgen.H500, nr3_m=2, l3m=0, l4m=0),
+           "H6W": PKG.potgen.H6W, "H4W": PKG.potgen.H4W}
 
 
 def potential_text(header_name, seed, enum="product"):

@@ -88,7 +88,8 @@ def test_energy_gradient_equals_oracle_b_vectors(tmp_path):
     assert err < 1e-10
 
 
-@pytest.mark.parametrize("header,seed,tol", [("H500", 2041, 1e-9), ("H500CORE", 2042, 1e-7), ("HFE", 2043, 1e-7)])
+@pytest.mark.parametrize("header,seed,tol", [("H500", 2041, 1e-9), ("H500CORE", 2042, 1e-7), ("HFE", 2043, 1e-7),
+                                             ("H6W", 2046, 1e-9)])
 def test_full_gradient_matches_central_differences(tmp_path, header, seed, tol):
     say("fit: dL/dc vs central differences,", header)
     load(tmp_path, header, seed)
@@ -102,7 +103,7 @@ def test_full_gradient_matches_central_differences(tmp_path, header, seed, tol):
     def central(d):
         # linear embedding: L is linear in c and one central difference is exact for any step;
         # nonlinear embeddings: Richardson extrapolation of steps h and h/2 (error O(h^4))
-        if header == "H500":
+        if header in ("H500", "H6W"):
             h = 1e-2
             return (loss(batch, c0 + h * d, gE, gF) - loss(batch, c0 - h * d, gE, gF)) / (2 * h)
         h = 1e-5

@@ -207,3 +207,32 @@ def test_gradient_records_equal_the_monomial_derivative(hname, tmp_path):
         got[x[c]] += np.sum(rw[r] * Z[ridx[r, 0]] * Z[ridx[r, 1]] * Z[ridx[r, 2]])
     scale = np.abs(want[:nz]).max()
     assert np.abs(got[:nz] - want[:nz]).max() <= 1e-12 * scale
+
+
+def test_tuning_table_applied_at_init(tmp_path):
+    """The tuning table compiled into the library (csrc/tuning_table.inc, written by
+    mag-ace_amd/tune.py) sets the density variants of a listed header at init; an unlisted
+    header keeps the built-in rule; magace_set_tuning validates its values."""
+    import re
+    from helpers import HEADERS
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "mag-ace_amd", "csrc",
+                       "tuning_table.inc")
+    rows = {}
+    for line in open(inc):
+        m = re.match(r"\{\{([-\d, ]+)\}, \{(-?\d+), (\d+)\}, \{(-?\d+), (\d+)\}\},\s*//\s*(\w+)", line.strip())
+        if m:
+            rows[m.group(6)] = (tuple(int(v) for v in m.group(1).split(",")), (int(m.group(2)), int(m.group(3))),
+                                (int(m.group(4)), int(m.group(5))))
+    keys = ("nb_a", "nb_m", "nr2_m", "l2m", "nr3_a", "nr3_m", "l3a", "l3m", "nr4_a", "nr4_m", "l4a", "l4m")
+    for name, (hdr, v64, v32) in rows.items():
+        assert hdr == tuple(HEADERS[name][k] for k in keys), name
+        write_potential(str(tmp_path / (name + ".in")), name, 3)
+        capi.init_file(str(tmp_path / (name + ".in")))
+        assert capi.get_tuning() == (v64, v32), name
+    write_potential(str(tmp_path / "m0.in"), "H500M0", 3)  # not a tuned header
+    capi.init_file(str(tmp_path / "m0.in"))
+    assert capi.get_tuning() == ((-1, 0), (-1, 0))
+    capi.set_tuning(fp64=(1, 2), mixed=(2, 3))
+    assert capi.get_tuning() == ((1, 2), (2, 3))
+    with pytest.raises(capi.MagaceError):
+        capi.set_tuning(fp64=(3, 0))
