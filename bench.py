@@ -288,7 +288,7 @@ def main():
                             stream=stream)
 
     res = {}
-    for prec, mode in (("fp64", capi.FP64), ("mixed", capi.MIXED)):
+    for prec, mode in (("fp64", capi.FP64), ("mixed", capi.MIXED), ("mixed_acc", capi.MIXED_ACC)):
         capi.set_precision(mode)
         outs = alloc()
         for _ in range(args.warmup):
@@ -379,7 +379,8 @@ def main():
         except Exception as e:  # the headline line must still be printed
             log("cpu baseline failed:", e)
         try:
-            orc = oracle_errors(pot, a, {"mixed": o32, "fp64": o64}, min(2000, N))
+            orc = oracle_errors(pot, a, {"mixed": o32, "fp64": o64, "mixed_acc": res["mixed_acc"]["out"]},
+                                min(2000, N))
             log("errors vs the oracle:", orc)
         except Exception as e:
             log("oracle comparison failed:", e)
@@ -404,6 +405,9 @@ def main():
         "fp64": {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms_per_step"],
                  "kernel_ms": res["fp64"]["kernel_ms"]},
         "mixed_over_fp64": res["mixed"]["value"] / res["fp64"]["value"],
+        "mixed_acc": {"value": res["mixed_acc"]["value"], "ms_per_step": res["mixed_acc"]["ms_per_step"],
+                      "kernel_ms": res["mixed_acc"]["kernel_ms"],
+                      "note": "mixed with Kahan-compensated A sums and FP64 rho accumulation (MAGACE_MIXED_ACC)"},
         "mixed_error_vs_fp64": err,
         "kernel_ms": dict(zip(kernels, km)),
         "flops_per_atom_step": fm,

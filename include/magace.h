@@ -24,6 +24,10 @@ extern "C" {
 /* Precision modes */
 #define MAGACE_FP64 0  /* FP64 everywhere; agrees with libace to ~1e-14 relative */
 #define MAGACE_MIXED 1 /* FP32 splines/Y_lm/A/B products, FP64 rho, energy, outputs */
+/* MAGACE_MIXED with the accumulation recipe: the A and A2 sums over neighbours with Kahan
+ * compensation, and rho = sum_k c_k B_k accumulated in FP64 over the (FP32) monomial products
+ * with FP64 weights; slower than MAGACE_MIXED, errors against FP64 reported by bench.py */
+#define MAGACE_MIXED_ACC 2
 
 /* Replaces libace acelibini_ (src/ace.F90:10-75; caller lammps/pair_spin_ace.cpp:61,288).
  * nele must be 1; elenumbers[0] = atomic number + 1; xcnumber 1 = pbe, 2 = lda; reads

@@ -13,7 +13,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libmagace.so")
 
-FP64, MIXED = 0, 1
+FP64, MIXED, MIXED_ACC = 0, 1, 2
 
 _D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _I = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")

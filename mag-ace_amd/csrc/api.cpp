@@ -315,8 +315,13 @@ int upload_records(Context& c) {
         std::memcpy(&fb, &wf, 4);
         r64[i] = make_uint4(((uint32_t)q[0] << 4) | ((uint32_t)q[1] << 20), (uint32_t)q[2] << 4,
                             (uint32_t)(wb & 0xffffffffu), (uint32_t)(wb >> 32));
-        r32[i] = make_uint4(((uint32_t)q[0] << 3) | ((uint32_t)q[1] << 19), (uint32_t)q[2] << 3, fb, 0u);
-        r16[i] = make_uint4(((uint32_t)q[0] << 4) | ((uint32_t)q[1] << 20), (uint32_t)q[2] << 4, fb, 0u);
+        // the FP32 records carry the weight's low part too (w - (float)w), read by the
+        // accumulation recipe's FP64 sums (MAGACE_MIXED_ACC)
+        const float lo = (float)(w - (double)wf);
+        uint32_t lb;
+        std::memcpy(&lb, &lo, 4);
+        r32[i] = make_uint4(((uint32_t)q[0] << 3) | ((uint32_t)q[1] << 19), (uint32_t)q[2] << 3, fb, lb);
+        r16[i] = make_uint4(((uint32_t)q[0] << 4) | ((uint32_t)q[1] << 20), (uint32_t)q[2] << 4, fb, lb);
     }
     HIPCHK(hipMemcpy(c.crec64, r64.data(), nc * sizeof(uint4), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c.crec32, r32.data(), nc * sizeof(uint4), hipMemcpyHostToDevice));
@@ -332,6 +337,7 @@ int upload_records(Context& c) {
         HIPCHK(hipMemcpy(c.cw2_32, w32.data(), nc * sizeof(float), hipMemcpyHostToDevice));
         c.P64.cw2 = c.cw2_64;
         c.P32.cw2 = c.cw2_32;
+        c.P32.cw2d = c.cw2_64;
     }
     return 0;
 }
@@ -476,8 +482,9 @@ hipError_t launch(Context& c, int natom, int atom_base, const int* off, const do
     if (c.precision == MAGACE_FP64)
         return Launch<double>::run(c.P64, natom, atom_base, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist,
                                    tlist, tlist0, B, s, c.shape_class, ev, c.dvar[0]);
+    // MAGACE_MIXED_ACC runs the FP32 kernels with the accumulation recipe and the mixed tuning
     return Launch<float>::run(c.P32, natom, atom_base, off, dlist, elist, dmomi, emomi, dmom, emom, eout, flist,
-                              tlist, tlist0, B, s, c.shape_class, ev, c.dvar[1]);
+                              tlist, tlist0, B, s, c.shape_class, ev, c.dvar[1], c.precision == MAGACE_MIXED_ACC);
 }
 
 int run_device(Context& c, int natom, const int* off, const double* dlist, const double* elist,
@@ -707,8 +714,8 @@ int magace_finalize(void) {
 }
 
 int magace_set_precision(int mode) {
-    if (mode != MAGACE_FP64 && mode != MAGACE_MIXED) {
-        set_error(MAGACE_ERR_INPUT, "precision must be MAGACE_FP64 or MAGACE_MIXED");
+    if (mode != MAGACE_FP64 && mode != MAGACE_MIXED && mode != MAGACE_MIXED_ACC) {
+        set_error(MAGACE_ERR_INPUT, "precision must be MAGACE_FP64, MAGACE_MIXED or MAGACE_MIXED_ACC");
         return MAGACE_ERR_INPUT;
     }
     if (!g_ctx) {

@@ -94,6 +94,16 @@ __device__ __forceinline__ cplx<T> cconj_sign(cplx<T> a, int m) {  // (-1)^m con
     return (m & 1) ? cplx<T>{-a.re, a.im} : cplx<T>{a.re, -a.im};
 }
 
+// Kahan step: s += x, with c holding minus the part of the running sum that s has lost
+// (the compensated sum is s - c)
+template <typename T>
+__device__ __forceinline__ void kahan_add(T& s, T& c, T x) {
+    const T y = x - c;
+    const T t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
 // cubic Hermite row entry: value and derivative (d/dr) at fractional position w
 template <typename T>
 __device__ __forceinline__ void horner(const T* __restrict__ c, T w, T rs, T& v, T& dv) {
@@ -260,8 +270,10 @@ __device__ __forceinline__ void store_a2_half(const DevParams<T>& P, const cplx<
     }
 }
 
-template <typename T, int L, int CH, int NB, int NM3X, int NB2, int NR2X, bool SPLIT, bool TASKSPLIT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6 && NM3X <= 4 ? (sizeof(T) == 4 ? 5 : 3) : (sizeof(T) == 4 ? 3 : 2), 8))) k_density(DevParams<T> P, const int* __restrict__ off,
+// ACC (FP32 only): A and A2 summed with Kahan compensation (the accumulation recipe of
+// MAGACE_MIXED_ACC); twice the accumulator registers, so one wave per SIMD less
+template <typename T, int L, int CH, int NB, int NM3X, int NB2, int NR2X, bool SPLIT, bool TASKSPLIT, bool ACC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6 && NM3X <= 4 ? (sizeof(T) == 4 ? (ACC ? 3 : 5) : 3) : (sizeof(T) == 4 && !ACC ? 3 : 2), 8))) k_density(DevParams<T> P, const int* __restrict__ off,
                                                 const double* __restrict__ dlist,
                                                 const double* __restrict__ elist,
                                                 const double* __restrict__ dmomi,
@@ -357,6 +369,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
         for (int a = 0; a < (NB > NB2 ? NB : NB2); ++a)
 #pragma unroll
             for (int b = 0; b < (NM3X > NR2X ? NM3X : NR2X); ++b) acc[a][b] = {0, 0};
+        constexpr int kCa = ACC ? (NB > NB2 ? NB : NB2) : 1, kCq = ACC ? (NM3X > NR2X ? NM3X : NR2X) : 1;
+        cplx<T> cmp[kCa][kCq];
+#pragma unroll
+        for (int a = 0; a < kCa; ++a)
+#pragma unroll
+            for (int b = 0; b < kCq; ++b) cmp[a][b] = {0, 0};
         // decode task
         // only light headers split tasks; the others keep a compile-time SA = 1
         constexpr bool kSplit = TASKSPLIT;
@@ -523,8 +541,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                         const T r = rr[b];
 #pragma unroll
                         for (int q = 0; q < NM3X; ++q) {
-                            acc[b][q].re = fma(r, yh[q].re, acc[b][q].re);
-                            acc[b][q].im = fma(r, yh[q].im, acc[b][q].im);
+                            if constexpr (ACC) {
+                                kahan_add(acc[b][q].re, cmp[ACC ? b : 0][ACC ? q : 0].re, r * yh[q].re);
+                                kahan_add(acc[b][q].im, cmp[ACC ? b : 0][ACC ? q : 0].im, r * yh[q].im);
+                            } else {
+                                acc[b][q].re = fma(r, yh[q].re, acc[b][q].re);
+                                acc[b][q].im = fma(r, yh[q].im, acc[b][q].im);
+                            }
                         }
                     }
                 }
@@ -543,12 +566,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                         const T g = gr[b];
 #pragma unroll
                         for (int q = 0; q < NR2X; ++q) {
-                            acc[b][q].re = fma(g, my[q].re, acc[b][q].re);
-                            acc[b][q].im = fma(g, my[q].im, acc[b][q].im);
+                            if constexpr (ACC) {
+                                kahan_add(acc[b][q].re, cmp[ACC ? b : 0][ACC ? q : 0].re, g * my[q].re);
+                                kahan_add(acc[b][q].im, cmp[ACC ? b : 0][ACC ? q : 0].im, g * my[q].im);
+                            } else {
+                                acc[b][q].re = fma(g, my[q].re, acc[b][q].re);
+                                acc[b][q].im = fma(g, my[q].im, acc[b][q].im);
+                            }
                         }
                     }
                 }
             }
+        }
+        if constexpr (ACC) {  // the compensated sums
+#pragma unroll
+            for (int a = 0; a < kCa; ++a)
+#pragma unroll
+                for (int b = 0; b < kCq; ++b) {
+                    acc[a][b].re -= cmp[a][b].re;
+                    acc[a][b].im -= cmp[a][b].im;
+                }
         }
         if (kSplit && sa > 1) {  // one round only: every thread of the block reaches these barriers
             cplx<T>* scr = (cplx<T>*)smem;  // [SA - 1][nTA][NB][NM3X]
@@ -723,7 +760,10 @@ __device__ __forceinline__ void combine_partials(const P_t& P, V* __restrict__ p
 
 // ND2: two densities (rho_d = sum_k c_{k,d} B_k, E = fc sum_d F(rho_d) + e_hc,
 // src/forces.F90:976-1089): a second weight per record, a second gradient array in LDS
-template <typename T, bool ND2>
+// ACC (FP32 only, MAGACE_MIXED_ACC): rho = sum of the A0 targets' gradient records summed in
+// FP64 (double-float record weights: the FP32 weight in z plus its low part in w); in every
+// round where some lane of the wave has an A0 target, the whole wave accumulates in FP64
+template <typename T, bool ND2, bool ACC>
 __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int atom_base,
                                                 const cplx<T>* __restrict__ Zin,
                                                 cplx<T>* __restrict__ Wout, char* __restrict__ Pt,
@@ -769,6 +809,8 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int
     const int nj = (P.nchunks + nthr - 1) / nthr;
     const int rot = nj > 0 ? (int)((unsigned)(atom_base + atom) % (unsigned)nj) : 0;
     cplx<T> acc = {0, 0}, acc2 = {0, 0}, head = {0, 0}, head2 = {0, 0};
+    cplx<double> acd = {0, 0}, acd2 = {0, 0};  // ACC: the run's FP64 rounds
+    double rho = 0.0, rho2 = 0.0;              // ACC: summed at the flushes
     int j = rot;
     int2 ch = tid + j * nthr < P.nchunks ? P.chunks[tid + j * nthr] : make_int2(0, 0);
     bool split = false;
@@ -785,6 +827,31 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int
             const bool sole = (ch.y >> 18) & 1;  // the target's only chunk
             const uint4* __restrict__ rec = P.crec + ch.x;  // records of this lane, stride 64
             const T* __restrict__ w2p = ND2 ? P.cw2 + ch.x : nullptr;
+            const double* __restrict__ w2d = ND2 && ACC ? P.cw2d + ch.x : nullptr;
+            if (ACC && __any(x >= P.oA0)) {
+                for (int k = 0; k < cnt; k += 4) {
+                    uint4 r[4];
+                    double w2[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        r[u] = rec[(size_t)(k + u) * 64];
+                        if (ND2) w2[u] = w2d[(size_t)(k + u) * 64];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const double w = (double)__uint_as_float(r[u].z) + (double)__uint_as_float(r[u].w);
+                        const char* zb0 = (const char*)sZ;
+                        const cplx<T> pr = cmul(cmul(lds_cplx<T>(zb0, r[u].x & 0xffff), lds_cplx<T>(zb0, r[u].x >> 16)),
+                                                lds_cplx<T>(zb0, r[u].y));
+                        acd.re = fma(w, (double)pr.re, acd.re);
+                        acd.im = fma(w, (double)pr.im, acd.im);
+                        if (ND2) {
+                            acd2.re = fma(w2[u], (double)pr.re, acd2.re);
+                            acd2.im = fma(w2[u], (double)pr.im, acd2.im);
+                        }
+                    }
+                }
+            } else
             for (int k = 0; k < cnt; k += 4) {
                 uint4 r[4];
                 T w2[4];
@@ -810,6 +877,19 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int
                 }
             }
             if (!(jn == j + 1 && has_next && ((chn.y ^ ch.y) & 0xfff) == 0)) {
+                if constexpr (ACC) {  // the run's FP64 and FP32 rounds; rho of an A0 target in FP64
+                    const cplx<double> t = {acd.re + (double)acc.re, acd.im + (double)acc.im};
+                    const cplx<double> t2 = {acd2.re + (double)acc2.re, acd2.im + (double)acc2.im};
+                    if (x >= P.oA0) {
+                        const cplx<T> z = sZ[x];
+                        rho += (double)z.re * t.re - (double)z.im * t.im;
+                        if (ND2) rho2 += (double)z.re * t2.re - (double)z.im * t2.im;
+                    }
+                    acc = {(T)t.re, (T)t.im};
+                    acc2 = {(T)t2.re, (T)t2.im};
+                    acd = {0, 0};
+                    acd2 = {0, 0};
+                }
                 if (hold) {  // first part of the run split by the start of the walk
                     head = acc;
                     head2 = acc2;
@@ -845,12 +925,12 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int
     }
     __syncthreads();
     // rho = Re sum over A0 targets of Z[x] dRho/dZ[x] (every monomial has exactly one A0
-    // factor), summed in FP64 in a fixed order
-    double rho = 0.0, rho2 = 0.0;
-    for (int x = P.oA0 + tid; x < P.nz; x += nthr) {
-        rho += (double)re_mul(sZ[x], sW[x]);
-        if (ND2) rho2 += (double)re_mul(sZ[x], sW2[x]);
-    }
+    // factor), summed in FP64 in a fixed order (with ACC: already summed at the flushes)
+    if constexpr (!ACC)
+        for (int x = P.oA0 + tid; x < P.nz; x += nthr) {
+            rho += (double)re_mul(sZ[x], sW[x]);
+            if (ND2) rho2 += (double)re_mul(sZ[x], sW2[x]);
+        }
     const double tot = block_sum<T>(rho, red);
     double tot2 = 0.0;
     if (ND2) {
@@ -890,6 +970,7 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int
 // each record (byte offsets of 16-byte entries, FP32 weight) is loaded and decoded once for
 // both atoms.  Same chunk walk, carry, sole stores and rotation as k_adjoint; an odd last
 // atom is paired with itself and its second copy is not written.
+template <bool ACC>
 __global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<float> P, int atom_base,
                                                      const uint4* __restrict__ crec16,
                                                      const cplx<float>* __restrict__ Zin,
@@ -924,6 +1005,8 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<floa
     const int nj = (P.nchunks + nthr - 1) / nthr;
     const int rot = nj > 0 ? (int)((unsigned)((atom_base >> 1) + (int)blockIdx.x) % (unsigned)nj) : 0;
     cplx<T> acc0 = {0, 0}, acc1 = {0, 0}, head0 = {0, 0}, head1 = {0, 0};
+    cplx<double> acd0 = {0, 0}, acd1 = {0, 0};  // ACC: the run's FP64 rounds (k_adjoint's ACC)
+    double rho0 = 0.0, rho1s = 0.0;             // ACC: summed at the flushes
     int j = rot;
     int2 ch = tid + j * nthr < P.nchunks ? P.chunks[tid + j * nthr] : make_int2(0, 0);
     bool split = false;
@@ -940,6 +1023,26 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<floa
             const int x = ch.y & 0xfff, cnt = (ch.y >> 12) & 63;
             const bool sole = (ch.y >> 18) & 1;
             const uint4* __restrict__ rec = crec16 + ch.x;
+            if (ACC && __any(x >= P.oA0)) {
+                for (int k = 0; k < cnt; k += 4) {
+                    uint4 r[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) r[u] = rec[(size_t)(k + u) * 64];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const double w = (double)__uint_as_float(r[u].z) + (double)__uint_as_float(r[u].w);
+                        const float4 za = *(const float4*)(zb + (r[u].x & 0xffff));
+                        const float4 zbv = *(const float4*)(zb + (r[u].x >> 16));
+                        const float4 zc = *(const float4*)(zb + r[u].y);
+                        const cplx<T> p0 = cmul(cmul(cplx<T>{za.x, za.y}, cplx<T>{zbv.x, zbv.y}), cplx<T>{zc.x, zc.y});
+                        const cplx<T> p1 = cmul(cmul(cplx<T>{za.z, za.w}, cplx<T>{zbv.z, zbv.w}), cplx<T>{zc.z, zc.w});
+                        acd0.re = fma(w, (double)p0.re, acd0.re);
+                        acd0.im = fma(w, (double)p0.im, acd0.im);
+                        acd1.re = fma(w, (double)p1.re, acd1.re);
+                        acd1.im = fma(w, (double)p1.im, acd1.im);
+                    }
+                }
+            } else
             for (int k = 0; k < cnt; k += 4) {
                 uint4 r[4];
 #pragma unroll
@@ -959,6 +1062,19 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<floa
                 }
             }
             if (!(jn == j + 1 && has_next && ((chn.y ^ ch.y) & 0xfff) == 0)) {
+                if constexpr (ACC) {
+                    const cplx<double> t0 = {acd0.re + (double)acc0.re, acd0.im + (double)acc0.im};
+                    const cplx<double> t1 = {acd1.re + (double)acc1.re, acd1.im + (double)acc1.im};
+                    if (x >= P.oA0) {
+                        const float4 z = sZ[x];
+                        rho0 += (double)z.x * t0.re - (double)z.y * t0.im;
+                        rho1s += (double)z.z * t1.re - (double)z.w * t1.im;
+                    }
+                    acc0 = {(T)t0.re, (T)t0.im};
+                    acc1 = {(T)t1.re, (T)t1.im};
+                    acd0 = {0, 0};
+                    acd1 = {0, 0};
+                }
                 if (hold) {
                     head0 = acc0;
                     head1 = acc1;
@@ -986,12 +1102,12 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<floa
     __syncthreads();
     combine_partials(P, part, sW);
     __syncthreads();
-    double rho0 = 0.0, rho1s = 0.0;
-    for (int x = P.oA0 + tid; x < P.nz; x += nthr) {
-        const float4 z = sZ[x], w = sW[x];
-        rho0 += (double)re_mul(cplx<T>{z.x, z.y}, cplx<T>{w.x, w.y});
-        rho1s += (double)re_mul(cplx<T>{z.z, z.w}, cplx<T>{w.z, w.w});
-    }
+    if constexpr (!ACC)
+        for (int x = P.oA0 + tid; x < P.nz; x += nthr) {
+            const float4 z = sZ[x], w = sW[x];
+            rho0 += (double)re_mul(cplx<T>{z.x, z.y}, cplx<T>{w.x, w.y});
+            rho1s += (double)re_mul(cplx<T>{z.z, z.w}, cplx<T>{w.z, w.w});
+        }
     const double tot0 = block_sum<T>(rho0, red);
     __syncthreads();
     const double tot1 = block_sum<T>(rho1s, red + kAdjointThreads / 64);
@@ -1912,7 +2028,8 @@ int supported_nm3_class(int nm3, int nr2m, int nb_a, int l3a, int l2m, int nr3a)
     return -1;
 }
 
-template <typename T, int L, int CH, bool SPLIT, bool TASKSPLIT, int NB, int NM3X, int NB2, int NR2X, int NRAX, int NBAX>
+template <typename T, int L, int CH, bool SPLIT, bool TASKSPLIT, int NB, int NM3X, int NB2, int NR2X, int NRAX, int NBAX,
+          bool ACC>
 static hipError_t run_class(const DevParams<T>& P, int natom, int atom_base, const int* off, const double* dlist,
                             const double* elist, const double* dmomi, const double* emomi,
                             const double* dmom, const double* emom, double* eout, double* flist,
@@ -1921,9 +2038,9 @@ static hipError_t run_class(const DevParams<T>& P, int natom, int atom_base, con
     Pd.nr3a = P.nr3a; Pd.l3a = P.l3a; Pd.nr2m = P.nr2m; Pd.l2m = P.l2m; Pd.lha = P.lha;
     Pd.lhm = P.lhm; Pd.nm3 = P.nm3; Pd.nb_a = P.nb_a; Pd.nb_m = P.nb_m;
     const size_t lds1 = density_lds_bytes(Pd, CH, sizeof(T));
-    auto k1 = k_density<T, L, CH, NB, NM3X, NB2, NR2X, SPLIT, TASKSPLIT>;
+    static_assert(!ACC || sizeof(T) == 4, "the accumulation recipe belongs to the FP32 kernels");
+    auto k1 = k_density<T, L, CH, NB, NM3X, NB2, NR2X, SPLIT, TASKSPLIT, ACC>;
     const bool nd2 = P.nd == 2;
-    const void* k2 = nd2 ? (const void*)k_adjoint<T, true> : (const void*)k_adjoint<T, false>;
     // the Q1 instantiation exists only for the four-column class
     constexpr bool kQ1 = NM3X == 4;
     const bool q1 = kQ1 && P.nm3 == 4 && P.lsm3 == 4;
@@ -1944,8 +2061,8 @@ static hipError_t run_class(const DevParams<T>& P, int natom, int atom_base, con
         if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
         if ((e = hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev)) != hipSuccess)
             return e;
-        const void* ks[6] = {(const void*)k1, (const void*)k_adjoint<T, false>, (const void*)k3q,
-                             (const void*)k_adjoint<T, true>, (const void*)k3g, (const void*)k3q4};
+        const void* ks[6] = {(const void*)k1, (const void*)k_adjoint<T, false, ACC>, (const void*)k3q,
+                             (const void*)k_adjoint<T, true, ACC>, (const void*)k3g, (const void*)k3q4};
         for (int i = 0; i < 6; ++i) {
             // limit[5] belongs to k_adjoint_pair below
             const int li = i < 5 ? i : 6;
@@ -1958,9 +2075,9 @@ static hipError_t run_class(const DevParams<T>& P, int natom, int atom_base, con
         }
         if constexpr (sizeof(T) == 4) {
             hipFuncAttributes fa;
-            if ((e = hipFuncGetAttributes(&fa, (const void*)k_adjoint_pair)) != hipSuccess) return e;
+            if ((e = hipFuncGetAttributes(&fa, (const void*)k_adjoint_pair<ACC>)) != hipSuccess) return e;
             limit[5] = (size_t)maxsh > fa.sharedSizeBytes ? (size_t)maxsh - fa.sharedSizeBytes : 0;
-            if ((e = hipFuncSetAttribute((const void*)k_adjoint_pair, hipFuncAttributeMaxDynamicSharedMemorySize,
+            if ((e = hipFuncSetAttribute((const void*)k_adjoint_pair<ACC>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)limit[5])) != hipSuccess)
                 return e;
         }
@@ -1982,16 +2099,16 @@ static hipError_t run_class(const DevParams<T>& P, int natom, int atom_base, con
     if (ev) (void)hipEventRecord(ev[1], s);
     if constexpr (sizeof(T) == 4) {
         if (pair)
-            hipLaunchKernelGGL(k_adjoint_pair, dim3((natom + 1) / 2), dim3(kAdjointThreads), lds2p, s, P, atom_base,
+            hipLaunchKernelGGL(k_adjoint_pair<ACC>, dim3((natom + 1) / 2), dim3(kAdjointThreads), lds2p, s, P, atom_base,
                                P.crec16, (const cplx<T>*)B.Z, (cplx<T>*)B.W, (char*)B.Pt, B.ehc, B.rho1, eout, B.dE,
                                B.edf, B.rho, natom);
     }
     if (pair) {
     } else if (nd2)
-        hipLaunchKernelGGL((k_adjoint<T, true>), dim3(natom), dim3(kAdjointThreads), lds2, s, P, atom_base,
+        hipLaunchKernelGGL((k_adjoint<T, true, ACC>), dim3(natom), dim3(kAdjointThreads), lds2, s, P, atom_base,
                            (const cplx<T>*)B.Z, (cplx<T>*)B.W, (char*)B.Pt, B.ehc, B.rho1, eout, B.dE, B.edf, B.rho);
     else
-        hipLaunchKernelGGL((k_adjoint<T, false>), dim3(natom), dim3(kAdjointThreads), lds2, s, P, atom_base,
+        hipLaunchKernelGGL((k_adjoint<T, false, ACC>), dim3(natom), dim3(kAdjointThreads), lds2, s, P, atom_base,
                            (const cplx<T>*)B.Z, (cplx<T>*)B.W, (char*)B.Pt, B.ehc, B.rho1, eout, B.dE, B.edf, B.rho);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[2], s);
@@ -2006,7 +2123,7 @@ hipError_t Launch<T>::run(const DevParams<T>& P, int natom, int atom_base, const
                           const double* elist, const double* dmomi, const double* emomi,
                           const double* dmom, const double* emom, double* eout, double* flist,
                           double* tlist, double* tlist0, Buffers& B, hipStream_t s, int cls,
-                          hipEvent_t* ev, int dvar) {
+                          hipEvent_t* ev, int dvar, bool acc) {
     if (natom <= 0) return hipSuccess;
     // Density staging: with many radial functions per pair (lmax-6 headers) or in FP64 the
     // per-pair staging is spread over the whole block, 32 neighbours per round; light FP32
@@ -2015,22 +2132,26 @@ hipError_t Launch<T>::run(const DevParams<T>& P, int natom, int atom_base, const
     // also split their A-column tasks over idle threads (lmax 4: FP32 0.61 -> 0.52 ms, FP64
     // 0.93 -> 0.81 ms; with lmax 6 the runtime split costs FP64 3%).
     const bool heavy = P.nb_a + P.nr3a * (P.l3a + 1) >= 40;
-#define MAGACE_RUN(CH, SPLIT, TS)                                                                          \
-    (cls == 0   ? run_class<T, 6, CH, SPLIT, TS, 4, 4, 4, 2, 8, 10>(P, natom, atom_base, off, dlist, elist, dmomi,  \
-                                                                    emomi, dmom, emom, eout, flist, tlist, tlist0, B, \
-                                                                    s, ev)                                            \
-     : cls == 1 ? run_class<T, 2, CH, SPLIT, TS, 1, 27, 2, 5, 4, 16>(P, natom, atom_base, off, dlist, elist, dmomi, \
-                                                                     emomi, dmom, emom, eout, flist, tlist, tlist0, B, \
-                                                                     s, ev)                                           \
-                : run_class<T, 6, CH, SPLIT, TS, 4, 9, 4, 5, 8, 16>(P, natom, atom_base, off, dlist, elist, dmomi, \
-                                                                    emomi, dmom, emom, eout, flist, tlist, tlist0, B, \
-                                                                    s, ev))
+#define MAGACE_RUN_ACC(CH, SPLIT, TS, ACC)                                                                     \
+    (cls == 0   ? run_class<T, 6, CH, SPLIT, TS, 4, 4, 4, 2, 8, 10, ACC>(P, natom, atom_base, off, dlist, elist,      \
+                                                                         dmomi, emomi, dmom, emom, eout, flist,      \
+                                                                         tlist, tlist0, B, s, ev)                    \
+     : cls == 1 ? run_class<T, 2, CH, SPLIT, TS, 1, 27, 2, 5, 4, 16, ACC>(P, natom, atom_base, off, dlist, elist,     \
+                                                                          dmomi, emomi, dmom, emom, eout, flist,     \
+                                                                          tlist, tlist0, B, s, ev)                   \
+                : run_class<T, 6, CH, SPLIT, TS, 4, 9, 4, 5, 8, 16, ACC>(P, natom, atom_base, off, dlist, elist,      \
+                                                                         dmomi, emomi, dmom, emom, eout, flist,      \
+                                                                         tlist, tlist0, B, s, ev))
+    // the accumulation recipe (MAGACE_MIXED_ACC) exists for the FP32 kernels only
+#define MAGACE_RUN(CH, SPLIT, TS) \
+    (sizeof(T) == 4 && acc ? MAGACE_RUN_ACC(CH, SPLIT, TS, (sizeof(T) == 4)) : MAGACE_RUN_ACC(CH, SPLIT, TS, false))
     // a density variant chosen by the tuner (kDensityVariants) replaces the rule above
     if (dvar < 0) dvar = heavy ? 0 : (sizeof(T) == 8 ? 1 : 2);
     if (dvar == 0) return MAGACE_RUN(32, true, false);
     if (dvar == 1) return MAGACE_RUN(32, true, true);
     return MAGACE_RUN(64, false, true);
 #undef MAGACE_RUN
+#undef MAGACE_RUN_ACC
 }
 
 template struct Launch<double>;

@@ -49,6 +49,7 @@ struct DevParams {
     int nfemb = 0;
     int nd = 1;                  // densities (1 or 2)
     const T* cw2 = nullptr;      // record weights of density 2 (same order as crec)
+    const double* cw2d = nullptr;  // the same in FP64 (FP32 kernels' accumulation recipe)
     double p1[2][2] = {{1, 1}, {1, 1}};  // per density: embedding prefactor, exponent
     double ecut = 0, decut = 1;
     double c1[2][32] = {{0}};    // per density: coefficients of B1_n = g^m_n(mu_i) - 1
@@ -92,7 +93,7 @@ struct Launch {
                           const double* dlist, const double* elist, const double* dmomi,
                           const double* emomi, const double* dmom, const double* emom, double* eout,
                           double* flist, double* tlist, double* tlist0, Buffers& B, hipStream_t s,
-                          int nm3_class, hipEvent_t* ev, int dvar = -1);
+                          int nm3_class, hipEvent_t* ev, int dvar = -1, bool acc = false);
 };
 
 // fitting (FP64): forward-mode density, per-atom adjoint scalars, monomial accumulation

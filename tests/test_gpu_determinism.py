@@ -54,7 +54,7 @@ def same(x, y):
 
 
 @pytest.mark.parametrize("header", ["H500", "H2000"])
-@pytest.mark.parametrize("prec", [capi.FP64, capi.MIXED])
+@pytest.mark.parametrize("prec", [capi.FP64, capi.MIXED, capi.MIXED_ACC])
 def test_bitwise_repeatable_across_launches_and_splits(header, prec, tmp_path):
     pot = write_potential(str(tmp_path / "p.in"), header, 2027)
     capi.init_file(pot)

@@ -40,12 +40,13 @@ def test_fp64_matches_reference(name, tmp_path):
         assert e <= 1e-10, (k, e)
 
 
+@pytest.mark.parametrize("mode", [capi.MIXED, capi.MIXED_ACC])
 @pytest.mark.parametrize("name", CASES)
-def test_mixed_precision_error(name, tmp_path):
-    say("case", name, "mixed")
+def test_mixed_precision_error(name, mode, tmp_path):
+    say("case", name, "mixed" if mode == capi.MIXED else "mixed with the accumulation recipe")
     c = load_case(name, tmp_path)
     capi.init_file(c["pot"])
-    capi.set_precision(capi.MIXED)
+    capi.set_precision(mode)
     got = capi.acelib(c["a"])
     capi.set_precision(capi.FP64)
     de, df, dm = per_atom_errors(c["a"], got, c["ref"])
@@ -181,10 +182,15 @@ def test_c2_fp64_and_mixed_against_oracle_per_output(tmp_path):
         e = relerr(g64[k], ref[k])
         say("  fp64 %-6s rel %.2e" % (k, e))
         assert e <= 1e-10, k
-    per = {k: float(np.abs(g32[k] - ref[k]).max()) for k in ("eout", "flist", "tlist", "tlist0")}
-    de, df, dm = per_atom_errors(a, g32, ref)
-    say("  mixed vs oracle: E_i %.2e eV  F_p %.2e eV/A  T_p %.2e eV/muB  T0_i %.2e eV/muB  "
-        "F_i %.2e eV/A  fm_i %.2e eV/muB" % (per["eout"], per["flist"], per["tlist"], per["tlist0"], df, dm))
-    for k, v in per.items():
-        assert v <= 1e-4, (k, v)
-    assert de <= 1e-4 and df <= 1e-4 and dm <= 1e-4
+    capi.set_precision(capi.MIXED_ACC)
+    gacc = capi.acelib(a)
+    capi.set_precision(capi.FP64)
+    for label, g in (("mixed", g32), ("mixed+acc", gacc)):
+        per = {k: float(np.abs(g[k] - ref[k]).max()) for k in ("eout", "flist", "tlist", "tlist0")}
+        de, df, dm = per_atom_errors(a, g, ref)
+        say("  %s vs oracle: E_i %.2e eV  F_p %.2e eV/A  T_p %.2e eV/muB  T0_i %.2e eV/muB  "
+            "F_i %.2e eV/A  fm_i %.2e eV/muB" % (label, per["eout"], per["flist"], per["tlist"], per["tlist0"], df,
+                                                  dm))
+        for k, v in per.items():
+            assert v <= 1e-4, (label, k, v)
+        assert de <= 1e-4 and df <= 1e-4 and dm <= 1e-4
