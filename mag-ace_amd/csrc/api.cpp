@@ -115,6 +115,7 @@ struct Context {
     size_t fit_cap_pairs = 0;
     double *f_fl = nullptr, *f_tl = nullptr, *f_t0 = nullptr, *f_eout = nullptr;
     size_t fit_cap_atoms = 0;
+    size_t fit_cap_wd = 0;     // atoms that fit.Wd holds
     int fit_natom = -1;
     // per-kernel timing (magace_profile): 4 events per evaluation
     bool profiling = false;
@@ -182,7 +183,7 @@ Context::~Context() {
     dfree(buf.Pt);
     dfree(buf.err);
     dfree(fit.Zdot); dfree(fit.ehc_dot); dfree(fit.alpha); dfree(fit.beta); dfree(fit.M); dfree(fit.idx);
-    dfree(fit.ub_ptr); dfree(fit.ub_t); dfree(fit.ub_u);
+    dfree(fit.ub_ptr); dfree(fit.ub_t); dfree(fit.ub_u); dfree(fit.Wd);
     dfree(f_fl); dfree(f_tl); dfree(f_t0); dfree(f_eout);
     for (auto& sl : slots) {
         dfree(sl.d_in);
@@ -471,6 +472,7 @@ Buffers buffers_from(const Buffers& B, size_t a, size_t zbytes, int nd, int nslo
     o.dE = B.dE + a * nd;
     o.edf = B.edf + a;
     o.rho = B.rho + a * nd;
+    if (B.Wd) o.Wd = (char*)B.Wd + a * 2 * zbytes;  // set only for FP64 (fitting)
     o.cap_atoms = B.cap_atoms - a;
     return o;
 }
@@ -889,6 +891,7 @@ int magace_compute_device_noalloc(int natom, int nei, const int* offsets, const 
     }
     const size_t zb = (size_t)c.model.z.nz * (c.precision == MAGACE_FP64 ? 16 : 8);
     Buffers B = c.buf;  // err flag of the context (allocated at set-up)
+    B.Wd = nullptr;
     const int nd = c.model.pot.h.ndensity;
     const size_t cap = carve_scratch((char*)base, scratch_bytes(), zb, nd, partials_bytes(c.model.terms.nslot, nd), &B);
     if ((size_t)natom > cap) {
@@ -1227,10 +1230,6 @@ int magace_fit_forward(int natom, int nei, const int* offsets, const int* nlist,
         return MAGACE_ERR_STATE;
     }
     Context& c = *g_ctx;
-    if (c.model.pot.h.ndensity != 1) {
-        set_error(MAGACE_ERR_INPUT, "fitting supports ndensity 1");
-        return MAGACE_ERR_INPUT;
-    }
     if (natom < 0 || nei < 0 || nstruc < 0 || ((nstruc > 0 || estruct) && !atom_struct) || !forces) {
         set_error(MAGACE_ERR_INPUT, "magace_fit_forward: invalid sizes");
         return MAGACE_ERR_INPUT;
@@ -1251,10 +1250,21 @@ int magace_fit_forward(int natom, int nei, const int* offsets, const int* nlist,
         HIPCHK(hipMalloc((void**)&c.f_eout, cap * sizeof(double)));
         c.fit_cap_atoms = cap;
     }
+    const int nd = c.model.pot.h.ndensity;
+    if (nd == 2 && (size_t)natom > c.fit_cap_wd) {
+        // with two densities the adjoint also writes each density's gradient dRho_d/dZ, which
+        // the backward needs for D_u rho_d
+        dfree(c.fit.Wd);
+        const size_t cap = (size_t)natom + natom / 4 + 16;
+        HIPCHK(hipMalloc(&c.fit.Wd, cap * 2 * c.model.z.nz * 2 * sizeof(double)));
+        c.fit_cap_wd = cap;
+    }
     const int saved = c.precision;
     c.precision = MAGACE_FP64;  // the adjoint runs on the FP64 Z and W of this evaluation
+    c.buf.Wd = nd == 2 ? c.fit.Wd : nullptr;
     const int rc = run_device(c, natom, offsets, dlist, elist, dlist_momi, elist_momi, dlist_mom, elist_mom,
                               c.f_eout, c.f_fl, c.f_tl, c.f_t0, s);
+    c.buf.Wd = nullptr;
     c.precision = saved;
     if (rc) return rc;
     HIPCHK(fit_assemble(natom, offsets, nlist, c.f_fl, c.f_eout, atom_struct, nstruc, forces, estruct, s));
@@ -1284,6 +1294,7 @@ int magace_fit_backward(int natom, int nei, const int* offsets, const int* nlist
     hipStream_t s = (hipStream_t)stream;  // NULL: the default (null) stream
     const TermList& T = c.model.terms;
     FitBuffers& F = c.fit;
+    const size_t nd = (size_t)c.model.pot.h.ndensity;
     if (!F.idx) {
         HIPCHK(hipMalloc((void**)&F.idx, T.idx.size() * sizeof(uint16_t) + 64));
         HIPCHK(hipMemcpy(F.idx, T.idx.data(), T.idx.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
@@ -1293,20 +1304,23 @@ int magace_fit_backward(int natom, int nei, const int* offsets, const int* nlist
         HIPCHK(hipMemcpy(F.ub_ptr, T.ub_ptr.data(), T.ub_ptr.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(F.ub_t, T.ub_t.data(), T.ub_t.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(F.ub_u, T.ub_u.data(), T.ub_u.size() * sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(hipMalloc((void**)&F.M, 2 * T.nterms() * sizeof(double) + 64));
+        HIPCHK(hipMalloc((void**)&F.M, 2 * T.nterms() * nd * sizeof(double) + 64));
     }
     if ((size_t)natom > F.cap_atoms) {
         dfree(F.Zdot); dfree(F.ehc_dot); dfree(F.alpha); dfree(F.beta);
         const size_t cap = (size_t)natom + natom / 4 + 16;
         HIPCHK(hipMalloc(&F.Zdot, cap * c.model.z.nz * 2 * sizeof(double)));
         HIPCHK(hipMalloc((void**)&F.ehc_dot, cap * sizeof(double)));
-        HIPCHK(hipMalloc((void**)&F.alpha, cap * sizeof(double)));
-        HIPCHK(hipMalloc((void**)&F.beta, cap * sizeof(double)));
+        HIPCHK(hipMalloc((void**)&F.alpha, cap * nd * sizeof(double)));
+        HIPCHK(hipMalloc((void**)&F.beta, cap * nd * sizeof(double)));
         F.cap_atoms = cap;
     }
-    const int nb234 = (int)T.ub_ptr.size() - 1;
+    const Basis& bs = c.model.pot.basis;
+    const int nbo[3] = {(int)bs.b2.size(), (int)bs.b3.size(), (int)bs.b4.size()};
+    Buffers B = c.buf;
+    B.Wd = F.Wd;
     HIPCHK(fit_backward(c.P64, c.shape_class, natom, offsets, nlist, gF, dlist, elist, dlist_momi, dlist_mom,
-                        elist_mom, gE ? atom_struct : nullptr, gE, c.buf, F, (int)T.nterms(), nb234, grad, s));
+                        elist_mom, gE ? atom_struct : nullptr, gE, B, F, (int)T.nterms(), nbo, grad, s));
     return 0;
 }
 

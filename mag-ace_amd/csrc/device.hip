@@ -766,7 +766,8 @@ __device__ __forceinline__ void combine_partials(const P_t& P, V* __restrict__ p
 template <typename T, bool ND2, bool ACC>
 __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int atom_base,
                                                 const cplx<T>* __restrict__ Zin,
-                                                cplx<T>* __restrict__ Wout, char* __restrict__ Pt,
+                                                cplx<T>* __restrict__ Wout, cplx<T>* __restrict__ Wd,
+                                                char* __restrict__ Pt,
                                                 const double* __restrict__ ehc,
                                                 const double* __restrict__ rho1,
                                                 double* __restrict__ eout, double* __restrict__ dEout,
@@ -945,23 +946,34 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int
     // W_eff on representatives of conjugate pairs: since Z_xbar = s conj(Z_x),
     // Re(W_x dZ_x + W_xbar dZ_xbar) = Re((W_x + s conj(W_xbar)) dZ_x); zero elsewhere
     // (the reference's half-range loops with factor 2, src/forces.F90:460-614)
+    // With ND2 and Wd set (fitting), each density's folded gradient dRho_d/dZ is also written,
+    // unscaled: Wd[atom][d][x]
     const T sc = (T)s_scale[0], sc2 = ND2 ? (T)s_scale[1] : (T)0;
     Wout += (size_t)atom * P.nz;
+    cplx<T>* wd = ND2 && Wd ? Wd + (size_t)atom * 2 * P.nz : nullptr;
     for (int x = tid; x < P.nz; x += blockDim.x) {
         const uint32_t f = P.fold[x];
-        cplx<T> w = {0, 0};
+        cplx<T> w = {0, 0}, w2 = {0, 0};
         if (f & (1u << 17)) {
             w = sW[x];
-            if (ND2) w = {sc * w.re + sc2 * sW2[x].re, sc * w.im + sc2 * sW2[x].im};
+            if (ND2) w2 = sW2[x];
             if (!(f & (1u << 18))) {
-                cplx<T> b = sW[f & 0xffff];
-                if (ND2) b = {sc * b.re + sc2 * sW2[f & 0xffff].re, sc * b.im + sc2 * sW2[f & 0xffff].im};
+                const cplx<T> b = sW[f & 0xffff];
                 const T s = (f & (1u << 16)) ? (T)-1 : (T)1;
                 w.re += s * b.re;
                 w.im -= s * b.im;
+                if (ND2) {
+                    const cplx<T> b2 = sW2[f & 0xffff];
+                    w2.re += s * b2.re;
+                    w2.im -= s * b2.im;
+                }
             }
         }
-        Wout[x] = ND2 ? w : cplx<T>{sc * w.re, sc * w.im};
+        Wout[x] = ND2 ? cplx<T>{sc * w.re + sc2 * w2.re, sc * w.im + sc2 * w2.im} : cplx<T>{sc * w.re, sc * w.im};
+        if (wd) {
+            wd[x] = w;
+            wd[P.nz + x] = w2;
+        }
     }
 }
 
@@ -1796,8 +1808,13 @@ __device__ inline void inner_cutoff(double e, double ecut, double decut, double&
     }
 }
 
-// alpha_i, beta_i (one block per atom; D_u rho = Re sum_x W_x Zdot_x / (fc F'), W = fc F' drho/dZ
-// folded onto representatives, which the conjugate symmetry of Zdot allows)
+// alpha_{i,d}, beta_{i,d} (one block per atom), the factors of B_{k,i} and D_u B_{k,i} in
+// dL/dc_{k,d} for L = sum_s gE_s E_s - D_u E:
+//   alpha = gE fc F'_d - fc F''_d D_u rho_d - F'_d dfc D_u e_hc,  beta = -fc F'_d.
+// D_u rho_d = Re sum_x G_x Zdot_x over the representatives of conjugate pairs, G = drho_d/dZ
+// folded (the conjugate symmetry of Zdot allows it): with one density G = W / (fc F'), with two
+// the adjoint's per-density output Wd[atom][d].
+template <int ND>
 __global__ void __launch_bounds__(256) k_fit_atom(DevParams<double> P, const cplx<double>* __restrict__ W,
                                                  const cplx<double>* __restrict__ Zdot,
                                                  const double* __restrict__ rho, const double* __restrict__ ehc,
@@ -1807,27 +1824,38 @@ __global__ void __launch_bounds__(256) k_fit_atom(DevParams<double> P, const cpl
                                                  double* __restrict__ beta) {
     __shared__ double red[8];
     const int atom = blockIdx.x;
-    double s = 0.0;
-    const cplx<double>* w = W + (size_t)atom * P.nz;
+    double tot[ND];
     const cplx<double>* zd = Zdot + (size_t)atom * P.nz;
-    for (int x = threadIdx.x; x < P.nz; x += blockDim.x) s += re_mul(w[x], zd[x]);
-    const double tot = block_sum<double>(s, red);
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        const cplx<double>* w = W + ((size_t)atom * ND + d) * P.nz;
+        double s = 0.0;
+        for (int x = threadIdx.x; x < P.nz; x += blockDim.x) s += re_mul(w[x], zd[x]);
+        tot[d] = block_sum<double>(s, red);
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
-        double F, dF, d2F, fc, dfc;
-        embed2(P.nfemb, rho[atom], P.p1[0][0], P.p1[0][1], F, dF, d2F);
+        double fc, dfc;
         inner_cutoff(ehc[atom], P.ecut, P.decut, fc, dfc);
-        const double sc = fc * dF;
-        const double rho_dot = sc != 0.0 ? tot / sc : 0.0;
         const double g = atom_struct ? gE[atom_struct[atom]] : 0.0;
-        alpha[atom] = g * fc * dF - fc * d2F * rho_dot - dF * dfc * ehc_dot[atom];
-        beta[atom] = -fc * dF;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            double F, dF, d2F;
+            embed2(P.nfemb, rho[(size_t)atom * ND + d], P.p1[d][0], P.p1[d][1], F, dF, d2F);
+            const double sc = fc * dF;
+            const double rho_dot = ND == 2 ? tot[d] : (sc != 0.0 ? tot[d] / sc : 0.0);
+            alpha[(size_t)atom * ND + d] = g * fc * dF - fc * d2F * rho_dot - dF * dfc * ehc_dot[atom];
+            beta[(size_t)atom * ND + d] = -fc * dF;
+        }
     }
 }
 
-// M_t += sum_i alpha_i P_t,i + beta_i D_u P_t,i over a chunk of atoms (Z, Zdot of one atom in
-// LDS at a time); each thread owns kFitMpt monomials so that an atom's Z and Zdot are staged
-// once per kFitMpt * 256 monomials; grid = (monomial tiles, atom chunks)
+// M_{t,d} += sum_i alpha_{i,d} P_t,i + beta_{i,d} D_u P_t,i over a chunk of atoms (Z, Zdot of one
+// atom in LDS at a time); each thread owns kFitMpt monomials so that an atom's Z and Zdot are
+// staged once per kFitMpt * 256 monomials; grid = (monomial tiles, atom chunks); M is
+// [ND][nterms] complex
 constexpr int kFitMpt = 8;
+template <int ND>
 __global__ void __launch_bounds__(256) k_fit_accum(int nterms, int nz, const ushort4* __restrict__ idx,
                                                   const cplx<double>* __restrict__ Z,
                                                   const cplx<double>* __restrict__ Zdot,
@@ -1838,12 +1866,13 @@ __global__ void __launch_bounds__(256) k_fit_accum(int nterms, int nz, const ush
     cplx<double>* sD = sZ + (nz + 1);          // [nz + 1]
     const int t0 = blockIdx.x * blockDim.x * kFitMpt + threadIdx.x;
     ushort4 ix[kFitMpt];
-    double accr[kFitMpt], acci[kFitMpt];
+    double accr[ND][kFitMpt], acci[ND][kFitMpt];
 #pragma unroll
     for (int j = 0; j < kFitMpt; ++j) {
         const int t = t0 + j * blockDim.x;
         ix[j] = t < nterms ? idx[t] : ushort4{0, 0, 0, 0};
-        accr[j] = acci[j] = 0.0;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) accr[d][j] = acci[d][j] = 0.0;
     }
     const int a0 = blockIdx.y * chunk, a1 = min(natom, a0 + chunk);
     for (int a = a0; a < a1; ++a) {
@@ -1857,7 +1886,12 @@ __global__ void __launch_bounds__(256) k_fit_accum(int nterms, int nz, const ush
             sD[nz] = {0, 0};
         }
         __syncthreads();
-        const double al = alpha[a], be = beta[a];
+        double al[ND], be[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            al[d] = alpha[(size_t)a * ND + d];
+            be[d] = beta[(size_t)a * ND + d];
+        }
 #pragma unroll
         for (int j = 0; j < kFitMpt; ++j) {
             if (t0 + j * (int)blockDim.x >= nterms) break;
@@ -1871,22 +1905,28 @@ __global__ void __launch_bounds__(256) k_fit_accum(int nterms, int nz, const ush
             cfma(r, z2, d3);
             cplx<double> Pd = cmul(l, z23);
             cfma(Pd, z01, r);
-            accr[j] += al * Pt.re + be * Pd.re;
-            acci[j] += al * Pt.im + be * Pd.im;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                accr[d][j] += al[d] * Pt.re + be[d] * Pd.re;
+                acci[d][j] += al[d] * Pt.im + be[d] * Pd.im;
+            }
         }
     }
 #pragma unroll
     for (int j = 0; j < kFitMpt; ++j) {
         const int t = t0 + j * blockDim.x;
         if (t < nterms) {
-            atomicAdd(&M[2 * (size_t)t], accr[j]);
-            atomicAdd(&M[2 * (size_t)t + 1], acci[j]);
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                atomicAdd(&M[2 * ((size_t)d * nterms + t)], accr[d][j]);
+                atomicAdd(&M[2 * ((size_t)d * nterms + t) + 1], acci[d][j]);
+            }
         }
     }
 }
 
-// gradient of the B1 coefficients: sum_i alpha_i (g^m_n(mu_i) - 1)
-__global__ void k_fit_b1(DevParams<double> P, int natom, const double* __restrict__ dmomi,
+// gradient of the B1 coefficients: grad[d * nb_m + n] = sum_i alpha_{i,d} (g^m_n(mu_i) - 1)
+__global__ void k_fit_b1(DevParams<double> P, int natom, int nd, const double* __restrict__ dmomi,
                          const double* __restrict__ alpha, double* __restrict__ grad, int* err) {
     const int atom = blockIdx.x * blockDim.x + threadIdx.x;
     if (atom >= natom) return;
@@ -1896,18 +1936,27 @@ __global__ void k_fit_b1(DevParams<double> P, int natom, const double* __restric
     for (int n = 0; n < P.nb_m; ++n) {
         double v = unit ? 1.0 : 0.0, dv;
         if (row) horner(row + 4 * n, w, P.rs_m, v, dv);
-        atomicAdd(&grad[n], alpha[atom] * (v - 1.0));  // one density (fitting requires ndensity 1)
+        for (int d = 0; d < nd; ++d) atomicAdd(&grad[d * P.nb_m + n], alpha[(size_t)atom * nd + d] * (v - 1.0));
     }
 }
 
-// grad[off + k] = Re sum_t u_{k,t} M_t
-__global__ void k_fit_grad(int nb, const int* __restrict__ ptr, const int* __restrict__ tt,
-                           const double* __restrict__ uu, const double* __restrict__ M, double* __restrict__ grad) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nb) return;
+// gradient of the coefficients of orders 2..4: function k of the concatenated list (nbo[0]
+// functions of order 2, nbo[1] of order 3, nbo[2] of order 4), density d:
+// Re sum_t u_{k,t} M_{t,d}, stored at grad[nd * (functions of lower orders) + d * nbo[o] + j]
+// (the file's order: per body order, density-major); grid.y = density
+__global__ void k_fit_grad(int nterms, int nb2, int nb3, int nb4, const int* __restrict__ ptr,
+                           const int* __restrict__ tt, const double* __restrict__ uu,
+                           const double* __restrict__ M, double* __restrict__ grad) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, d = blockIdx.y, nd = gridDim.y;
+    if (k >= nb2 + nb3 + nb4) return;
+    const double* Md = M + 2 * (size_t)d * nterms;
     double s = 0.0;
-    for (int e = ptr[k]; e < ptr[k + 1]; ++e) s += uu[e] * M[2 * (size_t)tt[e]];
-    grad[k] = s;
+    for (int e = ptr[k]; e < ptr[k + 1]; ++e) s += uu[e] * Md[2 * (size_t)tt[e]];
+    int pos;
+    if (k < nb2) pos = d * nb2 + k;
+    else if (k < nb2 + nb3) pos = nd * nb2 + d * nb3 + (k - nb2);
+    else pos = nd * (nb2 + nb3) + d * nb4 + (k - nb2 - nb3);
+    grad[pos] = s;
 }
 
 // per-atom forces f_i = -sum_p F_p (own pairs) + sum_{p: j(p) = i} F_p, and per-structure
@@ -1969,8 +2018,11 @@ static hipError_t run_jvp_class(const DevParams<double>& P, int natom, const int
 hipError_t fit_backward(const DevParams<double>& P, int cls, int natom, const int* off, const int* nlist,
                         const double* du, const double* dlist, const double* elist, const double* dmomi,
                         const double* dmom, const double* emom, const int* atom_struct, const double* gE,
-                        Buffers& B, FitBuffers& F, int nterms, int nb234, double* grad, hipStream_t s) {
+                        Buffers& B, FitBuffers& F, int nterms, const int* nbo, double* grad, hipStream_t s) {
     if (natom <= 0) return hipSuccess;
+    const int nd = P.nd == 2 ? 2 : 1;
+    if (nd == 2 && !B.Wd) return hipErrorInvalidValue;
+    const int nb234 = nbo[0] + nbo[1] + nbo[2];
     hipError_t e = cls == 0   ? run_jvp_class<6, 32, 4, 4, 4, 2>(P, natom, off, nlist, du, dlist, elist, dmom, emom,
                                                                  (cplx<double>*)F.Zdot, F.ehc_dot, B.err, s)
                    : cls == 1 ? run_jvp_class<2, 32, 1, 27, 2, 5>(P, natom, off, nlist, du, dlist, elist, dmom, emom,
@@ -1978,20 +2030,32 @@ hipError_t fit_backward(const DevParams<double>& P, int cls, int natom, const in
                               : run_jvp_class<6, 32, 4, 9, 4, 5>(P, natom, off, nlist, du, dlist, elist, dmom, emom,
                                                                  (cplx<double>*)F.Zdot, F.ehc_dot, B.err, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_fit_atom, dim3(natom), dim3(256), 0, s, P, (const cplx<double>*)B.W,
-                       (const cplx<double>*)F.Zdot, B.rho, B.ehc, F.ehc_dot, atom_struct, gE, F.alpha, F.beta);
-    const size_t nM = 2 * (size_t)nterms;
+    if (nd == 2)
+        hipLaunchKernelGGL(k_fit_atom<2>, dim3(natom), dim3(256), 0, s, P, (const cplx<double>*)B.Wd,
+                           (const cplx<double>*)F.Zdot, B.rho, B.ehc, F.ehc_dot, atom_struct, gE, F.alpha, F.beta);
+    else
+        hipLaunchKernelGGL(k_fit_atom<1>, dim3(natom), dim3(256), 0, s, P, (const cplx<double>*)B.W,
+                           (const cplx<double>*)F.Zdot, B.rho, B.ehc, F.ehc_dot, atom_struct, gE, F.alpha, F.beta);
+    const size_t nM = 2 * (size_t)nterms * nd;
     hipLaunchKernelGGL(k_fill_double, dim3((unsigned)((nM + 255) / 256)), dim3(256), 0, s, F.M, nM);
-    const size_t ng = (size_t)P.nb_m + nb234;
+    const size_t ng = ((size_t)P.nb_m + nb234) * nd;
     hipLaunchKernelGGL(k_fill_double, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s, grad, ng);
     const int chunk = 64;
     const size_t lds = 2 * (size_t)(P.nz + 1) * sizeof(cplx<double>);
     dim3 grid((unsigned)((nterms + 256 * kFitMpt - 1) / (256 * kFitMpt)), (unsigned)((natom + chunk - 1) / chunk));
-    hipLaunchKernelGGL(k_fit_accum, grid, dim3(256), lds, s, nterms, P.nz, (const ushort4*)F.idx,
-                       (const cplx<double>*)B.Z, (const cplx<double>*)F.Zdot, F.alpha, F.beta, natom, chunk, F.M);
-    hipLaunchKernelGGL(k_fit_b1, dim3((natom + 255) / 256), dim3(256), 0, s, P, natom, dmomi, F.alpha, grad, B.err);
-    hipLaunchKernelGGL(k_fit_grad, dim3((nb234 + 255) / 256), dim3(256), 0, s, nb234, F.ub_ptr, F.ub_t, F.ub_u,
-                       F.M, grad + P.nb_m);
+    if (nd == 2)
+        hipLaunchKernelGGL(k_fit_accum<2>, grid, dim3(256), lds, s, nterms, P.nz, (const ushort4*)F.idx,
+                           (const cplx<double>*)B.Z, (const cplx<double>*)F.Zdot, F.alpha, F.beta, natom, chunk,
+                           F.M);
+    else
+        hipLaunchKernelGGL(k_fit_accum<1>, grid, dim3(256), lds, s, nterms, P.nz, (const ushort4*)F.idx,
+                           (const cplx<double>*)B.Z, (const cplx<double>*)F.Zdot, F.alpha, F.beta, natom, chunk,
+                           F.M);
+    hipLaunchKernelGGL(k_fit_b1, dim3((natom + 255) / 256), dim3(256), 0, s, P, natom, nd, dmomi, F.alpha, grad,
+                       B.err);
+    if (nb234 > 0)
+        hipLaunchKernelGGL(k_fit_grad, dim3((nb234 + 255) / 256, nd), dim3(256), 0, s, nterms, nbo[0], nbo[1],
+                           nbo[2], F.ub_ptr, F.ub_t, F.ub_u, F.M, grad + (size_t)nd * P.nb_m);
     return hipGetLastError();
 }
 
@@ -2106,10 +2170,12 @@ static hipError_t run_class(const DevParams<T>& P, int natom, int atom_base, con
     if (pair) {
     } else if (nd2)
         hipLaunchKernelGGL((k_adjoint<T, true, ACC>), dim3(natom), dim3(kAdjointThreads), lds2, s, P, atom_base,
-                           (const cplx<T>*)B.Z, (cplx<T>*)B.W, (char*)B.Pt, B.ehc, B.rho1, eout, B.dE, B.edf, B.rho);
+                           (const cplx<T>*)B.Z, (cplx<T>*)B.W, (cplx<T>*)B.Wd, (char*)B.Pt, B.ehc, B.rho1, eout,
+                           B.dE, B.edf, B.rho);
     else
         hipLaunchKernelGGL((k_adjoint<T, false, ACC>), dim3(natom), dim3(kAdjointThreads), lds2, s, P, atom_base,
-                           (const cplx<T>*)B.Z, (cplx<T>*)B.W, (char*)B.Pt, B.ehc, B.rho1, eout, B.dE, B.edf, B.rho);
+                           (const cplx<T>*)B.Z, (cplx<T>*)B.W, (cplx<T>*)nullptr, (char*)B.Pt, B.ehc, B.rho1, eout,
+                           B.dE, B.edf, B.rho);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(k3, dim3(natom), dim3(64), lds3, s, P, off, dlist, elist, dmomi, emomi, dmom, emom,

@@ -71,6 +71,8 @@ struct Buffers {
     double* edf = nullptr;   // d f_in/d e_hc * F + 1
     double* rho = nullptr;   // rho_i (including the B1 part), [natom][nd]
     void* Pt = nullptr;      // adjoint partial sums, [natom][nd][nslot] x 16 bytes (partials_bytes)
+    void* Wd = nullptr;      // fitting with two densities: [natom][2][nz] per-density dRho/dZ (FP64),
+                             // written by the adjoint when set
     int* err = nullptr;
     size_t cap_atoms = 0;
 };
@@ -100,9 +102,10 @@ struct Launch {
 struct FitBuffers {
     void* Zdot = nullptr;        // [natom][nz] complex
     double* ehc_dot = nullptr;   // [natom]
-    double* alpha = nullptr;     // [natom]
-    double* beta = nullptr;      // [natom]
-    double* M = nullptr;         // [nterms] complex
+    double* alpha = nullptr;     // [natom][nd]
+    double* beta = nullptr;      // [natom][nd]
+    double* M = nullptr;         // [nd][nterms] complex
+    void* Wd = nullptr;          // [natom][2][nz] complex: per-density dRho/dZ (two densities)
     unsigned short* idx = nullptr;  // [nterms][4] monomial factors
     int *ub_ptr = nullptr, *ub_t = nullptr;
     double* ub_u = nullptr;
@@ -111,7 +114,7 @@ struct FitBuffers {
 hipError_t fit_backward(const DevParams<double>& P, int cls, int natom, const int* off, const int* nlist,
                         const double* du, const double* dlist, const double* elist, const double* dmomi,
                         const double* dmom, const double* emom, const int* atom_struct, const double* gE,
-                        Buffers& B, FitBuffers& F, int nterms, int nb234, double* grad, hipStream_t s);
+                        Buffers& B, FitBuffers& F, int nterms, const int* nbo, double* grad, hipStream_t s);
 hipError_t fit_assemble(int natom, const int* off, const int* nlist, const double* flist, const double* eout,
                         const int* atom_struct, int nstruc, double* f, double* estruct, hipStream_t s);
 

@@ -2,7 +2,7 @@
 //
 // Scope: the per-atom energy / force / magnetic-gradient path of libace
 // (src/ace.F90:79-321 -> prepA, prepB, forceweights, forcecompute, torquecompute),
-// single element, body order <= 4, ndensity 1.  The host side parses the potential file,
+// single element, body order <= 4, ndensity 1 or 2.  The host side parses the potential file,
 // enumerates the basis, builds Clebsch-Gordan couplings and FP64 spline tables, and
 // compiles everything into flat tables the HIP kernels consume.
 #pragma once
