@@ -1203,6 +1203,30 @@ struct LegRows {
     }
 };
 
+// one entry P_l^m, dP_l^m/dz of the rows above: P_m^m = prod_{k<=m} c(k), then
+// P_{m+1}^m = d(m+1) z P_m^m and the three-term recurrence up to l
+template <typename T>
+__device__ __forceinline__ void legendre_one(int l, int m, T z, T& p, T& dp) {
+    T pmm = (T)1;
+    for (int k = 1; k <= m; ++k) pmm *= LegC<T>::c(k);
+    p = pmm;
+    dp = (T)0;
+    if (l == m) return;
+    T p2 = pmm, dp2 = (T)0;
+    const T dm = LegC<T>::d(m + 1);
+    T p1 = dm * pmm * z, dp1 = dm * pmm;
+    for (int ll = m + 2; ll <= l; ++ll) {
+        const int k = ll * (ll + 1) / 2 + m;
+        const T a = LegC<T>::a(k), b = LegC<T>::b(k);
+        const T pn = a * fma(z, p1, b * p2);
+        const T dpn = a * (p1 + fma(z, dp1, b * dp2));
+        p2 = p1; dp2 = dp1;
+        p1 = pn; dp1 = dpn;
+    }
+    p = p1;
+    dp = dp1;
+}
+
 template <typename T>
 __device__ __forceinline__ void conj_all(cplx<T>& Y, cplx<T> (&g)[3], int am) {
     Y = cconj_sign(Y, am);
@@ -1516,8 +1540,11 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
             }
         }
     }
-    // ---- self term T0 = dE_i/dm_i over m' >= 0 (src/forces.F90:712-749)
-    if (lane == 0) {
+    // ---- self term T0 = dE_i/dm_i over m' >= 0 (src/forces.F90:712-749), spread over the
+    // wave: lane k takes B1 function k and the (l', m' >= 0) entries k, k + 64, ... of the half
+    // set (each with its own Legendre recurrence), and the wave sums the three components in a
+    // fixed butterfly order
+    {
         const double mu_d = dmomi[atom];
         const T invmu = (T)(1.0 / mu_d);
         const T sx = (T)emomi[3 * atom], sy = (T)emomi[3 * atom + 1], sz = (T)emomi[3 * atom + 2];
@@ -1525,41 +1552,50 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
         bool unit = false;
         const T* rowm = magnetic_row(P, mu_d, wm, unit, err);
         const T rsm = (T)P.rs_m;
+        T Tx = 0, Ty = 0, Tz = 0;
         // B1: dE/dg^m_n = dE/drho c1_n
-        double dE[2] = {dEin[(size_t)atom * P.nd], P.nd > 1 ? dEin[(size_t)atom * P.nd + 1] : 0.0};
-        T s1 = 0;
-        for (int n = 0; n < P.nb_m; ++n) {
+        const double dE0 = dEin[(size_t)atom * P.nd], dE1 = P.nd > 1 ? dEin[(size_t)atom * P.nd + 1] : 0.0;
+        for (int n = lane; n < P.nb_m; n += 64) {
             T v = unit ? (T)1 : (T)0, dv = 0;
             if (rowm) horner(rowm + 4 * n, wm, rsm, v, dv);
-            s1 += (T)(dE[0] * P.c1[0][n] + dE[1] * P.c1[1][n]) * dv;
+            const T s1 = (T)(dE0 * P.c1[0][n] + dE1 * P.c1[1][n]) * dv;
+            Tx = fma(s1, sx, Tx);
+            Ty = fma(s1, sy, Ty);
+            Tz = fma(s1, sz, Tz);
         }
-        T Tx = s1 * sx, Ty = s1 * sy, Tz = s1 * sz;
-        cplx<T> phm[LM + 1];
-        phm[0] = {1, 0};
-#pragma unroll
-        for (int m = 1; m <= LM; ++m) phm[m] = cmul(phm[m - 1], cplx<T>{sx, sy});
-        LegRows<T, LM> Lr;
-        for (int lp = 0; lp <= P.l2m; ++lp) {
-            Lr.step(lp, sz);
-#pragma unroll
-            for (int am = 0; am <= LM; ++am) {
-                if (am > lp) break;
-                cplx<T> Y, g[3];
-                ylm_grad<T>(am, Lr.P[am], Lr.dP[am], phm[am], phm[am > 0 ? am - 1 : 0], sx, sy, sz, Y, g);
-                for (int np = 0; np < P.nr2m; ++np) {
-                    T v = unit ? (T)1 : (T)0, dv = 0;
-                    if (rowm) horner(rowm + 4 * (P.nb_m + lp * P.nr2m + np), wm, rsm, v, dv);
-                    const cplx<T> w0 = W[P.oA0 + np * P.lsm + lp * lp + lp + am];
-                    const T a = dv * re_mul(w0, Y), b = v * invmu;
-                    Tx += fma(a, sx, b * re_mul(w0, g[0]));
-                    Ty += fma(a, sy, b * re_mul(w0, g[1]));
-                    Tz += fma(a, sz, b * re_mul(w0, g[2]));
-                }
+        for (int it = lane; it < P.lhm; it += 64) {
+            int lp = 0;
+            while ((lp + 1) * (lp + 2) / 2 <= it) ++lp;
+            const int am = it - lp * (lp + 1) / 2;
+            T pl, dpl;
+            legendre_one<T>(lp, am, sz, pl, dpl);
+            cplx<T> ph = {1, 0}, ph1 = {1, 0};
+            for (int k = 0; k < am; ++k) {
+                ph1 = ph;
+                ph = cmul(ph, cplx<T>{sx, sy});
+            }
+            cplx<T> Y, g[3];
+            ylm_grad<T>(am, pl, dpl, ph, ph1, sx, sy, sz, Y, g);
+            for (int np = 0; np < P.nr2m; ++np) {
+                T v = unit ? (T)1 : (T)0, dv = 0;
+                if (rowm) horner(rowm + 4 * (P.nb_m + lp * P.nr2m + np), wm, rsm, v, dv);
+                const cplx<T> w0 = W[P.oA0 + np * P.lsm + lp * lp + lp + am];
+                const T a = dv * re_mul(w0, Y), b = v * invmu;
+                Tx += fma(a, sx, b * re_mul(w0, g[0]));
+                Ty += fma(a, sy, b * re_mul(w0, g[1]));
+                Tz += fma(a, sz, b * re_mul(w0, g[2]));
             }
         }
-        tlist0[3 * (size_t)atom] = (double)Tx;
-        tlist0[3 * (size_t)atom + 1] = (double)Ty;
-        tlist0[3 * (size_t)atom + 2] = (double)Tz;
+        for (int o = 32; o > 0; o >>= 1) {
+            Tx += __shfl_xor(Tx, o, 64);
+            Ty += __shfl_xor(Ty, o, 64);
+            Tz += __shfl_xor(Tz, o, 64);
+        }
+        if (lane == 0) {
+            tlist0[3 * (size_t)atom] = (double)Tx;
+            tlist0[3 * (size_t)atom + 1] = (double)Ty;
+            tlist0[3 * (size_t)atom + 2] = (double)Tz;
+        }
     }
 }
 
