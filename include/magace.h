@@ -218,15 +218,16 @@ int magace_fit_backward(int natom, int nei, const int *offsets, const int *nlist
  * core, rad_atomic, rad_magnetic, clu as src/util.F90:1110-1362 writes them). */
 int magace_write_potential(const char *path);
 
-/* Basis introspection: counts of B functions of orders 1..4 of the loaded potential
- * (the numbers src/util.F90:1831,1929,2235,2596 print). */
-int magace_basis_counts(int *nvar4);
-/* Enumerate the basis of a header without a file: hdr[12] = nb_a nb_m nr2_m l2m nr3_a nr3_m
- * l3a l3m nr4_a nr4_m l4a l4m.  Writes index tuples (b1index/b2index/b3index/b4index columns)
- * when cap suffices; returns the count, or a negative error code. */
+/* Basis introspection: counts of B functions of orders 1..6 of the loaded potential (the
+ * numbers src/util.F90:1831,1929,2235,2596,3095,3760 print); writes min(n, 6) of them. */
+int magace_basis_counts(int *nvar, int n);
+/* Enumerate the basis of a header without a file: hdr[20] = nb_a nb_m nr2_m l2m nr3_a nr3_m
+ * l3a l3m nr4_a nr4_m l4a l4m nr5_a nr5_m l5a l5m nr6_a nr6_m l6a l6m.  Writes index tuples
+ * (b1index .. b6index columns) of order 1..6 when cap suffices; returns the count, or a
+ * negative error code. */
 int magace_enumerate(const int *hdr, int order, int *out, int cap);
-/* info[0..9] = nz, merged terms, raw m-combinations, raw order 2/3/4, shape class, oA2,
- * gradient contributions, gradient chunks */
+/* info[0..12] = nz, merged terms, raw m-combinations, raw order 2/3/4, shape class, oA2,
+ * gradient contributions, gradient chunks, raw order 5/6, products of two A entries */
 int magace_model_info(long long *info, int n);
 /* The merged monomial table rho = Re sum_t w_t prod_s Z[idx[4t+s]] (Z[nz] = 1) that replaces
  * the per-B m-combination loops of src/b_basis.F90:91-307; copies when cap >= count and

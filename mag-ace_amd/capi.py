@@ -62,7 +62,7 @@ def lib():
     L.magace_get_tuning.argtypes = [_I, C.c_int]
     L.magace_tune.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 11 + [C.c_int, _D, _I, C.c_void_p]
     L.magace_check_errors.argtypes = [C.c_void_p]
-    L.magace_basis_counts.argtypes = [_I]
+    L.magace_basis_counts.argtypes = [_I, C.c_int]
     L.magace_enumerate.argtypes = [_I, C.c_int, C.c_void_p, C.c_int]
     L.magace_model_info.argtypes = [np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), C.c_int]
     L.magace_neighbors_device.argtypes = [C.c_int, _I, _I, _D, C.c_double, C.c_void_p, C.c_void_p,
@@ -192,20 +192,22 @@ def check_errors(stream=0):
 
 
 def basis_counts():
-    n = np.zeros(4, np.int32)
-    _check(lib().magace_basis_counts(n))
+    """Numbers of B functions of orders 1..6 of the loaded potential."""
+    n = np.zeros(6, np.int32)
+    _check(lib().magace_basis_counts(n, 6))
     return [int(x) for x in n]
 
 
-HDR_KEYS = ("nb_a", "nb_m", "nr2_m", "l2m", "nr3_a", "nr3_m", "l3a", "l3m", "nr4_a", "nr4_m", "l4a", "l4m")
+HDR_KEYS = ("nb_a", "nb_m", "nr2_m", "l2m", "nr3_a", "nr3_m", "l3a", "l3m", "nr4_a", "nr4_m", "l4a", "l4m",
+            "nr5_a", "nr5_m", "l5a", "l5m", "nr6_a", "nr6_m", "l6a", "l6m")
 
 
 def enumerate_basis(header: dict):
     """Index tuples of the basis of `header` (order -> int array), from the C++ enumeration."""
     L = lib()
-    h = np.array([header[k] for k in HDR_KEYS], np.int32)
+    h = np.array([int(header.get(k, 0)) for k in HDR_KEYS], np.int32)
     out = {}
-    for order, width in ((1, 1), (2, 4), (3, 9), (4, 15)):
+    for order, width in ((1, 1), (2, 4), (3, 9), (4, 15), (5, 21), (6, 27)):
         n = L.magace_enumerate(h, order, None, 0)
         if n < 0:
             _check(n)
@@ -216,9 +218,10 @@ def enumerate_basis(header: dict):
 
 
 def model_info():
-    v = np.zeros(10, np.int64)
-    _check(lib().magace_model_info(v, 10))
-    keys = ("nz", "terms", "raw", "raw2", "raw3", "raw4", "shape_class", "oA2", "contribs", "chunks")
+    v = np.zeros(13, np.int64)
+    _check(lib().magace_model_info(v, 13))
+    keys = ("nz", "terms", "raw", "raw2", "raw3", "raw4", "shape_class", "oA2", "contribs", "chunks", "raw5", "raw6",
+            "products")
     return dict(zip(keys, (int(x) for x in v)))
 
 

@@ -89,6 +89,9 @@ struct Context {
     int2* comb = nullptr;
     int2* comb2 = nullptr;
     uint32_t* fold = nullptr;
+    int2* ypair = nullptr;     // products of two A entries (body orders 5, 6), TermList::ypair
+    int2* yft = nullptr;       // their gradient's way back to Z (TermList::yft, yfe)
+    int2* yfe = nullptr;
     Buffers buf;
     std::vector<int> h_off;
     hipStream_t stream = nullptr;
@@ -178,6 +181,7 @@ void dfree(T*& p) {
 Context::~Context() {
     dfree(t64.lut_a); dfree(t64.lut_m); dfree(t32.lut_a); dfree(t32.lut_m);
     dfree(crec64); dfree(crec32); dfree(crec16); dfree(chunks); dfree(comb); dfree(comb2); dfree(fold); dfree(cw2_64);
+    dfree(ypair); dfree(yft); dfree(yfe);
     dfree(cw2_32);
     dfree(buf.Z); dfree(buf.W); dfree(buf.ehc); dfree(buf.rho1); dfree(buf.dE); dfree(buf.edf); dfree(buf.rho);
     dfree(buf.Pt);
@@ -215,7 +219,7 @@ void fill_params(DevParams<T>& P, const Model& M) {
     P.rs_a = M.spl.rs_a;
     P.rs_m = M.spl.rs_m;
     P.nb_a = h.nb_a; P.nb_m = h.nb_m; P.nr2m = h.nr2_m; P.l2m = h.l2m;
-    P.nr3a = h.nr3_a; P.nr3m = h.nr3_m; P.l3a = h.l3a; P.l3m = h.l3m;
+    P.nr3a = h.nr3_a; P.nr3m = h.nr3_m; P.l3a = h.l3a; P.l3m = M.z.lAm;  // the magnetic l range of A
     P.lsa = M.z.lsa; P.lsm = M.z.lsm; P.lsm3 = M.z.lsm3; P.nm3 = M.z.nm3;
     P.oA2 = M.z.oA2; P.oA0 = M.z.oA0; P.nz = M.z.nz;
     for (int q = 0; q < M.z.nm3 && q < 32; ++q) {
@@ -230,6 +234,9 @@ void fill_params(DevParams<T>& P, const Model& M) {
     P.ncomb = (int)(M.terms.comb.size() / 2);
     P.ncomb2 = (int)(M.terms.comb2.size() / 2);
     P.nslot = M.terms.nslot;
+    P.ny = M.terms.ny;
+    P.nyt = (int)(M.terms.yft.size() / 2);
+    P.nzx = M.terms.nzx();
     P.nfemb = h.nfemb;
     P.nd = h.ndensity;
     for (int d = 0; d < 2; ++d) {
@@ -271,6 +278,10 @@ int build_context(Potential&& pot, std::unique_ptr<Context>& out) {
         set_error(MAGACE_ERR_INPUT, "feature vector too long (the adjoint addresses Z with 16-bit byte offsets)");
         return MAGACE_ERR_INPUT;
     }
+    if (!M.pot.basis.undefined.empty()) {
+        set_error(MAGACE_ERR_INPUT, M.pot.basis.undefined);
+        return MAGACE_ERR_INPUT;
+    }
     const Header& h = M.pot.h;
     ctx->shape_class = supported_nm3_class(M.z.nm3, h.nr2_m, h.nb_a, h.l3a, h.l2m, h.nr3_a);
     if (ctx->shape_class < 0) {
@@ -286,9 +297,11 @@ int build_context(Potential&& pot, std::unique_ptr<Context>& out) {
         set_error(MAGACE_ERR_INPUT, "gradient record table too large (2^27 records)");
         return MAGACE_ERR_INPUT;
     }
-    // chunk headers hold the target in 12 bits and the run's slot in 13
-    if (M.z.nz >= 4096 || M.terms.nslot >= 8192) {
-        set_error(MAGACE_ERR_INPUT, "gradient table too large (feature vector >= 4096 or >= 8192 partial sums)");
+    // chunk headers hold the target in 12 bits and the run's slot in 13; records address the
+    // extended vector [Z | 1 | Y] with 16-bit byte offsets of 16-byte entries
+    if (16 * (size_t)M.terms.nzx() >= 65536 || M.terms.nslot >= 8192) {
+        set_error(MAGACE_ERR_INPUT, "gradient table too large (feature vector and products >= 4096 entries or "
+                                    ">= 8192 partial sums)");
         return MAGACE_ERR_INPUT;
     }
     M.spl = build_splines(M.pot);
@@ -348,14 +361,14 @@ int refresh_weights(Context& c) {
     Potential& pot = c.model.pot;
     TermList& T = c.model.terms;
     const Basis& B = pot.basis;
-    const size_t n2 = B.b2.size(), n3 = B.b3.size(), n4 = B.b4.size(), nb = n2 + n3 + n4;
     const size_t nt = T.nterms(), nr = T.ncontrib();
     for (int d = 0; d < T.nd; ++d) {
+        // coefficients of the B functions of orders 2..6 (the index space of T.ut_k)
         std::vector<double> cv;
-        cv.insert(cv.end(), pot.c2.begin() + d * n2, pot.c2.begin() + (d + 1) * n2);
-        cv.insert(cv.end(), pot.c3.begin() + d * n3, pot.c3.begin() + (d + 1) * n3);
-        cv.insert(cv.end(), pot.c4.begin() + d * n4, pot.c4.begin() + (d + 1) * n4);
-        (void)nb;
+        for (const auto* c : {&pot.c2, &pot.c3, &pot.c4, &pot.c5, &pot.c6}) {
+            const size_t n = c->size() / T.nd;
+            cv.insert(cv.end(), c->begin() + d * n, c->begin() + (d + 1) * n);
+        }
         for (size_t t = 0; t < nt; ++t) {
             double w = 0.0;
             for (int e = T.ut_ptr[t]; e < T.ut_ptr[t + 1]; ++e) w += cv[T.ut_k[e]] * T.ut_u[e];
@@ -401,6 +414,14 @@ int upload(Context& c) {
     if (ncomb2) HIPCHK(hipMemcpy(c.comb2, M.terms.comb2.data(), ncomb2 * sizeof(int2), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc((void**)&c.fold, M.terms.fold.size() * sizeof(uint32_t) + 64));
     HIPCHK(hipMemcpy(c.fold, M.terms.fold.data(), M.terms.fold.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    auto up_int2 = [](int2*& d, const std::vector<int32_t>& v) -> int {
+        HIPCHK(hipMalloc((void**)&d, v.size() * sizeof(int32_t) + 64));
+        if (!v.empty()) HIPCHK(hipMemcpy(d, v.data(), v.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        return 0;
+    };
+    if (int rc = up_int2(c.ypair, M.terms.ypair)) return rc;
+    if (int rc = up_int2(c.yft, M.terms.yft)) return rc;
+    if (int rc = up_int2(c.yfe, M.terms.yfe)) return rc;
     if (nc) {
         if (int rc = upload_records(c)) return rc;
         HIPCHK(hipMemcpy(c.chunks, M.terms.chunks.data(), nch * sizeof(int2), hipMemcpyHostToDevice));
@@ -416,6 +437,9 @@ int upload(Context& c) {
     c.P64.comb = c.P32.comb = c.comb;
     c.P64.comb2 = c.P32.comb2 = c.comb2;
     c.P64.fold = c.P32.fold = c.fold;
+    c.P64.ypair = c.P32.ypair = c.ypair;
+    c.P64.yft = c.P32.yft = c.yft;
+    c.P64.yfe = c.P32.yfe = c.yfe;
     c.uploaded = true;
     c.device = dev;
     bump_generation();
@@ -1097,28 +1121,35 @@ int magace_profile_read(double* ms, int* count) {
     return 0;
 }
 
-int magace_basis_counts(int* nvar) {
+int magace_basis_counts(int* nvar, int n) {
     if (!g_ctx) {
         set_error(MAGACE_ERR_STATE, "library not initialised");
         return MAGACE_ERR_STATE;
     }
     const Basis& B = g_ctx->model.pot.basis;
-    nvar[0] = (int)B.b1.size(); nvar[1] = (int)B.b2.size(); nvar[2] = (int)B.b3.size(); nvar[3] = (int)B.b4.size();
+    const int v[6] = {(int)B.b1.size(), (int)B.b2.size(), (int)B.b3.size(), (int)B.b4.size(), (int)B.b5.size(),
+                      (int)B.b6.size()};
+    for (int i = 0; i < n && i < 6; ++i) nvar[i] = v[i];
     return 0;
 }
 
 int magace_enumerate(const int* hdr, int order, int* out, int cap) {
     clear_error();
     Header h;
-    int* f[12] = {&h.nb_a, &h.nb_m, &h.nr2_m, &h.l2m, &h.nr3_a, &h.nr3_m, &h.l3a, &h.l3m,
-                  &h.nr4_a, &h.nr4_m, &h.l4a, &h.l4m};
-    for (int i = 0; i < 12; ++i) *f[i] = hdr[i];
+    int* f[20] = {&h.nb_a, &h.nb_m, &h.nr2_m, &h.l2m, &h.nr3_a, &h.nr3_m, &h.l3a, &h.l3m,
+                  &h.nr4_a, &h.nr4_m, &h.l4a, &h.l4m, &h.nr5_a, &h.nr5_m, &h.l5a, &h.l5m,
+                  &h.nr6_a, &h.nr6_m, &h.l6a, &h.l6m};
+    for (int i = 0; i < 20; ++i) *f[i] = hdr[i];
     std::string err;
     if (!check_header(h, err)) {
         set_error(MAGACE_ERR_INPUT, err);
         return -MAGACE_ERR_INPUT;
     }
     const Basis B = enumerate_basis(h);
+    if (!B.undefined.empty()) {
+        set_error(MAGACE_ERR_INPUT, B.undefined);
+        return -MAGACE_ERR_INPUT;
+    }
     auto copy = [&](const auto& v, int width) {
         const int n = (int)v.size();
         if (out && cap >= n * width)
@@ -1131,8 +1162,10 @@ int magace_enumerate(const int* hdr, int order, int* out, int cap) {
         case 2: return copy(B.b2, 4);
         case 3: return copy(B.b3, 9);
         case 4: return copy(B.b4, 15);
+        case 5: return copy(B.b5, 21);
+        case 6: return copy(B.b6, 27);
     }
-    set_error(MAGACE_ERR_INPUT, "order must be 1..4");
+    set_error(MAGACE_ERR_INPUT, "order must be 1..6");
     return -MAGACE_ERR_INPUT;
 }
 
@@ -1172,7 +1205,7 @@ int magace_fit_ncoef(void) {
         return -MAGACE_ERR_STATE;
     }
     const Potential& p = g_ctx->model.pot;
-    return (int)(p.c1.size() + p.c2.size() + p.c3.size() + p.c4.size());
+    return (int)(p.c1.size() + p.c2.size() + p.c3.size() + p.c4.size() + p.c5.size() + p.c6.size());
 }
 
 int magace_fit_get_coefficients(double* c, int n) {
@@ -1185,7 +1218,7 @@ int magace_fit_get_coefficients(double* c, int n) {
     }
     const Potential& p = g_ctx->model.pot;
     size_t o = 0;
-    for (const auto* v : {&p.c1, &p.c2, &p.c3, &p.c4})
+    for (const auto* v : {&p.c1, &p.c2, &p.c3, &p.c4, &p.c5, &p.c6})
         for (double x : *v) c[o++] = x;
     return 0;
 }
@@ -1200,7 +1233,7 @@ int magace_fit_set_coefficients(const double* c, int n) {
     }
     Potential& p = g_ctx->model.pot;
     size_t o = 0;
-    for (auto* v : {&p.c1, &p.c2, &p.c3, &p.c4})
+    for (auto* v : {&p.c1, &p.c2, &p.c3, &p.c4, &p.c5, &p.c6})
         for (double& x : *v) x = c[o++];
     g_ctx->fit_natom = -1;  // Z and W of the last forward no longer match
     return refresh_weights(*g_ctx);
@@ -1316,7 +1349,8 @@ int magace_fit_backward(int natom, int nei, const int* offsets, const int* nlist
         F.cap_atoms = cap;
     }
     const Basis& bs = c.model.pot.basis;
-    const int nbo[3] = {(int)bs.b2.size(), (int)bs.b3.size(), (int)bs.b4.size()};
+    const int nbo[5] = {(int)bs.b2.size(), (int)bs.b3.size(), (int)bs.b4.size(), (int)bs.b5.size(),
+                        (int)bs.b6.size()};
     Buffers B = c.buf;
     B.Wd = F.Wd;
     HIPCHK(fit_backward(c.P64, c.shape_class, natom, offsets, nlist, gF, dlist, elist, dlist_momi, dlist_mom,
@@ -1330,10 +1364,11 @@ int magace_model_info(long long* info, int n) {
         return MAGACE_ERR_STATE;
     }
     const Model& M = g_ctx->model;
-    long long v[10] = {(long long)M.z.nz, (long long)M.terms.nterms(), (long long)M.terms.raw, M.terms.n2,
+    long long v[13] = {(long long)M.z.nz, (long long)M.terms.nterms(), (long long)M.terms.raw, M.terms.n2,
                        M.terms.n3, M.terms.n4, g_ctx->shape_class, (long long)M.z.oA2,
-                       (long long)M.terms.ncontrib(), (long long)(M.terms.chunks.size() / 2)};
-    for (int i = 0; i < n && i < 10; ++i) info[i] = v[i];
+                       (long long)M.terms.ncontrib(), (long long)(M.terms.chunks.size() / 2), M.terms.n5,
+                       M.terms.n6, M.terms.ny};
+    for (int i = 0; i < n && i < 13; ++i) info[i] = v[i];
     return 0;
 }
 

@@ -1,7 +1,7 @@
 // basis.cpp -- basis enumeration, Clebsch-Gordan couplings and the merged term list.
 //
-// Enumeration follows the selection rules of setuptables/bcreate (src/util.F90:1801-2814,
-// 4673-4960); couplings follow setupcg/setuphighercouplings (src/util.F90:4149-4388).
+// Enumeration follows the selection rules of setuptables/bcreate (src/util.F90:1801-3760,
+// 4673-5330); couplings follow setupcg/setuphighercouplings (src/util.F90:4149-4548).
 // Instead of the reference's per-B m-combination loops with stored dB/dA
 // (src/b_basis.F90:91-307), every B function is expanded once, at init, into monomials
 // w * prod Z[i] of the per-atom feature vector Z = [A | A2 | A0]; identical monomials are
@@ -21,16 +21,32 @@ namespace magace {
 
 bool check_header(const Header& h, std::string& err) {
     auto fail = [&](const char* m) { err = m; return false; };
-    if (h.nr5_a || h.nr5_m || h.nr6_a || h.nr6_m) return fail("body order > 4 is not supported");
     if (h.ndensity < 1 || h.ndensity > 2) return fail("ndensity must be 1 or 2 (src/forces.F90:1066-1085)");
     if (h.nfemb < 0 || h.nfemb > 2) return fail("embedding function not implemented (nfemb 0..2)");
     // limit checks of src/util.F90:233-272
-    if (h.l3a < h.l4a) return fail(" l3max_atomic needs to be largest.");
-    if (h.l2m < h.l3m || h.l2m < h.l4m) return fail(" l2max_magnetic needs to be largest.");
-    if (h.nr3_a < h.nr4_a) return fail(" nradial_atomic needs to be largest.");
-    if (h.nr2_m < h.nr3_m || h.nr2_m < h.nr4_m) return fail(" nradial_magnetic needs to be largest.");
+    if (h.l3a < h.l4a || h.l3a < h.l5a || h.l3a < h.l6a) return fail(" l3max_atomic needs to be largest.");
+    if (h.l2m < h.l3m || h.l2m < h.l4m || h.l2m < h.l5m || h.l2m < h.l6m)
+        return fail(" l2max_magnetic needs to be largest.");
+    if (h.nr3_a < h.nr4_a || h.nr3_a < h.nr5_a || h.nr3_a < h.nr6_a) return fail(" nradial_atomic needs to be largest.");
+    if (h.nr2_m < h.nr3_m || h.nr2_m < h.nr4_m || h.nr2_m < h.nr5_m || h.nr2_m < h.nr6_m)
+        return fail(" nradial_magnetic needs to be largest.");
+    {
+        // nradial = max(nradial3_atomic, nradial2_magnetic) against the orders' maxima
+        const int nradial = std::max(h.nr3_a, h.nr2_m);
+        if (nradial < std::max(h.nr4_a, h.nr4_m) || nradial < std::max(h.nr5_a, h.nr5_m) ||
+            nradial < std::max(h.nr6_a, h.nr6_m))
+            return fail(" nradial needs to be largest.");
+    }
     if (h.nr3_a > h.nb_a) return fail(" nradial_atomic < nradbase_atomic required");
     if (h.nr2_m > h.nb_m) return fail(" nradial_magnetic < nradbase_magnetic required");
+    if (2 * h.l6a > h.l3a || 2 * h.l6m > h.l2m) return fail("6-body couplings not worked for lmax > 6");
+    // the reference's A array holds nradial3_magnetic magnetic radial functions
+    // (src/a_basis.F90:346); higher orders index it with their own nradial_magnetic
+    if (h.nr4_m > h.nr3_m || h.nr5_m > h.nr3_m || h.nr6_m > h.nr3_m)
+        return fail("nradial_magnetic of orders 4-6 above nradial3_magnetic (the reference reads outside its A array)");
+    // prepB skips the 6-body terms when the header has no 5-body terms (src/b_basis.F90:313-315)
+    if ((h.nr6_a || h.nr6_m) && !h.nr5_a && !h.nr5_m)
+        return fail("6-body terms without 5-body terms (the reference's prepB skips them)");
     if (h.l2m > 12 || h.l3a > 12) return fail("angular momentum above 12 is not supported");
     if (h.nb_a > 32 || h.nb_m > 32) return fail("more than 32 radial base functions");
     for (int t : {h.radtype_a, h.radtype_m})
@@ -65,6 +81,163 @@ std::vector<int> rank4_couplings(const Keyer& key, const int nr[4], const int l[
     }
     if ((l[0] + l[1] + l[2] + l[3]) & 1) out.clear();
     return out;
+}
+
+// bcreate rank 5 ((l0 l1)L12 (l2 l3)L34) l4 (src/util.F90:4960-5070): (L12, L34) pairs
+std::vector<std::array<int, 2>> rank5_couplings(const Keyer& key, const int nr[5], const int l[5]) {
+    std::vector<std::array<int, 2>> out;
+    long k[5];
+    for (int i = 0; i < 5; ++i) k[i] = key(l[i], nr[i]);
+    const int lmx = *std::max_element(l, l + 5);
+    for (int a = 0; a <= 2 * lmx; ++a)
+        for (int b = 0; b <= 2 * lmx; ++b) {
+            if (a > l[0] + l[1] || b > l[2] + l[3]) continue;
+            if (a < std::abs(l[0] - l[1]) || b < std::abs(l[2] - l[3])) continue;
+            if (l[4] > a + b || l[4] < std::abs(a - b)) continue;
+            if (k[0] == k[1] && (a & 1)) continue;
+            if (k[2] == k[3] && (b & 1)) continue;
+            if (k[0] == k[2] && k[1] == k[3] && a == b && (l[4] & 1)) continue;
+            out.push_back({a, b});
+        }
+    if ((l[0] + l[1] + l[2] + l[3] + l[4]) & 1) out.clear();
+    return out;
+}
+
+// bcreate rank 6 ((l0 l1)L12 (l2 l3)L34)L56 (l4 l5)L56 (src/util.F90:5140-5230): (L12, L34, L56)
+std::vector<std::array<int, 3>> rank6_couplings(const Keyer& key, const int nr[6], const int l[6]) {
+    std::vector<std::array<int, 3>> out;
+    long k[6];
+    for (int i = 0; i < 6; ++i) k[i] = key(l[i], nr[i]);
+    const int lmx = *std::max_element(l, l + 6);
+    for (int a = 0; a <= 2 * lmx; ++a)
+        for (int b = 0; b <= 2 * lmx; ++b)
+            for (int c = 0; c <= 2 * lmx; ++c) {
+                if (a > l[0] + l[1] || b > l[2] + l[3] || c > l[4] + l[5]) continue;
+                if (a < std::abs(l[0] - l[1]) || b < std::abs(l[2] - l[3]) || c < std::abs(l[4] - l[5])) continue;
+                if (c > a + b || c < std::abs(a - b)) continue;
+                if (k[0] == k[1] && (a & 1)) continue;
+                if (k[2] == k[3] && (b & 1)) continue;
+                if (k[4] == k[5] && (c & 1)) continue;
+                if (k[0] == k[2] && k[1] == k[3] && a == b && (c & 1)) continue;
+                out.push_back({a, b, c});
+            }
+    if ((l[0] + l[1] + l[2] + l[3] + l[4] + l[5]) & 1) out.clear();
+    return out;
+}
+
+// Odometer over digits with per-digit ranges [lo, hi], the last digit fastest (the nesting
+// order of the reference's loops, outermost first).
+struct Odometer {
+    std::vector<int> lo, hi, v;
+    bool done = false;
+    Odometer(std::vector<int> l, std::vector<int> h) : lo(std::move(l)), hi(std::move(h)), v(lo) {
+        for (size_t i = 0; i < lo.size(); ++i)
+            if (hi[i] < lo[i]) done = true;
+    }
+    void next() {
+        for (int i = (int)v.size() - 1; i >= 0; --i) {
+            if (++v[i] <= hi[i]) return;
+            v[i] = lo[i];
+        }
+        done = true;
+    }
+};
+
+// Within each group of B functions made from one index tuple (the first entry excluded), the
+// reference exchanges the internal couplings in columns [c0, c1) so that their sums ascend
+// (src/util.F90:3077-3091 for order 5, :3713-3755 for order 6); replayed step by step.
+template <size_t N>
+void sort_group_lints(std::vector<std::array<int, N>>& v, size_t head, int nint, int c0, int c1) {
+    for (int m = 1; m <= nint - 2; ++m)
+        for (int k = m + 1; k <= nint - 1; ++k) {
+            int s1 = 0, s2 = 0;
+            for (int c = c0; c < c1; ++c) {
+                s1 += v[head + m][c];
+                s2 += v[head + k][c];
+            }
+            if (s1 > s2)
+                for (int c = c0; c < c1; ++c) std::swap(v[head + m][c], v[head + k][c]);
+        }
+}
+
+// 5-body functions (src/util.F90:2822-3093): index tuples ordered on combined (atomic,
+// magnetic) keys, rank-4 atomic and rank-5 magnetic couplings paired one to one
+void enumerate5(const Header& h, const Keyer& key, Basis& B) {
+    const long kmm = (long)h.nr5_m * (h.l5m + 1);
+    auto nc = [&](int na, int la, int nm, int lm) { return (kmm + 1) * ((long)h.nr5_a * la + na) + (h.nr5_m * lm + nm); };
+    // digits: n1..n4, n0'..n4', l1..l4, l0'..l4'
+    std::vector<int> lo(18, 1), hi(18);
+    for (int i = 0; i < 4; ++i) hi[i] = h.nr5_a;
+    for (int i = 4; i < 9; ++i) hi[i] = h.nr5_m;
+    for (int i = 9; i < 13; ++i) lo[i] = 0, hi[i] = h.l5a;
+    for (int i = 13; i < 18; ++i) lo[i] = 0, hi[i] = h.l5m;
+    for (Odometer od(lo, hi); !od.done; od.next()) {
+        const auto& d = od.v;
+        long c[4];
+        for (int i = 0; i < 4; ++i) c[i] = nc(d[i], d[9 + i], d[5 + i], d[14 + i]);
+        if (c[0] < c[1] || c[1] < c[2] || c[2] < c[3]) continue;
+        const int nra[4] = {d[0], d[1], d[2], d[3]}, la[4] = {d[9], d[10], d[11], d[12]};
+        const int nrm[5] = {d[4], d[5], d[6], d[7], d[8]}, lm[5] = {d[13], d[14], d[15], d[16], d[17]};
+        const std::vector<int> ca = rank4_couplings(key, nra, la);
+        const std::vector<std::array<int, 2>> cm = rank5_couplings(key, nrm, lm);
+        if (ca.empty() || cm.empty()) continue;
+        if (ca.size() != cm.size() && B.undefined.empty())
+            B.undefined = "5-body basis pairs " + std::to_string(ca.size()) + " atomic with " +
+                          std::to_string(cm.size()) + " magnetic couplings (src/util.F90:3015-3040 fills the "
+                          "rest with zero indices, which prepB reads outside its arrays)";
+        const size_t head = B.b5.size(), n = std::min(ca.size(), cm.size());
+        for (size_t t = 0; t < n; ++t)
+            B.b5.push_back({d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], d[9], d[10], d[11], d[12],
+                            d[13], d[14], d[15], d[16], d[17], ca[t], cm[t][0], cm[t][1]});
+        sort_group_lints(B.b5, head, (int)n, 19, 21);
+    }
+}
+
+// 6-body functions (src/util.F90:3439-3760): the ordering keys use the 5-body maxima, as in the
+// reference; rank-5 atomic and rank-6 magnetic couplings paired one to one
+void enumerate6(const Header& h, const Keyer& key, Basis& B) {
+    const long kmm = (long)h.nr5_m * (h.l5m + 1);
+    auto nc = [&](int na, int la, int nm, int lm) { return (kmm + 1) * ((long)h.nr6_a * la + na) + (h.nr6_m * lm + nm); };
+    // digits: n1..n5, n0'..n5', l1..l5, l0'..l5'
+    std::vector<int> lo(22, 1), hi(22);
+    for (int i = 0; i < 5; ++i) hi[i] = h.nr6_a;
+    for (int i = 5; i < 11; ++i) hi[i] = h.nr6_m;
+    for (int i = 11; i < 16; ++i) lo[i] = 0, hi[i] = h.l6a;
+    for (int i = 16; i < 22; ++i) lo[i] = 0, hi[i] = h.l6m;
+    for (Odometer od(lo, hi); !od.done; od.next()) {
+        const auto& d = od.v;
+        long c[5], ka[5];
+        for (int i = 0; i < 5; ++i) {
+            c[i] = nc(d[i], d[11 + i], d[6 + i], d[17 + i]);
+            ka[i] = (long)h.nr6_a * d[11 + i] + d[i];
+        }
+        if (c[0] < c[1] || c[1] < c[2] || c[2] < c[3] || c[3] < c[4]) continue;
+        if ((ka[0] < ka[1] || ka[1] < ka[2] || ka[2] < ka[3] || ka[3] < ka[4]) && B.undefined.empty())
+            B.undefined = "Error! the atomic basis is not ordered lexicographically (6-body, src/util.F90:3525-3530)";
+        const int nra[5] = {d[0], d[1], d[2], d[3], d[4]}, la[5] = {d[11], d[12], d[13], d[14], d[15]};
+        const int nrm[6] = {d[5], d[6], d[7], d[8], d[9], d[10]};
+        const int lm[6] = {d[16], d[17], d[18], d[19], d[20], d[21]};
+        const std::vector<std::array<int, 2>> ca = rank5_couplings(key, nra, la);
+        const std::vector<std::array<int, 3>> cm = rank6_couplings(key, nrm, lm);
+        if (ca.empty() || cm.empty()) continue;
+        if (ca.size() != cm.size() && B.undefined.empty())
+            B.undefined = "6-body basis pairs " + std::to_string(ca.size()) + " atomic with " +
+                          std::to_string(cm.size()) + " magnetic couplings (src/util.F90:3660-3690 fills the "
+                          "rest with zero indices, which prepB reads outside its arrays)";
+        const size_t head = B.b6.size(), n = std::min(ca.size(), cm.size());
+        for (size_t t = 0; t < n; ++t) {
+            std::array<int, 27> e;
+            for (int i = 0; i < 22; ++i) e[i] = d[i];
+            e[22] = ca[t][0];
+            e[23] = ca[t][1];
+            e[24] = cm[t][0];
+            e[25] = cm[t][1];
+            e[26] = cm[t][2];
+            B.b6.push_back(e);
+        }
+        sort_group_lints(B.b6, head, (int)n, 22, 24);
+        sort_group_lints(B.b6, head, (int)n, 24, 27);
+    }
 }
 
 }  // namespace
@@ -130,6 +303,8 @@ Basis enumerate_basis(const Header& h) {
                 B.b4.push_back({n1, n2, n3, m0, m1, m2, m3, l1, l2, l3, k0, k1, k2, k3, L});
         }
     }
+    if (h.nr5_a > 0 && h.nr5_m > 0) enumerate5(h, key, B);
+    if (h.nr6_a > 0 && h.nr6_m > 0) enumerate6(h, key, B);
     return B;
 }
 
@@ -161,7 +336,8 @@ ZLayout make_layout(const Header& h) {
     ZLayout z;
     z.lsa = (h.l3a + 1) * (h.l3a + 1);
     z.lsm = (h.l2m + 1) * (h.l2m + 1);
-    z.lsm3 = (h.l3m + 1) * (h.l3m + 1);
+    z.lAm = h.l3m;
+    z.lsm3 = (z.lAm + 1) * (z.lAm + 1);
     z.nm3 = h.nr3_m * z.lsm3;
     z.nr2m_ = h.nr2_m;
     z.oA = 0;
@@ -188,7 +364,7 @@ Conj conj_map(const Header& h, const ZLayout& z) {
         for (int l = 0; l <= h.l3a; ++l)
             for (int m = -l; m <= l; ++m)
                 for (int np = 0; np < h.nr3_m; ++np)
-                    for (int lp = 0; lp <= h.l3m; ++lp)
+                    for (int lp = 0; lp <= z.lAm; ++lp)
                         for (int mp = -lp; mp <= lp; ++mp) {
                             int x = z.a(n, l, m, np, lp, mp);
                             c.partner[x] = z.a(n, l, -m, np, lp, -mp);
@@ -213,10 +389,28 @@ Conj conj_map(const Header& h, const ZLayout& z) {
     return c;
 }
 
-uint64_t pack4(std::array<int, 4> v) {
+// canonical key of a monomial of up to six factors (indices < 2^16, padded with `one`)
+using Key = unsigned __int128;
+Key pack6(std::array<int, 6> v) {
     std::sort(v.begin(), v.end());
-    return ((uint64_t)v[0] << 48) | ((uint64_t)v[1] << 32) | ((uint64_t)v[2] << 16) | (uint64_t)v[3];
+    Key k = 0;
+    for (int i = 0; i < 6; ++i) k = (k << 16) | (Key)(uint16_t)v[i];
+    return k;
 }
+std::array<int, 6> unpack6(Key k) {
+    std::array<int, 6> v;
+    for (int i = 5; i >= 0; --i) {
+        v[i] = (int)(uint16_t)k;
+        k >>= 16;
+    }
+    return v;
+}
+struct KeyHash {
+    size_t operator()(Key k) const {
+        const uint64_t a = (uint64_t)k, b = (uint64_t)(k >> 64);
+        return std::hash<uint64_t>()(a ^ (b * 0x9e3779b97f4a7c15ull));
+    }
+};
 
 }  // namespace
 
@@ -325,20 +519,19 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
     // monomial set does not depend on c and the merged weights w_t = sum_k c_k u_{k,t} can be
     // recomputed for new coefficients (fitting).
     const Conj cj = conj_map(h, z);
-    std::unordered_map<uint64_t, double> one_b;  // canonical key -> u of the current B function
-    std::vector<std::vector<std::pair<uint64_t, double>>> per_b;
-    auto emit = [&](double u, int a, int b, int c, int d) {
+    std::unordered_map<Key, double, KeyHash> one_b;  // canonical key -> u of the current B function
+    std::vector<std::vector<std::pair<Key, double>>> per_b;
+    auto emit6 = [&](double u, const std::array<int, 6>& v) {
         if (u == 0.0) return;
         ++T.raw;
         // fold complex-conjugate partners: Re(u P) + Re(u' s conj P) = (u + s u') Re P
-        const std::array<int, 4> v = {a, b, c, d};
-        std::array<int, 4> vb;
+        std::array<int, 6> vb;
         int sg = 1;
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < 6; ++i) {
             vb[i] = cj.partner[v[i]];
             sg *= cj.sign[v[i]];
         }
-        const uint64_t key = pack4(v), kb = pack4(vb);
+        const Key key = pack6(v), kb = pack6(vb);
         if (kb == key) {
             // a self-conjugate monomial with s = -1 is purely imaginary: Re P == 0
             if (sg == 1) one_b[key] += u;
@@ -348,8 +541,9 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
             one_b[key] += u;
         }
     };
+    auto emit = [&](double u, int a, int b, int c, int d) { emit6(u, {a, b, c, d, one, one}); };
     auto close_b = [&]() {
-        std::vector<std::pair<uint64_t, double>> v(one_b.begin(), one_b.end());
+        std::vector<std::pair<Key, double>> v(one_b.begin(), one_b.end());
         std::sort(v.begin(), v.end());
         per_b.push_back(std::move(v));
         one_b.clear();
@@ -390,6 +584,12 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
         const int n1 = b[0] - 1, n2 = b[1] - 1, n3 = b[2] - 1;
         const int p0 = b[3] - 1, p1 = b[4] - 1, p2 = b[5] - 1, p3 = b[6] - 1;
         const int l1 = b[7], l2 = b[8], l3 = b[9], k0 = b[10], k1 = b[11], k2 = b[12], k3 = b[13], L = b[14];
+        // A holds the magnetic l' <= l3max_magnetic only (src/a_basis.F90:61,248-251: the
+        // entries beyond stay zero), so a function with a factor beyond vanishes
+        if (k1 > h.l3m || k2 > h.l3m || k3 > h.l3m) {
+            close_b();
+            continue;
+        }
         for (int m1 = -l1; m1 <= l1; ++m1)
             for (int m2 = -l2; m2 <= l2; ++m2) {
                 const int m3 = -(m1 + m2);
@@ -413,30 +613,194 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
             }
         close_b();
     }
+    // order 5 (b_basis.F90:320-414, couplings util.F90:4392-4455): atomic
+    // ((l1 l2)L (l3 l4)L)0 over M = m1 + m2 = -(m3 + m4); magnetic ((l0' l1')L01 (l2' l3')L23)l4'
+    // with m4' = -(M01 + M23), coupled to zero with l4'
+    for (size_t k = 0; k < B.b5.size(); ++k) {
+        const auto& b = B.b5[k];
+        const int n[4] = {b[0] - 1, b[1] - 1, b[2] - 1, b[3] - 1}, p0 = b[4] - 1;
+        const int p[4] = {b[5] - 1, b[6] - 1, b[7] - 1, b[8] - 1};
+        const int l[4] = {b[9], b[10], b[11], b[12]}, k0 = b[13];
+        const int kk[4] = {b[14], b[15], b[16], b[17]};
+        const int L = b[18], L01 = b[19], L23 = b[20];
+        if (*std::max_element(kk, kk + 4) > h.l3m) {  // a factor beyond A's l' range: zero, as above
+            close_b();
+            continue;
+        }
+        for (int M = -L; M <= L; ++M)
+            for (int m1 = -l[0]; m1 <= l[0]; ++m1) {
+                const int m2 = M - m1;
+                if (std::abs(m2) > l[1]) continue;
+                for (int m3 = -l[2]; m3 <= l[2]; ++m3) {
+                    const int m4 = -M - m3;
+                    if (std::abs(m4) > l[3]) continue;
+                    const double ca = clebsch_gordan(l[0], l[1], L, m1, m2, M) *
+                                      clebsch_gordan(l[2], l[3], L, m3, m4, -M) * clebsch_gordan(L, L, 0, M, -M, 0);
+                    if (ca == 0.0) continue;
+                    const int m[4] = {m1, m2, m3, m4};
+                    for (int M01 = -L01; M01 <= L01; ++M01)
+                        for (int M23 = -L23; M23 <= L23; ++M23) {
+                            const int q4 = -(M01 + M23);
+                            if (std::abs(q4) > kk[3]) continue;
+                            for (int q0 = -k0; q0 <= k0; ++q0) {
+                                const int q1 = M01 - q0;
+                                if (std::abs(q1) > kk[0]) continue;
+                                for (int q2 = -kk[1]; q2 <= kk[1]; ++q2) {
+                                    const int q3 = M23 - q2;
+                                    if (std::abs(q3) > kk[2]) continue;
+                                    ++T.n5;
+                                    const double cm = clebsch_gordan(k0, kk[0], L01, q0, q1, M01) *
+                                                      clebsch_gordan(kk[1], kk[2], L23, q2, q3, M23) *
+                                                      clebsch_gordan(L01, L23, kk[3], M01, M23, -q4) *
+                                                      clebsch_gordan(kk[3], kk[3], 0, -q4, q4, 0);
+                                    const int q[4] = {q1, q2, q3, q4};
+                                    std::array<int, 6> v;
+                                    v[0] = z.a0(p0, k0, q0);
+                                    for (int i = 0; i < 4; ++i) v[1 + i] = z.a(n[i], l[i], m[i], p[i], kk[i], q[i]);
+                                    v[5] = one;
+                                    emit6(ca * cm, v);
+                                }
+                            }
+                        }
+                }
+            }
+        close_b();
+    }
+    // order 6 (b_basis.F90:420-528, couplings util.F90:4457-4548): atomic
+    // ((l1 l2)L12 (l3 l4)L34)l5 over m5 = -(M12 + M34), coupled to zero with l5; magnetic
+    // ((l0' l1')L01 (l2' l3')L23)L45 (l4' l5')L45 with M45 = -(M01 + M23)
+    for (size_t k = 0; k < B.b6.size(); ++k) {
+        const auto& b = B.b6[k];
+        int n[5], p[5], l[5], kk[5];
+        for (int i = 0; i < 5; ++i) {
+            n[i] = b[i] - 1;
+            p[i] = b[6 + i] - 1;
+            l[i] = b[11 + i];
+            kk[i] = b[17 + i];
+        }
+        const int p0 = b[5] - 1, k0 = b[16];
+        const int L12 = b[22], L34 = b[23], L01 = b[24], L23 = b[25], L45 = b[26];
+        if (*std::max_element(kk, kk + 5) > h.l3m) {  // a factor beyond A's l' range: zero, as above
+            close_b();
+            continue;
+        }
+        for (int M12 = -L12; M12 <= L12; ++M12)
+            for (int M34 = -L34; M34 <= L34; ++M34) {
+                const int m5 = -(M12 + M34);
+                if (std::abs(m5) > l[4]) continue;
+                for (int m1 = -l[0]; m1 <= l[0]; ++m1) {
+                    const int m2 = M12 - m1;
+                    if (std::abs(m2) > l[1]) continue;
+                    for (int m3 = -l[2]; m3 <= l[2]; ++m3) {
+                        const int m4 = M34 - m3;
+                        if (std::abs(m4) > l[3]) continue;
+                        const double ca = clebsch_gordan(l[0], l[1], L12, m1, m2, M12) *
+                                          clebsch_gordan(l[2], l[3], L34, m3, m4, M34) *
+                                          clebsch_gordan(L12, L34, l[4], M12, M34, -m5) *
+                                          clebsch_gordan(l[4], l[4], 0, -m5, m5, 0);
+                        if (ca == 0.0) continue;
+                        const int m[5] = {m1, m2, m3, m4, m5};
+                        for (int M01 = -L01; M01 <= L01; ++M01)
+                            for (int M23 = -L23; M23 <= L23; ++M23) {
+                                const int M45 = -(M01 + M23);
+                                if (std::abs(M45) > L45) continue;
+                                for (int q0 = -k0; q0 <= k0; ++q0) {
+                                    const int q1 = M01 - q0;
+                                    if (std::abs(q1) > kk[0]) continue;
+                                    for (int q2 = -kk[1]; q2 <= kk[1]; ++q2) {
+                                        const int q3 = M23 - q2;
+                                        if (std::abs(q3) > kk[2]) continue;
+                                        for (int q4 = -kk[3]; q4 <= kk[3]; ++q4) {
+                                            const int q5 = M45 - q4;
+                                            if (std::abs(q5) > kk[4]) continue;
+                                            ++T.n6;
+                                            const double cm = clebsch_gordan(k0, kk[0], L01, q0, q1, M01) *
+                                                              clebsch_gordan(kk[1], kk[2], L23, q2, q3, M23) *
+                                                              clebsch_gordan(kk[3], kk[4], L45, q4, q5, M45) *
+                                                              clebsch_gordan(L01, L23, L45, M01, M23, -M45) *
+                                                              clebsch_gordan(L45, L45, 0, -M45, M45, 0);
+                                            const int q[5] = {q1, q2, q3, q4, q5};
+                                            std::array<int, 6> v;
+                                            v[0] = z.a0(p0, k0, q0);
+                                            for (int i = 0; i < 5; ++i)
+                                                v[1 + i] = z.a(n[i], l[i], m[i], p[i], kk[i], q[i]);
+                                            emit6(ca * cm, v);
+                                        }
+                                    }
+                                }
+                            }
+                    }
+                }
+            }
+        close_b();
+    }
     // the monomial set: every key with a nonzero coefficient in some B function
-    std::map<uint64_t, int> keys;
+    std::map<Key, int> keys;
     for (const auto& v : per_b)
         for (const auto& e : v)
             if (e.second != 0.0) keys.emplace(e.first, 0);
     // device order: slot 0 = the A0 factor (every monomial has exactly one), slots 1..3 = the
-    // other factors ascending, padding last; sorted lexicographically
-    std::vector<std::pair<std::array<int, 4>, uint64_t>> rows;
+    // other factors ascending, padding last; sorted lexicographically.  Five or six factors:
+    // the last two (and the two before them) become products Y (TermList::ypair), numbered in
+    // the order of the monomial keys
+    std::map<std::pair<int, int>, int> ymap;
+    auto ycol = [&](int a, int b) {
+        auto it = ymap.emplace(std::make_pair(a, b), (int)ymap.size()).first;
+        return one + 1 + it->second;
+    };
+    std::vector<std::pair<std::array<int, 4>, Key>> rows;
     rows.reserve(keys.size());
     for (const auto& kv : keys) {
-        const uint64_t key = kv.first;
-        std::array<int, 4> v = {(int)(key >> 48), (int)((key >> 32) & 0xffff), (int)((key >> 16) & 0xffff),
-                                (int)(key & 0xffff)};
-        std::array<int, 4> o;
-        int a0 = -1, k = 1;
-        for (int i = 0; i < 4; ++i) {
+        const std::array<int, 6> v = unpack6(kv.first);
+        int a0 = -1;
+        std::vector<int> f;
+        for (int i = 0; i < 6; ++i) {
             if (a0 < 0 && v[i] >= z.oA0 && v[i] < z.nz) a0 = v[i];
-            else o[k++] = v[i];
+            else if (v[i] != one) f.push_back(v[i]);
         }
-        o[0] = a0;
+        std::sort(f.begin(), f.end());
+        std::array<int, 4> o = {a0, one, one, one};
+        if (f.size() <= 3) {
+            for (size_t i = 0; i < f.size(); ++i) o[1 + i] = f[i];
+        } else if (f.size() == 4) {
+            o[1] = f[0];
+            o[2] = f[1];
+            o[3] = ycol(f[2], f[3]);
+        } else {
+            o[1] = f[0];
+            o[2] = ycol(f[1], f[2]);
+            o[3] = ycol(f[3], f[4]);
+        }
         std::sort(o.begin() + 1, o.end());
-        rows.emplace_back(o, key);
+        rows.emplace_back(o, kv.first);
     }
     std::sort(rows.begin(), rows.end());
+    T.ny = (int)ymap.size();
+    const int nzx = z.nz + 1 + T.ny;  // targets: Z entries and Y entries (index nz is the padding)
+    T.nzx_ = nzx;
+    T.ypair.assign(2 * (size_t)T.ny, 0);
+    for (const auto& kv : ymap) {
+        T.ypair[2 * kv.second] = kv.first.first;
+        T.ypair[2 * kv.second + 1] = kv.first.second;
+    }
+    {
+        // dRho/dZ[x] += dRho/dY_i Z[other] for both factors of every Y_i, grouped by x
+        std::vector<std::vector<std::pair<int, int>>> byx(z.nz);
+        for (int i = 0; i < T.ny; ++i) {
+            const int a = T.ypair[2 * i], b = T.ypair[2 * i + 1];
+            byx[a].emplace_back(one + 1 + i, b);
+            byx[b].emplace_back(one + 1 + i, a);
+        }
+        for (int x = 0; x < z.nz; ++x) {
+            if (byx[x].empty()) continue;
+            T.yft.push_back((int32_t)(T.yfe.size() / 2));
+            T.yft.push_back(x | ((int)byx[x].size() << 12));
+            for (const auto& e : byx[x]) {
+                T.yfe.push_back(e.first);
+                T.yfe.push_back(e.second);
+            }
+        }
+    }
     for (size_t t = 0; t < rows.size(); ++t) keys[rows[t].second] = (int)t;
     // U (by B function and transposed by monomial), then w_t = sum_k c_k u_{k,t}
     const size_t nt = rows.size();
@@ -444,13 +808,17 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
     T.nd = nd;
     T.ub_ptr.assign(1, 0);
     std::vector<std::vector<std::pair<int, double>>> byt(nt);
-    // coefficients of the B functions of orders 2..4, density-major
-    const size_t n2 = B.b2.size(), n3 = B.b3.size(), n4 = B.b4.size(), nb = n2 + n3 + n4;
+    // coefficients of the B functions of orders 2..6, density-major
+    const size_t n2 = B.b2.size(), n3 = B.b3.size(), n4 = B.b4.size(), n5 = B.b5.size(), n6 = B.b6.size();
+    const size_t nb = n2 + n3 + n4 + n5 + n6;
     std::vector<double> cvec(nb * nd);
     for (int d = 0; d < nd; ++d) {
-        for (size_t k = 0; k < n2; ++k) cvec[d * nb + k] = pot.c2[d * n2 + k];
-        for (size_t k = 0; k < n3; ++k) cvec[d * nb + n2 + k] = pot.c3[d * n3 + k];
-        for (size_t k = 0; k < n4; ++k) cvec[d * nb + n2 + n3 + k] = pot.c4[d * n4 + k];
+        size_t o = d * nb;
+        for (size_t k = 0; k < n2; ++k) cvec[o++] = pot.c2[d * n2 + k];
+        for (size_t k = 0; k < n3; ++k) cvec[o++] = pot.c3[d * n3 + k];
+        for (size_t k = 0; k < n4; ++k) cvec[o++] = pot.c4[d * n4 + k];
+        for (size_t k = 0; k < n5; ++k) cvec[o++] = pot.c5[d * n5 + k];
+        for (size_t k = 0; k < n6; ++k) cvec[o++] = pot.c6[d * n6 + k];
     }
     for (size_t k = 0; k < per_b.size(); ++k) {
         for (const auto& e : per_b[k]) {
@@ -479,7 +847,7 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
     }
     // transposed gradient: contributions grouped by target
     // (entries: the three other factors, the monomial index)
-    std::vector<std::vector<std::pair<std::array<int, 3>, int>>> by(z.nz);
+    std::vector<std::vector<std::pair<std::array<int, 3>, int>>> by(nzx);
     for (size_t t = 0; t < nt; ++t)
         for (int s = 0; s < 4; ++s) {
             const int x = rows[t].first[s];
@@ -497,8 +865,8 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
     // records.  Chunks are padded (factor index nz, weight 0) to the longest chunk of their
     // group, rounded up to a multiple of 4.
     struct Ch { int x, off, cnt; };
-    std::vector<std::vector<Ch>> rowchunks(z.nz);
-    for (int x = 0; x < z.nz; ++x) {
+    std::vector<std::vector<Ch>> rowchunks(nzx);
+    for (int x = 0; x < nzx; ++x) {
         auto& v = by[x];
         std::sort(v.begin(), v.end());
         // the sorted contributions are dealt round robin over the target's full chunks (chunk c
@@ -527,7 +895,7 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
     std::vector<Ch> order, rest;
     {
         std::vector<int> tg;
-        for (int x = 0; x < z.nz; ++x)
+        for (int x = 0; x < nzx; ++x)
             if (!rowchunks[x].empty()) tg.push_back(x);
         auto nfull = [&](int x) {
             int n = 0;
@@ -613,7 +981,7 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
     // partial-sum slots of the runs of targets with more than one chunk (TermList::comb)
     {
         const size_t L = TermList::kLanes, nc = T.chunks.size() / 2;
-        std::vector<std::vector<std::vector<size_t>>> runs(z.nz);  // target -> runs -> chunks
+        std::vector<std::vector<std::vector<size_t>>> runs(nzx);  // target -> runs -> chunks
         for (size_t lane = 0; lane < L && lane < nc; ++lane)
             for (size_t c = lane; c < nc; c += L) {
                 const int x = T.chunks[2 * c + 1] & 0xfff;
@@ -624,7 +992,7 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
             }
         int slot = 0;
         const int G = TermList::kGroup;
-        for (int x = 0; x < z.nz; ++x) {
+        for (int x = 0; x < nzx; ++x) {
             if (runs[x].empty()) continue;
             const int n = (int)runs[x].size(), ng = (n + G - 1) / G;
             for (int g = 0; g < ng; ++g) {
@@ -654,7 +1022,7 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
         for (int l = 0; l <= h.l3a; ++l)
             for (int m = -l; m <= l; ++m)
                 for (int np = 0; np < h.nr3_m; ++np)
-                    for (int lp = 0; lp <= h.l3m; ++lp)
+                    for (int lp = 0; lp <= z.lAm; ++lp)
                         for (int mp = -lp; mp <= lp; ++mp) {
                             const int x = z.a(n, l, m, np, lp, mp);
                             const bool rep = mp > 0 || (mp == 0 && m >= 0);

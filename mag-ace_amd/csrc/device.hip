@@ -773,24 +773,28 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int
                                                 double* __restrict__ eout, double* __restrict__ dEout,
                                                 double* __restrict__ edfout, double* __restrict__ rho_out) {
     extern __shared__ __align__(16) unsigned char smem[];
-    cplx<T>* sZ = (cplx<T>*)smem;           // [nz+1], sZ[nz] = 1
-    cplx<T>* sW = sZ + (P.nz + 1);          // [nz+1]
-    cplx<T>* sW2 = sW + (P.nz + 1);         // [nz+1] (ND2 only)
-    double* red = (double*)(sW + (ND2 ? 2 : 1) * (P.nz + 1));
+    // extended vector Z' = [Z | 1 | Y] (nzx entries; Y: products for body orders 5, 6)
+    cplx<T>* sZ = (cplx<T>*)smem;           // [nzx], sZ[nz] = 1
+    cplx<T>* sW = sZ + P.nzx;               // [nzx]
+    cplx<T>* sW2 = sW + P.nzx;              // [nzx] (ND2 only)
+    double* red = (double*)(sW + (ND2 ? 2 : 1) * P.nzx);
     __shared__ double s_scale[2];
     const int atom = blockIdx.x, tid = threadIdx.x;
     Zin += (size_t)atom * P.nz;
-    for (int x = tid; x < P.nz; x += blockDim.x) {
-        sZ[x] = Zin[x];
+    for (int x = tid; x < P.nzx; x += blockDim.x) {
+        if (x < P.nz) sZ[x] = Zin[x];
         sW[x] = {0, 0};
         if (ND2) sW2[x] = {0, 0};
     }
-    if (tid == 0) {
-        sZ[P.nz] = {1, 0};
-        sW[P.nz] = {0, 0};
-        if (ND2) sW2[P.nz] = {0, 0};
-    }
+    if (tid == 0) sZ[P.nz] = {1, 0};
     __syncthreads();
+    if (P.ny > 0) {
+        for (int i = tid; i < P.ny; i += blockDim.x) {
+            const int2 pr = P.ypair[i];
+            sZ[P.nz + 1 + i] = cmul(sZ[pr.x], sZ[pr.y]);
+        }
+        __syncthreads();
+    }
     // Gather form of the gradient: a thread sums one chunk of contributions
     // w * Z[a] Z[b] Z[c] to a single target x in registers (four 16-byte records per step,
     // loads issued together; the 64 lanes of a wave read consecutive records).  Thread t
@@ -829,7 +833,7 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int
             const uint4* __restrict__ rec = P.crec + ch.x;  // records of this lane, stride 64
             const T* __restrict__ w2p = ND2 ? P.cw2 + ch.x : nullptr;
             const double* __restrict__ w2d = ND2 && ACC ? P.cw2d + ch.x : nullptr;
-            if (ACC && __any(x >= P.oA0)) {
+            if (ACC && __any(x >= P.oA0 && x < P.nz)) {
                 for (int k = 0; k < cnt; k += 4) {
                     uint4 r[4];
                     double w2[4];
@@ -881,7 +885,7 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int
                 if constexpr (ACC) {  // the run's FP64 and FP32 rounds; rho of an A0 target in FP64
                     const cplx<double> t = {acd.re + (double)acc.re, acd.im + (double)acc.im};
                     const cplx<double> t2 = {acd2.re + (double)acc2.re, acd2.im + (double)acc2.im};
-                    if (x >= P.oA0) {
+                    if (x >= P.oA0 && x < P.nz) {
                         const cplx<T> z = sZ[x];
                         rho += (double)z.re * t.re - (double)z.im * t.im;
                         if (ND2) rho2 += (double)z.re * t2.re - (double)z.im * t2.im;
@@ -925,6 +929,23 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int
         combine_partials(P, part2, sW2);
     }
     __syncthreads();
+    // products Y: dRho/dZ[x] += dRho/dY Z[other] for the Y entries that hold Z[x], one thread
+    // per target in a fixed order (the targets read only Y entries, and write only their own)
+    if (P.nyt > 0) {
+        for (int k = tid; k < P.nyt; k += nthr) {
+            const int2 e = P.yft[k];
+            const int x = e.y & 0xfff, n = e.y >> 12;
+            cplx<T> w = sW[x], w2 = ND2 ? sW2[x] : cplx<T>{0, 0};
+            for (int u = 0; u < n; ++u) {
+                const int2 t = P.yfe[e.x + u];
+                cfma(w, sW[t.x], sZ[t.y]);
+                if (ND2) cfma(w2, sW2[t.x], sZ[t.y]);
+            }
+            sW[x] = w;
+            if (ND2) sW2[x] = w2;
+        }
+        __syncthreads();
+    }
     // rho = Re sum over A0 targets of Z[x] dRho/dZ[x] (every monomial has exactly one A0
     // factor), summed in FP64 in a fixed order (with ACC: already summed at the flushes)
     if constexpr (!ACC)
@@ -993,24 +1014,33 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<floa
                                                      double* __restrict__ rho_out, int natom) {
     using T = float;
     extern __shared__ __align__(16) unsigned char smem[];
-    float4* sZ = (float4*)smem;                  // [nz+1], sZ[nz] = (1, 0, 1, 0)
-    float4* sW = sZ + (P.nz + 1);                // [nz+1]
-    double* red = (double*)(sW + (P.nz + 1));    // [2 * waves]: one sum per atom and wave
+    float4* sZ = (float4*)smem;                  // [nzx] (Z' of k_adjoint), sZ[nz] = (1, 0, 1, 0)
+    float4* sW = sZ + P.nzx;                     // [nzx]
+    double* red = (double*)(sW + P.nzx);         // [2 * waves]: one sum per atom and wave
     __shared__ double s_scale[2];
     const int a0 = 2 * (int)blockIdx.x, tid = threadIdx.x;
     const bool two = a0 + 1 < natom;
     const cplx<T>* Z0 = Zin + (size_t)a0 * P.nz;
     const cplx<T>* Z1 = two ? Z0 + P.nz : Z0;
-    for (int x = tid; x < P.nz; x += blockDim.x) {
-        const cplx<T> u = Z0[x], v = Z1[x];
-        sZ[x] = make_float4(u.re, u.im, v.re, v.im);
+    for (int x = tid; x < P.nzx; x += blockDim.x) {
+        if (x < P.nz) {
+            const cplx<T> u = Z0[x], v = Z1[x];
+            sZ[x] = make_float4(u.re, u.im, v.re, v.im);
+        }
         sW[x] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    if (tid == 0) {
-        sZ[P.nz] = make_float4(1.f, 0.f, 1.f, 0.f);
-        sW[P.nz] = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
+    if (tid == 0) sZ[P.nz] = make_float4(1.f, 0.f, 1.f, 0.f);
     __syncthreads();
+    if (P.ny > 0) {
+        for (int i = tid; i < P.ny; i += blockDim.x) {
+            const int2 pr = P.ypair[i];
+            const float4 a = sZ[pr.x], b = sZ[pr.y];
+            const cplx<T> y0 = cmul(cplx<T>{a.x, a.y}, cplx<T>{b.x, b.y});
+            const cplx<T> y1 = cmul(cplx<T>{a.z, a.w}, cplx<T>{b.z, b.w});
+            sZ[P.nz + 1 + i] = make_float4(y0.re, y0.im, y1.re, y1.im);
+        }
+        __syncthreads();
+    }
     // the walk, runs, slots and rotation of k_adjoint; a slot holds both atoms' sums
     float4* part = (float4*)(Pt + (size_t)blockIdx.x * partials_bytes(P.nslot, 1));
     const int nthr = (int)blockDim.x;
@@ -1035,7 +1065,7 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<floa
             const int x = ch.y & 0xfff, cnt = (ch.y >> 12) & 63;
             const bool sole = (ch.y >> 18) & 1;
             const uint4* __restrict__ rec = crec16 + ch.x;
-            if (ACC && __any(x >= P.oA0)) {
+            if (ACC && __any(x >= P.oA0 && x < P.nz)) {
                 for (int k = 0; k < cnt; k += 4) {
                     uint4 r[4];
 #pragma unroll
@@ -1077,7 +1107,7 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<floa
                 if constexpr (ACC) {
                     const cplx<double> t0 = {acd0.re + (double)acc0.re, acd0.im + (double)acc0.im};
                     const cplx<double> t1 = {acd1.re + (double)acc1.re, acd1.im + (double)acc1.im};
-                    if (x >= P.oA0) {
+                    if (x >= P.oA0 && x < P.nz) {
                         const float4 z = sZ[x];
                         rho0 += (double)z.x * t0.re - (double)z.y * t0.im;
                         rho1s += (double)z.z * t1.re - (double)z.w * t1.im;
@@ -1114,6 +1144,23 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<floa
     __syncthreads();
     combine_partials(P, part, sW);
     __syncthreads();
+    if (P.nyt > 0) {  // products Y, as in k_adjoint
+        for (int k = tid; k < P.nyt; k += nthr) {
+            const int2 e = P.yft[k];
+            const int x = e.y & 0xfff, n = e.y >> 12;
+            float4 w = sW[x];
+            for (int u = 0; u < n; ++u) {
+                const int2 t = P.yfe[e.x + u];
+                const float4 g = sW[t.x], z = sZ[t.y];
+                cplx<T> w0 = {w.x, w.y}, w1 = {w.z, w.w};
+                cfma(w0, cplx<T>{g.x, g.y}, cplx<T>{z.x, z.y});
+                cfma(w1, cplx<T>{g.z, g.w}, cplx<T>{z.z, z.w});
+                w = make_float4(w0.re, w0.im, w1.re, w1.im);
+            }
+            sW[x] = w;
+        }
+        __syncthreads();
+    }
     if constexpr (!ACC)
         for (int x = P.oA0 + tid; x < P.nz; x += nthr) {
             const float4 z = sZ[x], w = sW[x];
@@ -1892,14 +1939,16 @@ __global__ void __launch_bounds__(256) k_fit_atom(DevParams<double> P, const cpl
 // [ND][nterms] complex
 constexpr int kFitMpt = 8;
 template <int ND>
-__global__ void __launch_bounds__(256) k_fit_accum(int nterms, int nz, const ushort4* __restrict__ idx,
+__global__ void __launch_bounds__(256) k_fit_accum(int nterms, int nz, int ny, const int2* __restrict__ ypair,
+                                                  const ushort4* __restrict__ idx,
                                                   const cplx<double>* __restrict__ Z,
                                                   const cplx<double>* __restrict__ Zdot,
                                                   const double* __restrict__ alpha, const double* __restrict__ beta,
                                                   int natom, int chunk, double* __restrict__ M) {
     extern __shared__ __align__(16) unsigned char smem[];
-    cplx<double>* sZ = (cplx<double>*)smem;  // [nz + 1]
-    cplx<double>* sD = sZ + (nz + 1);          // [nz + 1]
+    const int nzx = nz + 1 + ny;              // Z' = [Z | 1 | Y] as in the adjoint
+    cplx<double>* sZ = (cplx<double>*)smem;  // [nzx]
+    cplx<double>* sD = sZ + nzx;              // [nzx]
     const int t0 = blockIdx.x * blockDim.x * kFitMpt + threadIdx.x;
     ushort4 ix[kFitMpt];
     double accr[ND][kFitMpt], acci[ND][kFitMpt];
@@ -1920,6 +1969,17 @@ __global__ void __launch_bounds__(256) k_fit_accum(int nterms, int nz, const ush
         if (threadIdx.x == 0) {
             sZ[nz] = {1, 0};
             sD[nz] = {0, 0};
+        }
+        if (ny > 0) {  // Y = Z[a] Z[b], D_u Y = D_u Z[a] Z[b] + Z[a] D_u Z[b]
+            __syncthreads();
+            for (int i = threadIdx.x; i < ny; i += blockDim.x) {
+                const int2 pr = ypair[i];
+                const cplx<double> za = sZ[pr.x], zb = sZ[pr.y];
+                sZ[nz + 1 + i] = cmul(za, zb);
+                cplx<double> d = cmul(sD[pr.x], zb);
+                cfma(d, za, sD[pr.y]);
+                sD[nz + 1 + i] = d;
+            }
         }
         __syncthreads();
         double al[ND], be[ND];
@@ -1976,23 +2036,23 @@ __global__ void k_fit_b1(DevParams<double> P, int natom, int nd, const double* _
     }
 }
 
-// gradient of the coefficients of orders 2..4: function k of the concatenated list (nbo[0]
-// functions of order 2, nbo[1] of order 3, nbo[2] of order 4), density d:
-// Re sum_t u_{k,t} M_{t,d}, stored at grad[nd * (functions of lower orders) + d * nbo[o] + j]
-// (the file's order: per body order, density-major); grid.y = density
-__global__ void k_fit_grad(int nterms, int nb2, int nb3, int nb4, const int* __restrict__ ptr,
-                           const int* __restrict__ tt, const double* __restrict__ uu,
-                           const double* __restrict__ M, double* __restrict__ grad) {
+// gradient of the coefficients of orders 2..6: function k of the concatenated list (nb.n[0]
+// functions of order 2, ..., nb.n[4] of order 6), density d: Re sum_t u_{k,t} M_{t,d}, stored at
+// grad[nd * (functions of lower orders) + d * nb.n[o] + j] (the file's order: per body order,
+// density-major); grid.y = density
+struct OrderCounts {
+    int n[5];
+};
+__global__ void k_fit_grad(int nterms, OrderCounts nb, const int* __restrict__ ptr, const int* __restrict__ tt,
+                           const double* __restrict__ uu, const double* __restrict__ M, double* __restrict__ grad) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x, d = blockIdx.y, nd = gridDim.y;
-    if (k >= nb2 + nb3 + nb4) return;
+    int lower = 0, o = 0;
+    while (o < 5 && k >= lower + nb.n[o]) lower += nb.n[o++];
+    if (o == 5) return;
     const double* Md = M + 2 * (size_t)d * nterms;
     double s = 0.0;
     for (int e = ptr[k]; e < ptr[k + 1]; ++e) s += uu[e] * Md[2 * (size_t)tt[e]];
-    int pos;
-    if (k < nb2) pos = d * nb2 + k;
-    else if (k < nb2 + nb3) pos = nd * nb2 + d * nb3 + (k - nb2);
-    else pos = nd * (nb2 + nb3) + d * nb4 + (k - nb2 - nb3);
-    grad[pos] = s;
+    grad[nd * lower + d * nb.n[o] + (k - lower)] = s;
 }
 
 // per-atom forces f_i = -sum_p F_p (own pairs) + sum_{p: j(p) = i} F_p, and per-structure
@@ -2054,11 +2114,13 @@ static hipError_t run_jvp_class(const DevParams<double>& P, int natom, const int
 hipError_t fit_backward(const DevParams<double>& P, int cls, int natom, const int* off, const int* nlist,
                         const double* du, const double* dlist, const double* elist, const double* dmomi,
                         const double* dmom, const double* emom, const int* atom_struct, const double* gE,
-                        Buffers& B, FitBuffers& F, int nterms, const int* nbo, double* grad, hipStream_t s) {
+                        Buffers& B, FitBuffers& F, int nterms, const int* nbo5, double* grad, hipStream_t s) {
     if (natom <= 0) return hipSuccess;
     const int nd = P.nd == 2 ? 2 : 1;
     if (nd == 2 && !B.Wd) return hipErrorInvalidValue;
-    const int nb234 = nbo[0] + nbo[1] + nbo[2];
+    OrderCounts nbo;
+    int nb234 = 0;
+    for (int o = 0; o < 5; ++o) nb234 += (nbo.n[o] = nbo5[o]);
     hipError_t e = cls == 0   ? run_jvp_class<6, 32, 4, 4, 4, 2>(P, natom, off, nlist, du, dlist, elist, dmom, emom,
                                                                  (cplx<double>*)F.Zdot, F.ehc_dot, B.err, s)
                    : cls == 1 ? run_jvp_class<2, 32, 1, 27, 2, 5>(P, natom, off, nlist, du, dlist, elist, dmom, emom,
@@ -2077,21 +2139,36 @@ hipError_t fit_backward(const DevParams<double>& P, int cls, int natom, const in
     const size_t ng = ((size_t)P.nb_m + nb234) * nd;
     hipLaunchKernelGGL(k_fill_double, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s, grad, ng);
     const int chunk = 64;
-    const size_t lds = 2 * (size_t)(P.nz + 1) * sizeof(cplx<double>);
+    const size_t lds = 2 * (size_t)P.nzx * sizeof(cplx<double>);
+    {
+        // Z' and its derivative of one atom in LDS: beyond the default dynamic limit for long
+        // feature vectors
+        static bool set[2] = {false, false};
+        const void* k = nd == 2 ? (const void*)k_fit_accum<2> : (const void*)k_fit_accum<1>;
+        if (!set[nd - 1]) {
+            int dev = 0, maxsh = 0;
+            if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+            if ((e = hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev)) != hipSuccess)
+                return e;
+            if ((e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, maxsh)) != hipSuccess)
+                return e;
+            set[nd - 1] = true;
+        }
+    }
     dim3 grid((unsigned)((nterms + 256 * kFitMpt - 1) / (256 * kFitMpt)), (unsigned)((natom + chunk - 1) / chunk));
     if (nd == 2)
-        hipLaunchKernelGGL(k_fit_accum<2>, grid, dim3(256), lds, s, nterms, P.nz, (const ushort4*)F.idx,
+        hipLaunchKernelGGL(k_fit_accum<2>, grid, dim3(256), lds, s, nterms, P.nz, P.ny, P.ypair, (const ushort4*)F.idx,
                            (const cplx<double>*)B.Z, (const cplx<double>*)F.Zdot, F.alpha, F.beta, natom, chunk,
                            F.M);
     else
-        hipLaunchKernelGGL(k_fit_accum<1>, grid, dim3(256), lds, s, nterms, P.nz, (const ushort4*)F.idx,
+        hipLaunchKernelGGL(k_fit_accum<1>, grid, dim3(256), lds, s, nterms, P.nz, P.ny, P.ypair, (const ushort4*)F.idx,
                            (const cplx<double>*)B.Z, (const cplx<double>*)F.Zdot, F.alpha, F.beta, natom, chunk,
                            F.M);
     hipLaunchKernelGGL(k_fit_b1, dim3((natom + 255) / 256), dim3(256), 0, s, P, natom, nd, dmomi, F.alpha, grad,
                        B.err);
     if (nb234 > 0)
-        hipLaunchKernelGGL(k_fit_grad, dim3((nb234 + 255) / 256, nd), dim3(256), 0, s, nterms, nbo[0], nbo[1],
-                           nbo[2], F.ub_ptr, F.ub_t, F.ub_u, F.M, grad + (size_t)nd * P.nb_m);
+        hipLaunchKernelGGL(k_fit_grad, dim3((nb234 + 255) / 256, nd), dim3(256), 0, s, nterms, nbo, F.ub_ptr, F.ub_t,
+                           F.ub_u, F.M, grad + (size_t)nd * P.nb_m);
     return hipGetLastError();
 }
 
@@ -2185,10 +2262,10 @@ static hipError_t run_class(const DevParams<T>& P, int natom, int atom_base, con
     }
     // FP32 with one density: two atoms per adjoint block (their Z and W interleaved in LDS)
     const bool pair = sizeof(T) == 4 && !nd2 && P.crec16 != nullptr;
-    const size_t lds2p = 2 * (size_t)(P.nz + 1) * 16 + 2 * (kAdjointThreads / 64) * sizeof(double);
+    const size_t lds2p = 2 * (size_t)P.nzx * 16 + 2 * (kAdjointThreads / 64) * sizeof(double);
     if (pair && lds2p > limit[5]) return hipErrorInvalidValue;
     const size_t lds2 =
-        (nd2 ? 3 : 2) * (size_t)(P.nz + 1) * sizeof(cplx<T>) + 2 * (kAdjointThreads / 64) * sizeof(double);
+        (nd2 ? 3 : 2) * (size_t)P.nzx * sizeof(cplx<T>) + 2 * (kAdjointThreads / 64) * sizeof(double);
     const size_t lds3 = (size_t)P.nz * sizeof(cplx<T>);
     if (lds1 > limit[0] || lds2 > limit[nd2 ? 3 : 1] || lds3 > limit[q1 ? (r4 ? 6 : 2) : 4])
         return hipErrorInvalidValue;

@@ -45,6 +45,13 @@ struct DevParams {
     const int2* comb2 = nullptr;
     int ncomb = 0, ncomb2 = 0, nslot = 0;  // groups, such targets, partial-sum slots per atom and density
     const uint32_t* fold = nullptr;  // conjugate-pair folding of W (see TermList::fold)
+    // products Y_i = Z[a] Z[b] at index nz + 1 + i (body orders 5, 6; TermList::ypair), and the
+    // way of their gradient back to Z: per target (first term, target | terms << 12), per term
+    // (index of Y, the other factor)
+    const int2* ypair = nullptr;
+    const int2* yft = nullptr;
+    const int2* yfe = nullptr;
+    int ny = 0, nyt = 0, nzx = 0;  // products, targets with product terms, nz + 1 + ny
     int nchunks = 0;
     int nfemb = 0;
     int nd = 1;                  // densities (1 or 2)
@@ -114,7 +121,7 @@ struct FitBuffers {
 hipError_t fit_backward(const DevParams<double>& P, int cls, int natom, const int* off, const int* nlist,
                         const double* du, const double* dlist, const double* elist, const double* dmomi,
                         const double* dmom, const double* emom, const int* atom_struct, const double* gE,
-                        Buffers& B, FitBuffers& F, int nterms, const int* nbo, double* grad, hipStream_t s);
+                        Buffers& B, FitBuffers& F, int nterms, const int* nbo5, double* grad, hipStream_t s);
 hipError_t fit_assemble(int natom, const int* off, const int* nlist, const double* flist, const double* eout,
                         const int* atom_struct, int nstruc, double* f, double* estruct, hipStream_t s);
 

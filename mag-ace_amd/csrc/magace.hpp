@@ -2,7 +2,7 @@
 //
 // Scope: the per-atom energy / force / magnetic-gradient path of libace
 // (src/ace.F90:79-321 -> prepA, prepB, forceweights, forcecompute, torquecompute),
-// single element, body order <= 4, ndensity 1 or 2.  The host side parses the potential file,
+// single element, body order <= 6, ndensity 1 or 2.  The host side parses the potential file,
 // enumerates the basis, builds Clebsch-Gordan couplings and FP64 spline tables, and
 // compiles everything into flat tables the HIP kernels consume.
 #pragma once
@@ -32,13 +32,21 @@ struct Basis {
     std::vector<std::array<int, 4>> b2;   // n0' n1' n1 l
     std::vector<std::array<int, 9>> b3;   // n1 n2 n0' n1' n2' l1 l0' l1' l2'
     std::vector<std::array<int, 15>> b4;  // n1 n2 n3 n0'..n3' l1 l2 l3 l0'..l3' L01'
+    // n1..n4 n0'..n4' l1..l4 l0'..l4' L12 L01' L23' (b5index)
+    std::vector<std::array<int, 21>> b5;
+    // n1..n5 n0'..n5' l1..l5 l0'..l5' L12 L34 L01' L23' L45' (b6index)
+    std::vector<std::array<int, 27>> b6;
+    // non-empty when the reference's enumeration pairs atomic and magnetic couplings of
+    // different counts (it then fills the missing side with zero indices and reads outside
+    // its arrays): the reason such a header is refused
+    std::string undefined;
 };
 
 struct Potential {
     Header h;
     Basis basis;
     // expansion coefficients, one per B function and density: c[d * nvar + k]
-    std::vector<double> c1, c2, c3, c4;
+    std::vector<double> c1, c2, c3, c4, c5, c6;
     std::vector<double> crad_a;          // [nr3_a][l3a+1][nb_a]
     std::vector<double> crad_m;          // [nr2_m][l2m+1][nb_m]
     double p1[2][2] = {{1.0, 1.0}, {1.0, 1.0}};  // per density: embedding prefactor, exponent
@@ -50,8 +58,12 @@ struct Potential {
 };
 
 // Per-atom complex feature vector Z = [A | A2 | A0] (all m, m' stored).
+// A covers the magnetic l' <= l3max_magnetic and nradial3_magnetic radial functions, as the
+// reference's A does (src/a_basis.F90:61,248-251,346: entries of higher l' stay zero, so
+// higher-order functions with such a factor vanish and are left out of the term list).
 struct ZLayout {
-    int lsa = 0, lsm = 0, lsm3 = 0;  // (l3a+1)^2, (l2m+1)^2, (l3m+1)^2
+    int lsa = 0, lsm = 0, lsm3 = 0;  // (l3a+1)^2, (l2m+1)^2, (lAm+1)^2
+    int lAm = 0;                     // magnetic l' range of A (l3max_magnetic)
     int nm3 = 0;                     // nr3_m * lsm3 (magnetic columns of A)
     int oA = 0, oA2 = 0, oA0 = 0, nz = 0;
     int a(int n, int l, int m, int np, int lp, int mp) const {
@@ -95,14 +107,25 @@ struct TermList {
     // folding of the gradient onto representatives of conjugate pairs (x, xbar):
     // partner | sign << 16 (1 = odd m+m') | representative << 17 | self-conjugate << 18
     std::vector<uint32_t> fold;
-    // coefficient structure: B function k (orders 2..4 in file order) -> monomials,
+    // Products of two A entries for monomials of five or six factors (body orders 5 and 6):
+    // entry i is Y_i = Z[a] Z[b] at index nz + 1 + i of the extended vector Z' = [Z | 1 | Y],
+    // so that every monomial has at most four factors of Z' (a five-factor monomial
+    // z0 z1 z2 z3 z4 is z0 z1 z2 Y(z3 z4), a six-factor one z0 z1 Y(z2 z3) Y(z4 z5)).
+    // ypair: 2 per entry (a, b).  The gradient of a Y entry goes back to its factors after the
+    // record walk: yft holds 2 per Z target with such terms (first entry, target | count << 12),
+    // yfe 2 per term (Z' index of Y, the other factor): dRho/dZ[x] += dRho/dY Z[other].
+    int ny = 0;
+    std::vector<int32_t> ypair, yft, yfe;
+    int nzx() const { return nzx_; }
+    int nzx_ = 0;                // nz + 1 + ny
+    // coefficient structure: B function k (orders 2..6 in file order) -> monomials,
     // rho = sum_k c_k B_k with B_k = Re sum_t u_{k,t} prod Z[idx_t]; CSR by k and by t
     std::vector<int32_t> ub_ptr, ub_t;
     std::vector<double> ub_u;
     std::vector<int32_t> ut_ptr, ut_k;
     std::vector<double> ut_u;
     std::vector<int32_t> rec_t;  // monomial of each gradient record (-1: padding)
-    int n2 = 0, n3 = 0, n4 = 0;  // raw (before merging) m-combination counts per order
+    int n2 = 0, n3 = 0, n4 = 0, n5 = 0, n6 = 0;  // raw (before merging) m-combination counts per order
     int raw = 0;
     int nd = 1;                  // densities
     size_t nterms() const { return idx.size() / 4; }

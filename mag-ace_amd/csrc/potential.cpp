@@ -101,11 +101,14 @@ bool parse_potential(const std::string& path, const std::string& element, int xc
 
     pot.basis = enumerate_basis(h);
     const Basis& B = pot.basis;
+    if (!B.undefined.empty()) { err = B.undefined; return false; }
     const int ndn = h.ndensity;
     pot.c1.assign(B.b1.size() * ndn, 0.0);
     pot.c2.assign(B.b2.size() * ndn, 0.0);
     pot.c3.assign(B.b3.size() * ndn, 0.0);
     pot.c4.assign(B.b4.size() * ndn, 0.0);
+    pot.c5.assign(B.b5.size() * ndn, 0.0);
+    pot.c6.assign(B.b6.size() * ndn, 0.0);
     pot.crad_a.assign((size_t)h.nr3_a * (h.l3a + 1) * h.nb_a, 0.0);
     for (int n = 0; n < h.nr3_a; ++n)
         for (int l = 0; l <= h.l3a; ++l) pot.crad_a[((size_t)n * (h.l3a + 1) + l) * h.nb_a + n] = 1.0;
@@ -116,10 +119,12 @@ bool parse_potential(const std::string& path, const std::string& element, int xc
     pot.core[0] = pot.core[1] = 3.0;
 
     // index tuple -> B function positions (a tuple can only occur once per order)
-    std::multimap<uint64_t, size_t> m2, m3, m4;
+    std::multimap<uint64_t, size_t> m2, m3, m4, m5, m6;
     for (size_t k = 0; k < B.b2.size(); ++k) m2.emplace(tuple_key(B.b2[k]), k);
     for (size_t k = 0; k < B.b3.size(); ++k) m3.emplace(tuple_key(B.b3[k]), k);
     for (size_t k = 0; k < B.b4.size(); ++k) m4.emplace(tuple_key(B.b4[k]), k);
+    for (size_t k = 0; k < B.b5.size(); ++k) m5.emplace(tuple_key(B.b5[k]), k);
+    for (size_t k = 0; k < B.b6.size(); ++k) m6.emplace(tuple_key(B.b6[k]), k);
     auto set_match = [](auto& map, const auto& basis, const auto& tup, std::vector<double>& c, size_t o, double v) {
         auto rng = map.equal_range(tuple_key(tup));
         for (auto it = rng.first; it != rng.second; ++it)
@@ -164,6 +169,14 @@ bool parse_potential(const std::string& path, const std::string& element, int xc
                 std::array<int, 15> a;
                 std::copy(iv.begin(), iv.begin() + 15, a.begin());
                 set_match(m4, B.b4, a, pot.c4, (nd - 1) * B.b4.size(), v);
+            } else if (norder == 5 && ni >= 21) {
+                std::array<int, 21> a;
+                std::copy(iv.begin(), iv.begin() + 21, a.begin());
+                set_match(m5, B.b5, a, pot.c5, (nd - 1) * B.b5.size(), v);
+            } else if (norder == 6 && ni >= 27) {
+                std::array<int, 27> a;
+                std::copy(iv.begin(), iv.begin() + 27, a.begin());
+                set_match(m6, B.b6, a, pot.c6, (nd - 1) * B.b6.size(), v);
             }
         } else if (type == "par") {
             if (norder < 1 || norder > 2) { err = "norder < nvarp1 required for par."; return false; }
@@ -233,6 +246,8 @@ bool write_potential(const std::string& path, const Potential& pot, std::string&
         for (size_t k = 0; k < B.b2.size(); ++k) clu(2, B.b2[k].data(), 4, pot.c2[d * B.b2.size() + k], d);
         for (size_t k = 0; k < B.b3.size(); ++k) clu(3, B.b3[k].data(), 9, pot.c3[d * B.b3.size() + k], d);
         for (size_t k = 0; k < B.b4.size(); ++k) clu(4, B.b4[k].data(), 15, pot.c4[d * B.b4.size() + k], d);
+        for (size_t k = 0; k < B.b5.size(); ++k) clu(5, B.b5[k].data(), 21, pot.c5[d * B.b5.size() + k], d);
+        for (size_t k = 0; k < B.b6.size(); ++k) clu(6, B.b6[k].data(), 27, pot.c6[d * B.b6.size() + k], d);
     }
     if (!out) {
         err = "write failed: " + path;

@@ -42,6 +42,10 @@ H6W = dict(BASE, nb_a=12, nb_m=4, nr2_m=2, l2m=6, nr3_a=4, nr3_m=1, l3a=6, l3m=2
 #: l3max_magnetic 2 at lmax 4: "16 6 5 4 / 6 1 4 2 / 2 1 2 2"
 H4W = dict(BASE, nb_a=16, nb_m=6, nr2_m=5, l2m=4, nr3_a=6, nr3_m=1, l3a=4, l3m=2,
            nr4_a=2, nr4_m=1, l4a=2, l4m=2)
+#: body order 5, radial only (l5max 0): two atomic, one magnetic radial function
+H5 = dict(H500, nr5_a=2, nr5_m=1)
+#: body orders 5 and 6, radial only, on two magnetic radial functions of A (medium class)
+H56 = dict(H500, nr3_m=2, nr4_m=2, nr5_a=2, nr5_m=2, nr6_a=2, nr6_m=2)
 #: tiny header for fast unit tests
 HTINY = dict(BASE, nb_a=4, nb_m=2, nr2_m=2, l2m=2, nr3_a=2, nr3_m=1, l3a=2, l3m=1,
              nr4_a=2, nr4_m=1, l4a=2, l4m=1)
@@ -65,8 +69,8 @@ def render(header: dict, basis: dict, seed: int, element: str = "Fe", xc: str = 
         if int(h.get("ndensity", 1)) != 1:
             raise ValueError("explicit values are supported for one density")
         ca, cm = np.asarray(values["crad_a"]), np.asarray(values["crad_m"])
-        cv = {k: np.asarray(values["c"][k]).ravel() for k in (1, 2, 3, 4)}
-        for k in (1, 2, 3, 4):
+        cv = {k: np.asarray(values["c"].get(k, np.zeros(0))).ravel() for k in (1, 2, 3, 4, 5, 6)}
+        for k in (1, 2, 3, 4, 5, 6):
             if len(cv[k]) != len(np.asarray(basis.get(k, np.zeros((0, 1))))):
                 raise ValueError("coefficient count of order %d does not match the basis" % k)
     L = []
@@ -104,7 +108,7 @@ def render(header: dict, basis: dict, seed: int, element: str = "Fe", xc: str = 
                 v = cm[nr - 1, l, nb - 1] if values is not None else rng.normal() / np.sqrt(nb_m)
                 L.append(" rad_magnetic 0 0")
                 L.append(" %s %s %d %d %d %s T" % (element, element, nb, nr, l, _f(v)))
-    for order in (1, 2, 3, 4):
+    for order in (1, 2, 3, 4, 5, 6):
         idx = np.asarray(basis.get(order, np.zeros((0, 1), np.int32)))
         els = " ".join([element] * order)
         for k, row in enumerate(idx):
@@ -117,7 +121,7 @@ def render(header: dict, basis: dict, seed: int, element: str = "Fe", xc: str = 
         for k, key in ((1, "p1a_%d" % nd), (2, "p1b_%d" % nd)):
             L.append(" par %d %d" % (k, nd))
             L.append(" %s %s F" % (element, _f(h.get(key, 1.0))))
-        for order in (1, 2, 3, 4):
+        for order in (1, 2, 3, 4, 5, 6):
             idx = np.asarray(basis.get(order, np.zeros((0, 1), np.int32)))
             els = " ".join([element] * order)
             for row in idx:

@@ -227,6 +227,7 @@ class RefDriver:
             if p.returncode != 0 or not os.path.exists(fout):
                 raise RuntimeError("reference driver failed (%d): %s %s" %
                                    (p.returncode, p.stdout[-2000:], p.stderr[-2000:]))
+            self.stdout = p.stdout  # the reference's own messages (basis counts)
             raw = np.fromfile(fout, dtype="<f8")
         o = 0
         out = {}

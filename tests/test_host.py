@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import CASES, GOLDEN, HEADERS, PKG, potential_text, write_potential
+from helpers import CASES, GOLDEN, HEADERS, HIGHER_HEADERS, PKG, potential_text, write_potential
 
 capi = PKG.capi
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -30,13 +30,62 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, name)
 
 
-@pytest.mark.parametrize("hname", sorted(HEADERS))
+@pytest.mark.parametrize("hname", sorted(h for h in HEADERS if h not in HIGHER_HEADERS))
 def test_enumeration_matches_oracle(hname):
     h = HEADERS[hname]
     mine = capi.enumerate_basis(h)
     o = O.Oracle.from_header(h)
     for k in (1, 2, 3, 4):
         assert np.array_equal(mine[k], o.basis(k)), (hname, k)
+    assert len(mine[5]) == len(mine[6]) == 0
+
+
+@pytest.mark.parametrize("name", ["o5_16", "o56_16", "l4m_16"])
+def test_higher_order_enumeration_counts_match_reference(name):
+    """Body orders 5 and 6: the numbers of B functions of orders 1..6 equal the counts the
+    reference printed when it wrote the fixture (src/util.F90:1831-3755), and the fixture's
+    potential, whose clu records carry this enumeration's index tuples, regenerates bit for
+    bit (the reference matched every record to one of its functions, or its outputs, checked
+    on the GPU, would differ)."""
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    b = capi.enumerate_basis(HEADERS[str(z["header"])])
+    assert [len(b[k]) for k in range(1, 7)] == [int(x) for x in z["reference_counts"]]
+
+
+def test_higher_order_structure():
+    """5-body tuples (b5index columns): radial-only functions couple every l to 0; the
+    6-body ordering keys use the 5-body maxima, as the reference's do."""
+    b = capi.enumerate_basis(HEADERS["H56"])
+    assert b[5].shape[1] == 21 and b[6].shape[1] == 27
+    assert np.all(b[5][:, 9:] == 0) and np.all(b[6][:, 11:] == 0)
+    # combined (atomic, magnetic) keys are non-increasing along each tuple
+    for row in b[5]:
+        keys = [(row[i], row[5 + i]) for i in range(4)]
+        assert keys == sorted(keys, reverse=True)
+
+
+@pytest.mark.parametrize("extra,msg", [
+    (dict(nr5_a=1, nr5_m=1, l5a=1), "pairs 2 atomic with 1 magnetic"),
+    (dict(nr5_a=1, nr5_m=1, l5m=1), "pairs 1 atomic with 2 magnetic"),
+    (dict(nr6_a=1, nr6_m=1), "without 5-body"),
+    (dict(nr4_m=2), "above nradial3_magnetic"),
+])
+def test_header_refusals(extra, msg):
+    """Headers the reference cannot evaluate are refused with the reason: higher-order
+    couplings it pairs with zero indices, 6-body terms without 5-body ones (its prepB skips
+    them), and radial indices of A beyond its array."""
+    h = dict(HEADERS["H500"], nr2_m=2, **extra)
+    with pytest.raises(capi.MagaceError, match=msg):
+        capi.enumerate_basis(h)
+
+
+def test_fourth_order_magnetic_l_beyond_third():
+    """l4max_magnetic above l3max_magnetic: the reference enumerates 4-body functions up to
+    l4max_magnetic but fills A only up to l3max_magnetic (src/a_basis.F90:248-251), so the
+    functions with a factor of higher l' vanish (golden case l4m_16 checks the outputs on
+    the GPU)."""
+    b = capi.enumerate_basis(HEADERS["HL4"])
+    assert HEADERS["HL4"]["l3m"] == 1 and b[4][:, 11:14].max() == 2
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -51,7 +100,7 @@ def test_reference_example_potential_counts():
     if not os.path.exists(path):
         pytest.skip("reference tree not mounted")
     capi.init_file(path, rcut=4.5, rcut_magnetic=4.0, smear=0.2)
-    assert capi.basis_counts() == [16, 720, 4932, 495]
+    assert capi.basis_counts() == [16, 720, 4932, 495, 0, 0]
 
 
 def _clu1(text):
@@ -129,9 +178,10 @@ def test_init_error_paths(tmp_path):
         capi.init_file(pot, rcut=4.0)
     with pytest.raises(capi.MagaceError, match="not found"):
         capi.init_file(str(tmp_path / "missing.in"))
-    bad = open(pot).read().replace("nradial5 l5max 0 0 0 0", "nradial5 l5max 1 1 0 0")
+    # a 5-body header with angular couplings the reference cannot pair
+    bad = open(pot).read().replace("nradial5 l5max 0 0 0 0", "nradial5 l5max 1 1 1 0")
     open(tmp_path / "b.in", "w").write(bad)
-    with pytest.raises(capi.MagaceError, match="body order"):
+    with pytest.raises(capi.MagaceError, match="5-body basis pairs"):
         capi.init_file(str(tmp_path / "b.in"))
     capi.init_file(pot)  # a good file still loads after failures
 
@@ -155,7 +205,7 @@ print("counts", capi.basis_counts())
     p = subprocess.run([sys.executable, "-c", code], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
     assert p.returncode == 0, p.stderr
     assert "multi: Multiple elements" in p.stdout
-    assert "counts [2, 36, 30, 186]" in p.stdout
+    assert "counts [2, 36, 30, 186, 0, 0]" in p.stdout
 
 
 @pytest.mark.parametrize("hname", ["HTINY", "H500", "H2000", "H500ND2"])
