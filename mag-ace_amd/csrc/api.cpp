@@ -561,15 +561,16 @@ int run_device(Context& c, int natom, const int* off, const double* dlist, const
     return 0;
 }
 
-// pairs per acelib_ chunk: MAGACE_ACELIB_CHUNK_PAIRS if set (1024 .. 2^30), else 393216; read
+// pairs per acelib_ chunk: MAGACE_ACELIB_CHUNK_PAIRS if set (1024 .. 2^30), else 262144; read
 // at every call.  Each of the two slots holds, in pinned host memory and again on the
-// device, 112 bytes per pair and 68 per atom of one chunk (44 MB at the default).
+// device, 112 bytes per pair and 68 per atom of one chunk (29 MB at the default).  On the
+// 16000-atom lmax-6 workload (917170 pairs) 262144 was the fastest of 131072 .. 1048576.
 int chunk_pairs() {
     if (const char* e = std::getenv("MAGACE_ACELIB_CHUNK_PAIRS")) {
         const long v = std::atol(e);
         if (v >= 1024 && v <= (1L << 30)) return (int)v;
     }
-    return 393216;
+    return 262144;
 }
 
 // Byte offsets of one acelib_ chunk of na atoms and np pairs.  Inputs: dlist, elist,
@@ -1056,9 +1057,16 @@ void acelib_(int* natom, int* nall, int* nei, int* nlist, int* occ, int* occlist
                              {tlist + 3 * p0, o + L.o_tl, 24 * L.np}});
         return true;
     };
-    for (size_t k = 0; k < nchunk; ++k) {
-        // the slot's previous chunk, k - 2, is done and unpacked
+    // Order of the host's work: chunk k's kernels and copy-back are enqueued, then chunk k + 1
+    // is packed and its upload enqueued, and only then are chunk k - 1's outputs unpacked, so
+    // that the packing of the next chunk does not wait for the previous one's kernels.  Slot
+    // k & 1 is reused by chunk k + 2; the device waits guard its buffers: the upload of k + 2
+    // waits for chunk k's kernels (d_in), its kernels for chunk k's copy-back (d_out); its
+    // copy-back is enqueued after chunk k is unpacked (h_out), its packing after chunk k's
+    // upload has finished (h_in).
+    auto pack_upload = [&](size_t k) {
         Context::Slot& sl = c.slots[k & 1];
+        if (k >= 2 && fail(hipEventSynchronize(sl.ev_in), "sync")) return false;  // h_in of chunk k - 2 sent
         const size_t a0 = cut[k], p0 = c.h_off[a0];
         const ChunkLayout L(cut[k + 1] - a0, c.h_off[cut[k + 1]] - p0);
         char* in = sl.h_in;
@@ -1070,9 +1078,17 @@ void acelib_(int* natom, int* nall, int* nei, int* nlist, int* occ, int* occlist
                              {in + L.i_em, elist_mom + 3 * p0, 24 * L.np},
                              {in + L.i_dmi, dlist_momi + a0, 8 * L.na},
                              {in + L.i_emi, elist_momi + 3 * a0, 24 * L.na}});
-        if (fail(hipMemcpyAsync(sl.d_in, in, L.in_bytes, hipMemcpyHostToDevice, c.s_in), "copy") ||
-            fail(hipEventRecord(sl.ev_in, c.s_in), "event") || fail(hipStreamWaitEvent(s, sl.ev_in, 0), "event"))
-            return;
+        if (k >= 2 && fail(hipStreamWaitEvent(c.s_in, sl.ev_comp, 0), "event")) return false;  // d_in free
+        return !(fail(hipMemcpyAsync(sl.d_in, in, L.in_bytes, hipMemcpyHostToDevice, c.s_in), "copy") ||
+                 fail(hipEventRecord(sl.ev_in, c.s_in), "event"));
+    };
+    if (nchunk > 0 && !pack_upload(0)) return;
+    for (size_t k = 0; k < nchunk; ++k) {
+        Context::Slot& sl = c.slots[k & 1];
+        const size_t a0 = cut[k], p0 = c.h_off[a0];
+        const ChunkLayout L(cut[k + 1] - a0, c.h_off[cut[k + 1]] - p0);
+        if (fail(hipStreamWaitEvent(s, sl.ev_in, 0), "event")) return;
+        if (k >= 2 && fail(hipStreamWaitEvent(s, sl.ev_done, 0), "event")) return;  // d_out of chunk k - 2 copied
         const char* d = sl.d_in;
         char* o = sl.d_out;
         if (run_device(c, (int)L.na, (const int*)(d + L.i_off), (const double*)(d + L.i_dl),
@@ -1085,6 +1101,7 @@ void acelib_(int* natom, int* nall, int* nei, int* nlist, int* occ, int* occlist
             fail(hipMemcpyAsync(sl.h_out, sl.d_out, L.out_bytes, hipMemcpyDeviceToHost, c.s_out), "copy") ||
             fail(hipEventRecord(sl.ev_done, c.s_out), "event"))
             return;
+        if (k + 1 < nchunk && !pack_upload(k + 1)) return;
         if (k > 0 && !unpack(k - 1)) return;
     }
     if (nchunk > 0 && !unpack(nchunk - 1)) return;
