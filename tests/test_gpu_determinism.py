@@ -3,7 +3,8 @@
 The adjoint's partial sums meet in a fixed order (no atomics) and each atom's walk over the
 gradient records depends only on its index in the whole system, so an evaluation gives the
 same bits on every launch, whether the atoms run in one launch, as two halves on two
-streams, or in the chunks of acelib_ on host arrays (FP64 and mixed precision).
+streams, or in the chunks of acelib_ on host arrays (FP64 and mixed precision), also with
+body orders 5 and 6, whose products of A entries send their gradient back in a fixed order.
 """
 import os
 import sys
@@ -53,7 +54,7 @@ def same(x, y):
     return all(np.array_equal(x[k], y[k]) for k in KEYS)
 
 
-@pytest.mark.parametrize("header", ["H500", "H2000"])
+@pytest.mark.parametrize("header", ["H500", "H2000", "H56"])
 @pytest.mark.parametrize("prec", [capi.FP64, capi.MIXED, capi.MIXED_ACC])
 def test_bitwise_repeatable_across_launches_and_splits(header, prec, tmp_path):
     pot = write_potential(str(tmp_path / "p.in"), header, 2027)
