@@ -273,7 +273,7 @@ __device__ __forceinline__ void store_a2_half(const DevParams<T>& P, const cplx<
 // ACC (FP32 only): A and A2 summed with Kahan compensation (the accumulation recipe of
 // MAGACE_MIXED_ACC); twice the accumulator registers, so one wave per SIMD less
 template <typename T, int L, int CH, int NB, int NM3X, int NB2, int NR2X, bool SPLIT, bool TASKSPLIT, bool ACC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6 && NM3X <= 4 ? (sizeof(T) == 4 ? (ACC ? 3 : 5) : 3) : (sizeof(T) == 4 && !ACC ? 3 : 2), 8))) k_density(DevParams<T> P, const int* __restrict__ off,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6 && NM3X <= 4 ? (sizeof(T) == 4 ? (ACC ? 3 : 4) : 3) : (sizeof(T) == 4 && !ACC ? 3 : 2), 8))) k_density(DevParams<T> P, const int* __restrict__ off,
                                                 const double* __restrict__ dlist,
                                                 const double* __restrict__ elist,
                                                 const double* __restrict__ dmomi,
