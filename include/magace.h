@@ -192,8 +192,9 @@ int magace_md_read(void *md, double *x, double *v, double *sdir, double *f, doub
 int magace_md_destroy(void *md);
 
 /* Fitting (FP64).  Coefficients are ordered as the potential file's B functions: c1 (order
- * 1, nradbase_magnetic of them), then orders 2, 3, 4 in enumeration order; the reference's
- * c0..c4 arrays (src/modules.F90) and its writer (src/util.F90:1110-1362) use the same order.
+ * 1, nradbase_magnetic of them), then orders 2 to 6 in enumeration order, each order
+ * density-major with two densities; the reference's c0..c6 arrays (src/modules.F90) and its
+ * writer (src/util.F90:1110-1362) use the same order.
  * magace_fit_set_coefficients recomputes the merged monomial weights (the contraction of
  * src/forces.F90:93-141) and takes effect for every later evaluation.
  * magace_fit_forward: an FP64 evaluation on device arrays (acelib_ meaning, 0-based CSR
