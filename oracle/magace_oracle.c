@@ -6,14 +6,14 @@
  *
  * It follows the reference data flow step by step:
  *   parser            src/util.F90:459-1108   (rwpara, read half)
- *   enumeration       src/util.F90:1613-2814  (setuptables, orders 1..4), 4673-4960 (bcreate)
+ *   enumeration       src/util.F90:1613-4140  (setuptables, orders 1..6), 4673-5330 (bcreate)
  *   Clebsch-Gordan    src/util.F90:4149-4252  (setupcg, Racah formula)
- *   couplings         src/util.F90:4256-4388  (setuphighercouplings)
+ *   couplings         src/util.F90:4256-4548  (setuphighercouplings)
  *   radial basis      src/radial.F90:6-33,430-668
  *   spline tables     src/radial.F90:37-49,164-426 ; lookups :53-160
  *   Y_lm              src/spherical.F90:5-282
  *   prepA             src/a_basis.F90:5-361
- *   prepB             src/b_basis.F90:6-307   (with the stored dB/dA of the reference)
+ *   prepB             src/b_basis.F90:6-532   (with the stored dB/dA of the reference)
  *   energy            src/forces.F90:976-1289
  *   forceweights      src/forces.F90:5-362
  *   forcecompute      src/forces.F90:366-634
@@ -71,6 +71,14 @@ struct orc_pot {
     int nvar4l_a, maddmax4a, *b4lindex_a /*[k][3]*/, *madd4_a, *b4lm_a /*[k][maddmax][3]*/;
     int nvar4l_m, maddmax4m, *b4lindex_m /*[k][5]*/, *madd4_m, *b4lm_m /*[k][maddmax][4]*/;
     int *b4map_a, *b4map_m;
+    /* orders 5 and 6 (util.F90:2822-3436, 3439-4140): index tuples (b5index, b6index columns)
+     * and, per function, the (atomic x magnetic) m-combinations with the product of their
+     * couplings (util.F90:4392-4548); m5lm rows: m1..m4, m0'..m4'; m6lm rows: m1..m5, m0'..m5' */
+    int nvar5, nvar6;
+    int *b5index;   /* [nvar5][21] */
+    int *b6index;   /* [nvar6][27] */
+    int *m5ptr, *m5lm; double *m5cf;
+    int *m6ptr, *m6lm; double *m6cf;
     /* couplings (per m-list entry, util.F90:4256-4388) */
     double *cg01;   /* [l2m+1][2*lmax+1]  indexed m+lmax */
     double *cg02a;  /* [l3a+1][2*lmax+1]  */
@@ -78,7 +86,7 @@ struct orc_pot {
     double *cg03a;  /* [nvar4l_a][maddmax4a] */
     double *cg03m;  /* [nvar4l_m][maddmax4m] */
     /* parameters */
-    double *c0, *c2, *c3, *c4;  /* [nvar][ndensity] */
+    double *c0, *c2, *c3, *c4, *c5, *c6;  /* [nvar][ndensity] */
     double *crad_a;             /* [nr3_a][l3a+1][nb_a]  (n'' fastest) */
     double *crad_m;             /* [nr2_m][l2m+1][nb_m] */
     double p1[2][2];            /* [k][nd] */
@@ -418,6 +426,259 @@ static int enumerate(orc_pot *p) {
     return 0;
 }
 
+/* ----------------------------------------------------------------- orders 5 and 6 */
+/* bcreate rank 5, coupling ((l1 l2)L12 (l3 l4)L34) l5 (util.F90:4960-5070): (L12, L34) pairs
+ * into out[2*i]; returns their number, 0 when the tuple is no basis function */
+static int bcreate5(const orc_pot *p, const int *vecnr, const int *vecl, int *out) {
+    int veck[5], lmx = 0, n = 0;
+    for (int i = 0; i < 5; ++i) {
+        veck[i] = ((1) * (p->lmax + 1) + vecl[i]) * (p->nradial + 1) + vecnr[i];
+        if (vecl[i] > lmx) lmx = vecl[i];
+    }
+    const int *l = vecl;
+    for (int a = 0; a <= 2 * lmx; ++a)
+        for (int b = 0; b <= 2 * lmx; ++b) {
+            if (a > l[0] + l[1] || b > l[2] + l[3]) continue;
+            if (a < abs(l[0] - l[1]) || b < abs(l[2] - l[3])) continue;
+            if (l[4] > a + b || l[4] < abs(a - b)) continue;
+            if (veck[0] == veck[1] && (a % 2) != 0) continue;
+            if (veck[2] == veck[3] && (b % 2) != 0) continue;
+            if (veck[0] == veck[2] && veck[1] == veck[3] && a == b && (l[4] % 2) != 0) continue;
+            out[2 * n] = a; out[2 * n + 1] = b; ++n;
+        }
+    if ((l[0] + l[1] + l[2] + l[3] + l[4]) % 2 != 0) n = 0;
+    return n;
+}
+
+/* bcreate rank 6, ((l1 l2)L12 (l3 l4)L34)L56 (l5 l6)L56 (util.F90:5140-5230) */
+static int bcreate6(const orc_pot *p, const int *vecnr, const int *vecl, int *out) {
+    int veck[6], lmx = 0, n = 0;
+    for (int i = 0; i < 6; ++i) {
+        veck[i] = ((1) * (p->lmax + 1) + vecl[i]) * (p->nradial + 1) + vecnr[i];
+        if (vecl[i] > lmx) lmx = vecl[i];
+    }
+    const int *l = vecl;
+    for (int a = 0; a <= 2 * lmx; ++a)
+        for (int b = 0; b <= 2 * lmx; ++b)
+            for (int c = 0; c <= 2 * lmx; ++c) {
+                if (a > l[0] + l[1] || b > l[2] + l[3] || c > l[4] + l[5]) continue;
+                if (a < abs(l[0] - l[1]) || b < abs(l[2] - l[3]) || c < abs(l[4] - l[5])) continue;
+                if (c > a + b || c < abs(a - b)) continue;
+                if (veck[0] == veck[1] && (a % 2) != 0) continue;
+                if (veck[2] == veck[3] && (b % 2) != 0) continue;
+                if (veck[4] == veck[5] && (c % 2) != 0) continue;
+                if (veck[0] == veck[2] && veck[1] == veck[3] && a == b && (c % 2) != 0) continue;
+                out[3 * n] = a; out[3 * n + 1] = b; out[3 * n + 2] = c; ++n;
+            }
+    if ((l[0] + l[1] + l[2] + l[3] + l[4] + l[5]) % 2 != 0) n = 0;
+    return n;
+}
+
+/* next index tuple in the reference's loop order (last digit fastest) */
+static int next_tuple(int *d, const int *lo, const int *hi, int n) {
+    for (int i = n - 1; i >= 0; --i) {
+        if (++d[i] <= hi[i]) return 1;
+        d[i] = lo[i];
+    }
+    return 0;
+}
+
+/* the exchange of internal couplings inside one group, first entry excluded
+ * (util.F90:3077-3091, 3713-3755): columns [c0, c1) of rows head+1 .. head+nint-1 */
+static void sort_lints(int *idx, int width, int head, int nint, int c0, int c1) {
+    for (int m = 1; m <= nint - 2; ++m)
+        for (int k = m + 1; k <= nint - 1; ++k) {
+            int *a = idx + (size_t)width * (head + m), *b = idx + (size_t)width * (head + k), s1 = 0, s2 = 0;
+            for (int c = c0; c < c1; ++c) { s1 += a[c]; s2 += b[c]; }
+            if (s1 > s2)
+                for (int c = c0; c < c1; ++c) { int t = a[c]; a[c] = b[c]; b[c] = t; }
+        }
+}
+
+static int grow(int **v, int *cap, int n, int width) {
+    if (n < *cap) return 0;
+    *cap = *cap ? 2 * *cap : 256;
+    int *q = realloc(*v, (size_t)*cap * width * sizeof(int));
+    if (!q) return -1;
+    *v = q;
+    return 0;
+}
+
+static int enumerate5(orc_pot *p) {
+    const orc_header *h = &p->h;
+    int lo[18], hi[18], d[18];
+    for (int i = 0; i < 18; ++i) {
+        lo[i] = i < 9 ? 1 : 0;
+        hi[i] = i < 4 ? h->nr5_a : i < 9 ? h->nr5_m : i < 13 ? h->l5a : h->l5m;
+        if (hi[i] < lo[i]) return 0;
+        d[i] = lo[i];
+    }
+    const int kmm = h->nr5_m * (h->l5m + 1);
+    int cap = 0, n = 0;
+    do {
+        int nc[4], ka[4];
+        for (int i = 0; i < 4; ++i) {
+            ka[i] = h->nr5_a * d[9 + i] + d[i];
+            nc[i] = (kmm + 1) * ka[i] + (h->nr5_m * d[14 + i] + d[5 + i]);
+        }
+        if (nc[0] < nc[1] || nc[1] < nc[2] || nc[2] < nc[3]) continue;
+        if (ka[0] < ka[1] || ka[1] < ka[2] || ka[2] < ka[3]) {
+            snprintf(g_err, sizeof g_err, "Error! the atomic basis is not ordered lexicographically");
+            return -1;
+        }
+        int vnr[4] = {d[0], d[1], d[2], d[3]}, vl[4] = {d[9], d[10], d[11], d[12]}, na, aa[64], iso, isb;
+        bcreate(p, 4, vnr, vl, &na, aa, &iso, &isb);
+        if (!iso) { snprintf(g_err, sizeof g_err, "Something wrong with rank = 4."); return -1; }
+        int vnrm[5] = {d[4], d[5], d[6], d[7], d[8]}, vlm[5] = {d[13], d[14], d[15], d[16], d[17]}, am[2 * 1024];
+        int nm = bcreate5(p, vnrm, vlm, am);
+        if (!isb || nm == 0) continue;
+        if (na != nm) {
+            snprintf(g_err, sizeof g_err, "5-body basis pairs %d atomic with %d magnetic couplings "
+                     "(the reference fills the rest with zero indices)", na, nm);
+            return -1;
+        }
+        const int head = n;
+        for (int t = 0; t < na; ++t) {
+            if (grow(&p->b5index, &cap, n, 21)) return -1;
+            int *b = p->b5index + 21 * n++;
+            for (int i = 0; i < 18; ++i) b[i] = d[i];
+            b[18] = aa[t]; b[19] = am[2 * t]; b[20] = am[2 * t + 1];
+        }
+        sort_lints(p->b5index, 21, head, na, 19, 21);
+    } while (next_tuple(d, lo, hi, 18));
+    p->nvar5 = n;
+    return 0;
+}
+
+static int enumerate6(orc_pot *p) {
+    const orc_header *h = &p->h;
+    int lo[22], hi[22], d[22];
+    for (int i = 0; i < 22; ++i) {
+        lo[i] = i < 11 ? 1 : 0;
+        hi[i] = i < 5 ? h->nr6_a : i < 11 ? h->nr6_m : i < 16 ? h->l6a : h->l6m;
+        if (hi[i] < lo[i]) return 0;
+        d[i] = lo[i];
+    }
+    /* the ordering keys use the 5-body maxima, as the reference's do (util.F90:3493-3494) */
+    const int kmm = h->nr5_m * (h->l5m + 1);
+    int cap = 0, n = 0;
+    do {
+        int nc[5], ka[5];
+        for (int i = 0; i < 5; ++i) {
+            ka[i] = h->nr6_a * d[11 + i] + d[i];
+            nc[i] = (kmm + 1) * ka[i] + (h->nr6_m * d[17 + i] + d[6 + i]);
+        }
+        if (nc[0] < nc[1] || nc[1] < nc[2] || nc[2] < nc[3] || nc[3] < nc[4]) continue;
+        if (ka[0] < ka[1] || ka[1] < ka[2] || ka[2] < ka[3] || ka[3] < ka[4]) {
+            snprintf(g_err, sizeof g_err, "Error! the atomic basis is not ordered lexicographically");
+            return -1;
+        }
+        int vnr[5] = {d[0], d[1], d[2], d[3], d[4]}, vl[5] = {d[11], d[12], d[13], d[14], d[15]}, aa[2 * 1024];
+        int na = bcreate5(p, vnr, vl, aa);
+        int vnrm[6] = {d[5], d[6], d[7], d[8], d[9], d[10]}, vlm[6] = {d[16], d[17], d[18], d[19], d[20], d[21]};
+        int am[3 * 4096];
+        int nm = bcreate6(p, vnrm, vlm, am);
+        if (na == 0 || nm == 0) continue;
+        if (na != nm) {
+            snprintf(g_err, sizeof g_err, "6-body basis pairs %d atomic with %d magnetic couplings "
+                     "(the reference fills the rest with zero indices)", na, nm);
+            return -1;
+        }
+        const int head = n;
+        for (int t = 0; t < na; ++t) {
+            if (grow(&p->b6index, &cap, n, 27)) return -1;
+            int *b = p->b6index + 27 * n++;
+            for (int i = 0; i < 22; ++i) b[i] = d[i];
+            b[22] = aa[2 * t]; b[23] = aa[2 * t + 1];
+            b[24] = am[3 * t]; b[25] = am[3 * t + 1]; b[26] = am[3 * t + 2];
+        }
+        sort_lints(p->b6index, 27, head, na, 22, 24);
+        sort_lints(p->b6index, 27, head, na, 24, 27);
+    } while (next_tuple(d, lo, hi, 22));
+    p->nvar6 = n;
+    return 0;
+}
+
+/* per function: the atomic m-list (util.F90:3285-3312 / 3945-3975) times the magnetic one
+ * (:3316-3350 / :3979-4015), each entry with the product of the couplings of
+ * setuphighercouplings (:4392-4455 / :4457-4548) */
+static void mlists56(orc_pot *p) {
+    int cap5 = 0, n5 = 0, cap6 = 0, n6 = 0;
+    p->m5ptr = xcalloc((size_t)p->nvar5 + 1, sizeof(int));
+    for (int c = 0; c < p->nvar5; ++c) {
+        const int *b = p->b5index + 21 * c;
+        const int *l = b + 9, *lp = b + 13, L = b[18], L01 = b[19], L23 = b[20];
+        for (int M = -L; M <= L; ++M)
+        for (int m1 = -l[0]; m1 <= l[0]; ++m1)
+        for (int m2 = -l[1]; m2 <= l[1]; ++m2)
+        for (int m3 = -l[2]; m3 <= l[2]; ++m3)
+        for (int m4 = -l[3]; m4 <= l[3]; ++m4) {
+            if (M - (m1 + m2) != 0 || M + (m3 + m4) != 0) continue;
+            double cfa = cg_racah(p->fac, l[0], l[1], L, m1, m2, M) * cg_racah(p->fac, l[2], l[3], L, m3, m4, -M) *
+                         cg_racah(p->fac, L, L, 0, M, -M, 0);
+            for (int M01 = -L01; M01 <= L01; ++M01)
+            for (int M23 = -L23; M23 <= L23; ++M23)
+            for (int q0 = -lp[0]; q0 <= lp[0]; ++q0)
+            for (int q1 = -lp[1]; q1 <= lp[1]; ++q1)
+            for (int q2 = -lp[2]; q2 <= lp[2]; ++q2)
+            for (int q3 = -lp[3]; q3 <= lp[3]; ++q3)
+            for (int q4 = -lp[4]; q4 <= lp[4]; ++q4) {
+                if (M01 - (q0 + q1) != 0 || M23 - (q2 + q3) != 0 || -q4 - (M01 + M23) != 0) continue;
+                double cfm = cg_racah(p->fac, lp[0], lp[1], L01, q0, q1, M01) *
+                             cg_racah(p->fac, lp[2], lp[3], L23, q2, q3, M23) *
+                             cg_racah(p->fac, L01, L23, lp[4], M01, M23, -q4) *
+                             cg_racah(p->fac, lp[4], lp[4], 0, -q4, q4, 0);
+                if (grow(&p->m5lm, &cap5, n5, 9)) return;
+                int *e = p->m5lm + 9 * n5;
+                e[0] = m1; e[1] = m2; e[2] = m3; e[3] = m4; e[4] = q0; e[5] = q1; e[6] = q2; e[7] = q3; e[8] = q4;
+                p->m5cf = realloc(p->m5cf, (size_t)cap5 * sizeof(double));
+                p->m5cf[n5++] = cfa * cfm;
+            }
+        }
+        p->m5ptr[c + 1] = n5;
+    }
+    p->m6ptr = xcalloc((size_t)p->nvar6 + 1, sizeof(int));
+    for (int c = 0; c < p->nvar6; ++c) {
+        const int *b = p->b6index + 27 * c;
+        const int *l = b + 11, *lp = b + 16, L12 = b[22], L34 = b[23], L01 = b[24], L23 = b[25], L45 = b[26];
+        for (int M12 = -L12; M12 <= L12; ++M12)
+        for (int M34 = -L34; M34 <= L34; ++M34)
+        for (int m1 = -l[0]; m1 <= l[0]; ++m1)
+        for (int m2 = -l[1]; m2 <= l[1]; ++m2)
+        for (int m3 = -l[2]; m3 <= l[2]; ++m3)
+        for (int m4 = -l[3]; m4 <= l[3]; ++m4)
+        for (int m5 = -l[4]; m5 <= l[4]; ++m5) {
+            if (M12 - (m1 + m2) != 0 || M34 - (m3 + m4) != 0 || -m5 - (M12 + M34) != 0) continue;
+            double cfa = cg_racah(p->fac, l[0], l[1], L12, m1, m2, M12) * cg_racah(p->fac, l[2], l[3], L34, m3, m4, M34) *
+                         cg_racah(p->fac, L12, L34, l[4], M12, M34, -m5) * cg_racah(p->fac, l[4], l[4], 0, -m5, m5, 0);
+            for (int M01 = -L01; M01 <= L01; ++M01)
+            for (int M23 = -L23; M23 <= L23; ++M23)
+            for (int M45 = -L45; M45 <= L45; ++M45)
+            for (int q0 = -lp[0]; q0 <= lp[0]; ++q0)
+            for (int q1 = -lp[1]; q1 <= lp[1]; ++q1)
+            for (int q2 = -lp[2]; q2 <= lp[2]; ++q2)
+            for (int q3 = -lp[3]; q3 <= lp[3]; ++q3)
+            for (int q4 = -lp[4]; q4 <= lp[4]; ++q4)
+            for (int q5 = -lp[5]; q5 <= lp[5]; ++q5) {
+                if (M01 - (q0 + q1) != 0 || M23 - (q2 + q3) != 0 || M45 - (q4 + q5) != 0 ||
+                    -M45 - (M01 + M23) != 0) continue;
+                double cfm = cg_racah(p->fac, lp[0], lp[1], L01, q0, q1, M01) *
+                             cg_racah(p->fac, lp[2], lp[3], L23, q2, q3, M23) *
+                             cg_racah(p->fac, lp[4], lp[5], L45, q4, q5, M45) *
+                             cg_racah(p->fac, L01, L23, L45, M01, M23, -M45) *
+                             cg_racah(p->fac, L45, L45, 0, -M45, M45, 0);
+                if (grow(&p->m6lm, &cap6, n6, 11)) return;
+                int *e = p->m6lm + 11 * n6;
+                e[0] = m1; e[1] = m2; e[2] = m3; e[3] = m4; e[4] = m5;
+                e[5] = q0; e[6] = q1; e[7] = q2; e[8] = q3; e[9] = q4; e[10] = q5;
+                p->m6cf = realloc(p->m6cf, (size_t)cap6 * sizeof(double));
+                p->m6cf[n6++] = cfa * cfm;
+            }
+        }
+        p->m6ptr[c + 1] = n6;
+    }
+}
+
 /* ----------------------------------------------------------------- couplings */
 static void couplings(orc_pot *p) {
     const orc_header *h = &p->h;
@@ -467,13 +728,22 @@ static void couplings(orc_pot *p) {
 orc_pot *orc_create(const orc_header *h) {
     orc_pot *p = xcalloc(1, sizeof(orc_pot));
     p->h = *h;
-    if (h->nr5_a || h->nr5_m || h->nr6_a || h->nr6_m) FAIL("body order > 4 is out of scope");
     if (h->ndensity < 1 || h->ndensity > 2) FAIL("ndensity must be 1 or 2 (util.F90 embedding)");
     /* limit checks (util.F90:233-272) */
-    if (h->l3a < h->l4a) FAIL(" l3max_atomic needs to be largest.");
-    if (h->l2m < h->l3m || h->l2m < h->l4m) FAIL(" l2max_magnetic needs to be largest.");
-    if (h->nr3_a < h->nr4_a) FAIL(" nradial_atomic needs to be largest.");
-    if (h->nr2_m < h->nr3_m || h->nr2_m < h->nr4_m) FAIL(" nradial_magnetic needs to be largest.");
+    if (h->l3a < h->l4a || h->l3a < h->l5a || h->l3a < h->l6a) FAIL(" l3max_atomic needs to be largest.");
+    if (h->l2m < h->l3m || h->l2m < h->l4m || h->l2m < h->l5m || h->l2m < h->l6m)
+        FAIL(" l2max_magnetic needs to be largest.");
+    if (h->nr3_a < h->nr4_a || h->nr3_a < h->nr5_a || h->nr3_a < h->nr6_a) FAIL(" nradial_atomic needs to be largest.");
+    if (h->nr2_m < h->nr3_m || h->nr2_m < h->nr4_m || h->nr2_m < h->nr5_m || h->nr2_m < h->nr6_m)
+        FAIL(" nradial_magnetic needs to be largest.");
+    if (2 * h->l6a > h->l3a || 2 * h->l6m > h->l2m) FAIL("6-body couplings not worked for lmax > 6");
+    /* restated here: A with the magnetic indices of order 3 only (the reference's force pass
+     * differs from its energy's gradient beyond, and reads outside A for radial indices
+     * beyond); 6-body terms only with 5-body terms (prepB skips them otherwise) */
+    if (h->l4m > h->l3m || h->l5m > h->l3m || h->l6m > h->l3m) FAIL("magnetic l of orders 4-6 above l3max_magnetic");
+    if (h->nr4_m > h->nr3_m || h->nr5_m > h->nr3_m || h->nr6_m > h->nr3_m)
+        FAIL("magnetic radial functions of orders 4-6 above nradial3_magnetic");
+    if ((h->nr6_a || h->nr6_m) && !h->nr5_a && !h->nr5_m) FAIL("6-body terms without 5-body terms");
     if (h->nr3_a > h->nb_a) FAIL(" nradial_atomic < nradbase_atomic required");
     if (h->nr2_m > h->nb_m) FAIL(" nradial_magnetic < nradbase_magnetic required");
     p->lmax = h->l3a > h->l2m ? h->l3a : h->l2m;
@@ -488,13 +758,18 @@ orc_pot *orc_create(const orc_header *h) {
         for (int l = 0; l < 8 * p->lmax + 1; ++l) { x *= (double)(l + 1); p->fac[l + 1] = x; }
     }
     if (enumerate(p)) goto fail;
+    if (h->nr5_a > 0 && h->nr5_m > 0 && enumerate5(p)) goto fail;
+    if (h->nr6_a > 0 && h->nr6_m > 0 && enumerate6(p)) goto fail;
     couplings(p);
+    mlists56(p);
     /* parameter defaults (util.F90:282-332) */
     int nd = h->ndensity;
     p->c0 = xcalloc((size_t)p->nvar1 * nd, sizeof(double));
     p->c2 = xcalloc((size_t)p->nvar2 * nd, sizeof(double));
     p->c3 = xcalloc((size_t)p->nvar3 * nd, sizeof(double));
     p->c4 = xcalloc((size_t)p->nvar4 * nd, sizeof(double));
+    p->c5 = xcalloc((size_t)p->nvar5 * nd + 1, sizeof(double));
+    p->c6 = xcalloc((size_t)p->nvar6 * nd + 1, sizeof(double));
     p->crad_a = xcalloc((size_t)h->nr3_a * (h->l3a + 1) * h->nb_a, sizeof(double));
     for (int m = 0; m < h->nr3_a; ++m)
         for (int l = 0; l <= h->l3a; ++l) p->crad_a[(m * (h->l3a + 1) + l) * h->nb_a + m] = 1.0;
@@ -517,12 +792,15 @@ void orc_free(orc_pot *p) {
     free(p->b4lm_m); free(p->b4map_a); free(p->b4map_m);
     free(p->cg01); free(p->cg02a); free(p->cg02m); free(p->cg03a); free(p->cg03m);
     free(p->c0); free(p->c2); free(p->c3); free(p->c4); free(p->crad_a); free(p->crad_m);
+    free(p->b5index); free(p->b6index); free(p->m5ptr); free(p->m5lm); free(p->m5cf);
+    free(p->m6ptr); free(p->m6lm); free(p->m6cf); free(p->c5); free(p->c6);
     free(p->lut_ga); free(p->lut_fa); free(p->lut_hc); free(p->lut_gm); free(p->lut_fm);
     free(p);
 }
 
-void orc_counts(const orc_pot *p, int nvar[4]) {
+void orc_counts(const orc_pot *p, int nvar[6]) {
     nvar[0] = p->nvar1; nvar[1] = p->nvar2; nvar[2] = p->nvar3; nvar[3] = p->nvar4;
+    nvar[4] = p->nvar5; nvar[5] = p->nvar6;
 }
 
 int orc_basis_index(const orc_pot *p, int order, int k, int *idx) {
@@ -531,6 +809,8 @@ int orc_basis_index(const orc_pot *p, int order, int k, int *idx) {
     case 2: if (k < 0 || k >= p->nvar2) return -1; memcpy(idx, p->b2index + 4 * k, 4 * sizeof(int)); return 4;
     case 3: if (k < 0 || k >= p->nvar3) return -1; memcpy(idx, p->b3index + 9 * k, 9 * sizeof(int)); return 9;
     case 4: if (k < 0 || k >= p->nvar4) return -1; memcpy(idx, p->b4index + 15 * k, 15 * sizeof(int)); return 15;
+    case 5: if (k < 0 || k >= p->nvar5) return -1; memcpy(idx, p->b5index + 21 * k, 21 * sizeof(int)); return 21;
+    case 6: if (k < 0 || k >= p->nvar6) return -1; memcpy(idx, p->b6index + 27 * k, 27 * sizeof(int)); return 27;
     }
     return -1;
 }
@@ -989,6 +1269,12 @@ static orc_pot *load_impl(const char *path, const char *element, double rcut, do
             } else if (norder == 4 && ni >= 15) {
                 for (int k = 0; k < p->nvar4; ++k)
                     if (!memcmp(p->b4index + 15 * k, iv, 15 * sizeof(int))) p->c4[k * nd_max + nd - 1] = xclu;
+            } else if (norder == 5 && ni >= 21) {
+                for (int k = 0; k < p->nvar5; ++k)
+                    if (!memcmp(p->b5index + 21 * k, iv, 21 * sizeof(int))) p->c5[k * nd_max + nd - 1] = xclu;
+            } else if (norder == 6 && ni >= 27) {
+                for (int k = 0; k < p->nvar6; ++k)
+                    if (!memcmp(p->b6index + 27 * k, iv, 27 * sizeof(int))) p->c6[k * nd_max + nd - 1] = xclu;
             }
         } else if (!strcmp(ptype, "par")) {
             if (norder < 1 || norder > 2) { snprintf(g_err, sizeof g_err, "norder < nvarp1 required for par."); goto fail; }
@@ -1081,7 +1367,7 @@ typedef struct {
     /* per atom */
     cplx *abase, *abase2, *abase2m;
     double *b;
-    cplx *db2, *db2_0, *db3, *db3_0, *db4, *db4_0;
+    cplx *db2, *db2_0, *db3, *db3_0, *db4, *db4_0, *db5, *db5_0, *db6, *db6_0;
     cplx *dw, *dw2, *dw0; double *dw1;
 } work;
 
@@ -1106,7 +1392,7 @@ static void work_init(const orc_pot *p, work *w) {
     w->abase = xcalloc((size_t)h->nr3_a * p->lsa * h->nr3_m * p->lsm, sizeof(cplx));
     w->abase2 = xcalloc((size_t)(h->nb_a ? h->nb_a : 1) * h->nr2_m * p->lsm, sizeof(cplx));
     w->abase2m = xcalloc((size_t)h->nr2_m * p->lsm, sizeof(cplx));
-    w->b = xcalloc((size_t)p->nvar1 + p->nvar2 + p->nvar3 + p->nvar4, 8);
+    w->b = xcalloc((size_t)p->nvar1 + p->nvar2 + p->nvar3 + p->nvar4 + p->nvar5 + p->nvar6, 8);
     int w2 = 2 * h->l2m + 1, w3a = 2 * h->l3a + 1, w3m = 2 * h->l3m + 1, w4a = 2 * h->l4a + 1,
         w4m = 2 * h->l4m + 1;
     int w30 = 2 * (h->l3a > h->l3m ? h->l3a : h->l3m) + 1, w40 = 2 * (h->l4a > h->l4m ? h->l4a : h->l4m) + 1;
@@ -1116,6 +1402,10 @@ static void work_init(const orc_pot *p, work *w) {
     w->db3_0 = xcalloc((size_t)p->nvar3 * w30, sizeof(cplx));
     w->db4 = xcalloc((size_t)p->nvar4 * 3 * w4a * w4m, sizeof(cplx));
     w->db4_0 = xcalloc((size_t)p->nvar4 * w40, sizeof(cplx));
+    w->db5 = xcalloc((size_t)p->nvar5 * 4 * (2 * h->l5a + 1) * (2 * h->l5m + 1) + 1, sizeof(cplx));
+    w->db5_0 = xcalloc((size_t)p->nvar5 * (2 * h->l5m + 1) + 1, sizeof(cplx));
+    w->db6 = xcalloc((size_t)p->nvar6 * 5 * (2 * h->l6a + 1) * (2 * h->l6m + 1) + 1, sizeof(cplx));
+    w->db6_0 = xcalloc((size_t)p->nvar6 * (2 * h->l6m + 1) + 1, sizeof(cplx));
     w->dw = xcalloc((size_t)w3a * (h->l3a + 1) * h->nr3_a * (h->l3m + 1) * (h->l3m + 1) * h->nr3_m, sizeof(cplx));
     w->dw2 = xcalloc((size_t)(h->nb_a ? h->nb_a : 1) * (h->l2m + 1) * (h->l2m + 1) * h->nr2_m, sizeof(cplx));
     w->dw0 = xcalloc((size_t)(h->l2m + 1) * (h->l2m + 1) * h->nr2_m, sizeof(cplx));
@@ -1127,6 +1417,7 @@ static void work_free(work *w) {
     free(w->frm); free(w->dfrm); free(w->sha); free(w->dsha); free(w->shm); free(w->dshm);
     free(w->abase); free(w->abase2); free(w->abase2m); free(w->b);
     free(w->db2); free(w->db2_0); free(w->db3); free(w->db3_0); free(w->db4); free(w->db4_0);
+    free(w->db5); free(w->db5_0); free(w->db6); free(w->db6_0);
     free(w->dw); free(w->dw2); free(w->dw0); free(w->dw1);
 }
 
@@ -1329,8 +1620,60 @@ static int eval_atom(const orc_pot *p, work *w, const job *J, int ia) {
             b4[c] = creal(xiy);
         }
     }
+    /* 5- and 6-body functions (b_basis.F90:320-532): B and dB/dA per factor, dB/dA0 */
+    double *b5 = b4 + p->nvar4, *b6 = b5 + p->nvar5;
+    const int W5A = 2 * h->l5a + 1, W5M = 2 * h->l5m + 1, W6A = 2 * h->l6a + 1, W6M = 2 * h->l6m + 1;
+#define DB5(s, c, m, mp) w->db5[((((size_t)(c) * 4 + (s)) * W5A + (m) + h->l5a) * W5M) + (mp) + h->l5m]
+#define DB6(s, c, m, mp) w->db6[((((size_t)(c) * 5 + (s)) * W6A + (m) + h->l6a) * W6M) + (mp) + h->l6m]
+    memset(w->db5, 0, sizeof(cplx) * ((size_t)p->nvar5 * 4 * W5A * W5M + 1));
+    memset(w->db5_0, 0, sizeof(cplx) * ((size_t)p->nvar5 * W5M + 1));
+    for (int c = 0; c < p->nvar5; ++c) {
+        const int *bi = p->b5index + 21 * c;
+        const int n0 = bi[4] - 1, l0 = bi[13];
+        cplx xiy = 0;
+        for (int e = p->m5ptr[c]; e < p->m5ptr[c + 1]; ++e) {
+            const int *mm = p->m5lm + 9 * e;
+            const double cf = p->m5cf[e];
+            cplx ap[5];
+            ap[0] = AB2M(n0, shx(l2m, l0, mm[4]));
+            for (int i = 0; i < 4; ++i)
+                ap[1 + i] = AB(bi[i] - 1, shx(l3a, bi[9 + i], mm[i]), bi[5 + i] - 1, shx(l2m, bi[14 + i], mm[5 + i]));
+            cplx xb1 = ap[0] * cf * ap[1] * ap[2], xb2 = ap[3] * ap[4];
+            xiy += xb1 * xb2;
+            DB5(0, c, mm[0], mm[5]) += ap[0] * cf * ap[2] * xb2;
+            DB5(1, c, mm[1], mm[6]) += ap[0] * cf * ap[1] * xb2;
+            DB5(2, c, mm[2], mm[7]) += xb1 * ap[4];
+            DB5(3, c, mm[3], mm[8]) += xb1 * ap[3];
+            w->db5_0[(size_t)c * W5M + mm[4] + h->l5m] += cf * ap[1] * ap[2] * xb2;
+        }
+        b5[c] = creal(xiy);
+    }
+    memset(w->db6, 0, sizeof(cplx) * ((size_t)p->nvar6 * 5 * W6A * W6M + 1));
+    memset(w->db6_0, 0, sizeof(cplx) * ((size_t)p->nvar6 * W6M + 1));
+    for (int c = 0; c < p->nvar6; ++c) {
+        const int *bi = p->b6index + 27 * c;
+        const int n0 = bi[5] - 1, l0 = bi[16];
+        cplx xiy = 0;
+        for (int e = p->m6ptr[c]; e < p->m6ptr[c + 1]; ++e) {
+            const int *mm = p->m6lm + 11 * e;
+            const double cf = p->m6cf[e];
+            cplx ap[6];
+            ap[0] = AB2M(n0, shx(l2m, l0, mm[5]));
+            for (int i = 0; i < 5; ++i)
+                ap[1 + i] = AB(bi[i] - 1, shx(l3a, bi[11 + i], mm[i]), bi[6 + i] - 1, shx(l2m, bi[17 + i], mm[6 + i]));
+            cplx xb1 = ap[0] * cf * ap[1] * ap[2], xb2 = ap[3] * ap[4] * ap[5];
+            xiy += xb1 * xb2;
+            DB6(0, c, mm[0], mm[6]) += ap[0] * cf * ap[2] * xb2;
+            DB6(1, c, mm[1], mm[7]) += ap[0] * cf * ap[1] * xb2;
+            DB6(2, c, mm[2], mm[8]) += xb1 * ap[4] * ap[5];
+            DB6(3, c, mm[3], mm[9]) += xb1 * ap[3] * ap[5];
+            DB6(4, c, mm[4], mm[10]) += xb1 * ap[3] * ap[4];
+            w->db6_0[(size_t)c * W6M + mm[5] + h->l6m] += cf * ap[1] * ap[2] * xb2;
+        }
+        b6[c] = creal(xiy);
+    }
     if (J->bout) {
-        size_t nb = (size_t)p->nvar1 + p->nvar2 + p->nvar3 + p->nvar4;
+        size_t nb = (size_t)p->nvar1 + p->nvar2 + p->nvar3 + p->nvar4 + p->nvar5 + p->nvar6;
         memcpy(J->bout + nb * ia, w->b, nb * sizeof(double));
     }
 
@@ -1344,6 +1687,8 @@ static int eval_atom(const orc_pot *p, work *w, const job *J, int ia) {
         if (nr2m > 0) { s = 0; for (int k = 0; k < p->nvar2; ++k) s += p->c2[k * nd + d] * b2[k]; rho[d] += s; }
         if (nr3a > 0 || nr3m > 0) { s = 0; for (int k = 0; k < p->nvar3; ++k) s += p->c3[k * nd + d] * b3[k]; rho[d] += s; }
         if (h->nr4_a > 0 || h->nr4_m > 0) { s = 0; for (int k = 0; k < p->nvar4; ++k) s += p->c4[k * nd + d] * b4[k]; rho[d] += s; }
+        if (h->nr5_a > 0 || h->nr5_m > 0) { s = 0; for (int k = 0; k < p->nvar5; ++k) s += p->c5[k * nd + d] * b5[k]; rho[d] += s; }
+        if (h->nr6_a > 0 || h->nr6_m > 0) { s = 0; for (int k = 0; k < p->nvar6; ++k) s += p->c6[k * nd + d] * b6[k]; rho[d] += s; }
     }
     for (int d = 0; d < nd; ++d) {
         double F, dF;
@@ -1397,6 +1742,27 @@ static int eval_atom(const orc_pot *p, work *w, const job *J, int ia) {
         for (int m3 = -l3; m3 <= l3; ++m3)
             for (int m7 = 0; m7 <= l7; ++m7) DW(m3, l3, bi[2] - 1, m7, l7, bi[6] - 1) += ce * DB4(2, c, m3, m7);
         for (int m4 = 0; m4 <= l4; ++m4) DW0(m4, l4, bi[3] - 1) += ce * w->db4_0[(size_t)c * W40 + m4 + L40];
+    }
+    /* 5- and 6-body weights (forces.F90:232-330) */
+    for (int c = 0; c < p->nvar5; ++c) {
+        const int *bi = p->b5index + 21 * c;
+        double ce = CEFF(p->c5, c);
+        for (int i = 0; i < 4; ++i) {
+            const int l = bi[9 + i], lp = bi[14 + i];
+            for (int m = -l; m <= l; ++m)
+                for (int mp = 0; mp <= lp; ++mp) DW(m, l, bi[i] - 1, mp, lp, bi[5 + i] - 1) += ce * DB5(i, c, m, mp);
+        }
+        for (int m0 = 0; m0 <= bi[13]; ++m0) DW0(m0, bi[13], bi[4] - 1) += ce * w->db5_0[(size_t)c * W5M + m0 + h->l5m];
+    }
+    for (int c = 0; c < p->nvar6; ++c) {
+        const int *bi = p->b6index + 27 * c;
+        double ce = CEFF(p->c6, c);
+        for (int i = 0; i < 5; ++i) {
+            const int l = bi[11 + i], lp = bi[17 + i];
+            for (int m = -l; m <= l; ++m)
+                for (int mp = 0; mp <= lp; ++mp) DW(m, l, bi[i] - 1, mp, lp, bi[6 + i] - 1) += ce * DB6(i, c, m, mp);
+        }
+        for (int m0 = 0; m0 <= bi[16]; ++m0) DW0(m0, bi[16], bi[5] - 1) += ce * w->db6_0[(size_t)c * W6M + m0 + h->l6m];
     }
 
     /* ---- forcecompute (forces.F90:418-623) */

@@ -19,8 +19,8 @@ typedef struct {
     int nb_a, nb_m, nr2_m, l2m;   /* "nradbase l2max" */
     int nr3_a, nr3_m, l3a, l3m;   /* "nradial3 l3max" */
     int nr4_a, nr4_m, l4a, l4m;   /* "nradial4 l4max" */
-    int nr5_a, nr5_m, l5a, l5m;   /* must be 0 */
-    int nr6_a, nr6_m, l6a, l6m;   /* must be 0 */
+    int nr5_a, nr5_m, l5a, l5m;   /* "nradial5 l5max" */
+    int nr6_a, nr6_m, l6a, l6m;   /* "nradial6 l6max" */
     int ndensity, npot, nfemb, radtype_a, radtype_m;
     double lambda_a, lambda_m, cut_a, cut_m, dcut_a, dcut_m, ecut, decut;
 } orc_header;
@@ -38,7 +38,7 @@ orc_pot *orc_load(const char *path, const char *element, double rcut, double rcu
 void orc_free(orc_pot *p);
 
 /* nvar[0..3] = number of B functions of orders 1..4 */
-void orc_counts(const orc_pot *p, int nvar[4]);
+void orc_counts(const orc_pot *p, int nvar[6]);
 /* index tuple (Fortran b*index column, 1-based radial, l values) of B function k (0-based) */
 int orc_basis_index(const orc_pot *p, int order, int k, int *idx);
 

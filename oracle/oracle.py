@@ -120,13 +120,14 @@ class Oracle:
             self.h = None
 
     def counts(self):
-        n = np.zeros(4, np.int32)
+        """Numbers of B functions of orders 1..6."""
+        n = np.zeros(6, np.int32)
         lib().orc_counts(self.h, n)
         return [int(x) for x in n]
 
     def basis(self, order):
         n = self.counts()[order - 1]
-        width = {1: 1, 2: 4, 3: 9, 4: 15}[order]
+        width = {1: 1, 2: 4, 3: 9, 4: 15, 5: 21, 6: 27}[order]
         out = np.zeros((n, width), np.int32)
         buf = np.zeros(width, np.int32)
         for k in range(n):

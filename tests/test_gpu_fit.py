@@ -83,7 +83,7 @@ def gradient(batch, c, gE, gF):
     return g.cpu().numpy()
 
 
-@pytest.mark.parametrize("header", ["H500", "H500ND2LIN"])
+@pytest.mark.parametrize("header", ["H500", "H500ND2LIN", "H56"])
 def test_energy_gradient_equals_oracle_b_vectors(tmp_path, header):
     say("fit: energy gradient vs oracle B vectors,", header)
     pot = load(tmp_path, header, 2040)
@@ -101,7 +101,7 @@ def test_energy_gradient_equals_oracle_b_vectors(tmp_path, header):
         want1 += w * o.bvectors(a).sum(axis=0)
     # coefficient order: per body order, density-major
     cuts = np.cumsum([0] + list(o.counts()))
-    want = np.concatenate([np.tile(want1[cuts[k]:cuts[k + 1]], nd) for k in range(4)])
+    want = np.concatenate([np.tile(want1[cuts[k]:cuts[k + 1]], nd) for k in range(6)])
     assert len(want) == len(g)
     err = np.abs(g - want).max() / np.abs(want).max()
     say("  rel err %.2e" % err)

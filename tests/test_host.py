@@ -35,9 +35,8 @@ def test_enumeration_matches_oracle(hname):
     h = HEADERS[hname]
     mine = capi.enumerate_basis(h)
     o = O.Oracle.from_header(h)
-    for k in (1, 2, 3, 4):
+    for k in (1, 2, 3, 4, 5, 6):
         assert np.array_equal(mine[k], o.basis(k)), (hname, k)
-    assert len(mine[5]) == len(mine[6]) == 0
 
 
 @pytest.mark.parametrize("name", ["o5_16", "o56_16", "l4m_16"])
