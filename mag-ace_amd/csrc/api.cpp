@@ -83,6 +83,8 @@ struct Context {
     uint4* crec64 = nullptr;
     uint4* crec32 = nullptr;
     uint4* crec16 = nullptr;  // FP32 weights, offsets of 16-byte entries (two-atom adjoint)
+    uint4* crecu = nullptr;   // FP32, offsets unpacked (DevParams::crecu)
+    uint4* crec16u = nullptr;
     double* cw2_64 = nullptr;  // density-2 record weights (ndensity 2)
     float* cw2_32 = nullptr;
     int2* chunks = nullptr;
@@ -180,7 +182,7 @@ void dfree(T*& p) {
 
 Context::~Context() {
     dfree(t64.lut_a); dfree(t64.lut_m); dfree(t32.lut_a); dfree(t32.lut_m);
-    dfree(crec64); dfree(crec32); dfree(crec16); dfree(chunks); dfree(comb); dfree(comb2); dfree(fold); dfree(cw2_64);
+    dfree(crec64); dfree(crec32); dfree(crec16); dfree(crecu); dfree(crec16u); dfree(chunks); dfree(comb); dfree(comb2); dfree(fold); dfree(cw2_64);
     dfree(ypair); dfree(yft); dfree(yfe);
     dfree(cw2_32);
     dfree(buf.Z); dfree(buf.W); dfree(buf.ehc); dfree(buf.rho1); dfree(buf.dE); dfree(buf.edf); dfree(buf.rho);
@@ -318,7 +320,7 @@ int build_context(Potential&& pot, std::unique_ptr<Context>& out) {
 int upload_records(Context& c) {
     const TermList& T = c.model.terms;
     const size_t nc = T.ncontrib();
-    std::vector<uint4> r64(nc), r32(nc), r16(nc);
+    std::vector<uint4> r64(nc), r32(nc), r16(nc), r32u(nc), r16u(nc);
     for (size_t i = 0; i < nc; ++i) {
         const uint16_t* q = &T.cidx[4 * i];
         const double w = T.cw[i];
@@ -336,10 +338,14 @@ int upload_records(Context& c) {
         std::memcpy(&lb, &lo, 4);
         r32[i] = make_uint4(((uint32_t)q[0] << 3) | ((uint32_t)q[1] << 19), (uint32_t)q[2] << 3, fb, lb);
         r16[i] = make_uint4(((uint32_t)q[0] << 4) | ((uint32_t)q[1] << 20), (uint32_t)q[2] << 4, fb, lb);
+        r32u[i] = make_uint4((uint32_t)q[0] << 3, (uint32_t)q[1] << 3, (uint32_t)q[2] << 3, fb);
+        r16u[i] = make_uint4((uint32_t)q[0] << 4, (uint32_t)q[1] << 4, (uint32_t)q[2] << 4, fb);
     }
     HIPCHK(hipMemcpy(c.crec64, r64.data(), nc * sizeof(uint4), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c.crec32, r32.data(), nc * sizeof(uint4), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c.crec16, r16.data(), nc * sizeof(uint4), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c.crecu, r32u.data(), nc * sizeof(uint4), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c.crec16u, r16u.data(), nc * sizeof(uint4), hipMemcpyHostToDevice));
     if (T.nd > 1) {
         if (!c.cw2_64) {
             HIPCHK(hipMalloc((void**)&c.cw2_64, nc * sizeof(double) + 64));
@@ -405,6 +411,8 @@ int upload(Context& c) {
     HIPCHK(hipMalloc((void**)&c.crec64, nc * sizeof(uint4) + 64));
     HIPCHK(hipMalloc((void**)&c.crec32, nc * sizeof(uint4) + 64));
     HIPCHK(hipMalloc((void**)&c.crec16, nc * sizeof(uint4) + 64));
+    HIPCHK(hipMalloc((void**)&c.crecu, nc * sizeof(uint4) + 64));
+    HIPCHK(hipMalloc((void**)&c.crec16u, nc * sizeof(uint4) + 64));
     HIPCHK(hipMalloc((void**)&c.chunks, nch * sizeof(int2) + 64));
     const size_t ncomb = M.terms.comb.size() / 2;
     HIPCHK(hipMalloc((void**)&c.comb, ncomb * sizeof(int2) + 64));
@@ -433,6 +441,8 @@ int upload(Context& c) {
     c.P64.crec = c.crec64;
     c.P32.crec = c.crec32;
     c.P32.crec16 = c.crec16;
+    c.P32.crecu = c.crecu;
+    c.P32.crec16u = c.crec16u;
     c.P64.chunks = c.P32.chunks = c.chunks;
     c.P64.comb = c.P32.comb = c.comb;
     c.P64.comb2 = c.P32.comb2 = c.comb2;

@@ -665,14 +665,23 @@ __device__ inline void embed(int nfemb, double rho, double wpre, double mexp, do
     }
 }
 
-// one complex entry of an LDS array at a byte offset, as a single aligned vector read
-template <typename T> __device__ __forceinline__ cplx<T> lds_cplx(const char* base, uint32_t off);
-template <> __device__ __forceinline__ cplx<float> lds_cplx<float>(const char* base, uint32_t off) {
-    const float2 v = *(const float2*)(base + off);
-    return {v.x, v.y};
+// An aligned vector at LDS byte address `off`. The adjoint kernels declare no static LDS (the
+// host checks this before launching them), so their dynamic LDS, and with it Z' at its start,
+// begins at address 0: a record's byte offset into Z' is the address itself, and reading it
+// this way saves the add of the dynamic base, which the compiler resolves only after
+// instruction selection and so cannot fold away.
+template <typename E, int N>
+__device__ __forceinline__ E __attribute__((ext_vector_type(N))) lds_at(uint32_t off) {
+    typedef E V __attribute__((ext_vector_type(N)));
+    return *(const __attribute__((address_space(3))) V*)(size_t)off;
 }
-template <> __device__ __forceinline__ cplx<double> lds_cplx<double>(const char* base, uint32_t off) {
-    const double2 v = *(const double2*)(base + off);
+__device__ __forceinline__ float4 lds_f4(uint32_t off) {
+    const auto v = lds_at<float, 4>(off);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+// one complex entry of Z' at a byte offset, as a single aligned vector read
+template <typename T> __device__ __forceinline__ cplx<T> lds_cplx(uint32_t off) {
+    const auto v = lds_at<T, 2>(off);
     return {v.x, v.y};
 }
 
@@ -778,7 +787,9 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int
     cplx<T>* sW = sZ + P.nzx;               // [nzx]
     cplx<T>* sW2 = sW + P.nzx;              // [nzx] (ND2 only)
     double* red = (double*)(sW + (ND2 ? 2 : 1) * P.nzx);
-    __shared__ double s_scale[2];
+    // the energy's scale factors after the two block sums; in dynamic LDS, so that it starts at
+    // address 0 and the records' offsets address sZ without an add
+    double* s_scale = red + 2 * (kAdjointThreads / 64);
     const int atom = blockIdx.x, tid = threadIdx.x;
     Zin += (size_t)atom * P.nz;
     for (int x = tid; x < P.nzx; x += blockDim.x) {
@@ -830,7 +841,10 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int
         if (c < P.nchunks) {
             const int x = ch.y & 0xfff, cnt = (ch.y >> 12) & 63;
             const bool sole = (ch.y >> 18) & 1;  // the target's only chunk
-            const uint4* __restrict__ rec = P.crec + ch.x;  // records of this lane, stride 64
+            // records of this lane, stride 64; FP32 without the recipe reads the table whose
+            // offsets need no unpacking
+            constexpr bool kU = sizeof(T) == 4 && !ACC;
+            const uint4* __restrict__ rec = (kU ? P.crecu : P.crec) + ch.x;
             const T* __restrict__ w2p = ND2 ? P.cw2 + ch.x : nullptr;
             const double* __restrict__ w2d = ND2 && ACC ? P.cw2d + ch.x : nullptr;
             if (ACC && __any(x >= P.oA0 && x < P.nz)) {
@@ -845,9 +859,8 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         const double w = (double)__uint_as_float(r[u].z) + (double)__uint_as_float(r[u].w);
-                        const char* zb0 = (const char*)sZ;
-                        const cplx<T> pr = cmul(cmul(lds_cplx<T>(zb0, r[u].x & 0xffff), lds_cplx<T>(zb0, r[u].x >> 16)),
-                                                lds_cplx<T>(zb0, r[u].y));
+                        const cplx<T> pr = cmul(cmul(lds_cplx<T>(r[u].x & 0xffff), lds_cplx<T>(r[u].x >> 16)),
+                                                lds_cplx<T>(r[u].y));
                         acd.re = fma(w, (double)pr.re, acd.re);
                         acd.im = fma(w, (double)pr.im, acd.im);
                         if (ND2) {
@@ -867,11 +880,11 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const T w = rec_weight<T>(r[u]);
-                    const char* zb0 = (const char*)sZ;  // records hold byte offsets into sZ
-                    const cplx<T> za = lds_cplx<T>(zb0, r[u].x & 0xffff);
-                    const cplx<T> zb = lds_cplx<T>(zb0, r[u].x >> 16);
-                    const cplx<T> zc = lds_cplx<T>(zb0, r[u].y);
+                    const T w = kU ? (T)__uint_as_float(r[u].w) : rec_weight<T>(r[u]);
+                    // records hold byte offsets into sZ, which starts at LDS address 0
+                    const cplx<T> za = lds_cplx<T>(kU ? r[u].x : r[u].x & 0xffff);
+                    const cplx<T> zb = lds_cplx<T>(kU ? r[u].y : r[u].x >> 16);
+                    const cplx<T> zc = lds_cplx<T>(kU ? r[u].z : r[u].y);
                     const cplx<T> pr = cmul(cmul(za, zb), zc);
                     acc.re = fma(w, pr.re, acc.re);
                     acc.im = fma(w, pr.im, acc.im);
@@ -1017,7 +1030,7 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<floa
     float4* sZ = (float4*)smem;                  // [nzx] (Z' of k_adjoint), sZ[nz] = (1, 0, 1, 0)
     float4* sW = sZ + P.nzx;                     // [nzx]
     double* red = (double*)(sW + P.nzx);         // [2 * waves]: one sum per atom and wave
-    __shared__ double s_scale[2];
+    double* s_scale = red + 2 * (kAdjointThreads / 64);  // dynamic LDS, as in k_adjoint
     const int a0 = 2 * (int)blockIdx.x, tid = threadIdx.x;
     const bool two = a0 + 1 < natom;
     const cplx<T>* Z0 = Zin + (size_t)a0 * P.nz;
@@ -1054,7 +1067,6 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<floa
     bool split = false;
     if (rot > 0 && tid + rot * nthr < P.nchunks) split = ((P.chunks[tid + (rot - 1) * nthr].y ^ ch.y) & 0xfff) == 0;
     bool hold = split;
-    const char* zb = (const char*)sZ;
     for (int j0 = 0; j0 < nj; ++j0) {
         const int c = tid + j * nthr;
         const int jn = j + 1 < nj ? j + 1 : 0;
@@ -1073,9 +1085,9 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<floa
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         const double w = (double)__uint_as_float(r[u].z) + (double)__uint_as_float(r[u].w);
-                        const float4 za = *(const float4*)(zb + (r[u].x & 0xffff));
-                        const float4 zbv = *(const float4*)(zb + (r[u].x >> 16));
-                        const float4 zc = *(const float4*)(zb + r[u].y);
+                        const float4 za = lds_f4(r[u].x & 0xffff);
+                        const float4 zbv = lds_f4(r[u].x >> 16);
+                        const float4 zc = lds_f4(r[u].y);
                         const cplx<T> p0 = cmul(cmul(cplx<T>{za.x, za.y}, cplx<T>{zbv.x, zbv.y}), cplx<T>{zc.x, zc.y});
                         const cplx<T> p1 = cmul(cmul(cplx<T>{za.z, za.w}, cplx<T>{zbv.z, zbv.w}), cplx<T>{zc.z, zc.w});
                         acd0.re = fma(w, (double)p0.re, acd0.re);
@@ -1091,10 +1103,11 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<floa
                 for (int u = 0; u < 4; ++u) r[u] = rec[(size_t)(k + u) * 64];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const T w = __uint_as_float(r[u].z);
-                    const float4 za = *(const float4*)(zb + (r[u].x & 0xffff));
-                    const float4 zbv = *(const float4*)(zb + (r[u].x >> 16));
-                    const float4 zc = *(const float4*)(zb + r[u].y);
+                    // without the recipe the table's offsets need no unpacking (crec16u)
+                    const T w = __uint_as_float(ACC ? r[u].z : r[u].w);
+                    const float4 za = lds_f4(ACC ? r[u].x & 0xffff : r[u].x);
+                    const float4 zbv = lds_f4(ACC ? r[u].x >> 16 : r[u].y);
+                    const float4 zc = lds_f4(ACC ? r[u].y : r[u].z);
                     const cplx<T> p0 = cmul(cmul(cplx<T>{za.x, za.y}, cplx<T>{zbv.x, zbv.y}), cplx<T>{zc.x, zc.y});
                     const cplx<T> p1 = cmul(cmul(cplx<T>{za.z, za.w}, cplx<T>{zbv.z, zbv.w}), cplx<T>{zc.z, zc.w});
                     acc0.re = fma(w, p0.re, acc0.re);
@@ -2245,6 +2258,8 @@ static hipError_t run_class(const DevParams<T>& P, int natom, int atom_base, con
             const int li = i < 5 ? i : 6;
             hipFuncAttributes fa;
             if ((e = hipFuncGetAttributes(&fa, ks[i])) != hipSuccess) return e;
+            // the adjoints address their dynamic LDS from 0 (lds_at)
+            if ((i == 1 || i == 3) && fa.sharedSizeBytes != 0) return hipErrorInvalidKernelFile;
             limit[li] = (size_t)maxsh > fa.sharedSizeBytes ? (size_t)maxsh - fa.sharedSizeBytes : 0;
             if ((e = hipFuncSetAttribute(ks[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)limit[li])) !=
                 hipSuccess)
@@ -2253,6 +2268,7 @@ static hipError_t run_class(const DevParams<T>& P, int natom, int atom_base, con
         if constexpr (sizeof(T) == 4) {
             hipFuncAttributes fa;
             if ((e = hipFuncGetAttributes(&fa, (const void*)k_adjoint_pair<ACC>)) != hipSuccess) return e;
+            if (fa.sharedSizeBytes != 0) return hipErrorInvalidKernelFile;
             limit[5] = (size_t)maxsh > fa.sharedSizeBytes ? (size_t)maxsh - fa.sharedSizeBytes : 0;
             if ((e = hipFuncSetAttribute((const void*)k_adjoint_pair<ACC>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)limit[5])) != hipSuccess)
@@ -2262,10 +2278,10 @@ static hipError_t run_class(const DevParams<T>& P, int natom, int atom_base, con
     }
     // FP32 with one density: two atoms per adjoint block (their Z and W interleaved in LDS)
     const bool pair = sizeof(T) == 4 && !nd2 && P.crec16 != nullptr;
-    const size_t lds2p = 2 * (size_t)P.nzx * 16 + 2 * (kAdjointThreads / 64) * sizeof(double);
+    const size_t lds2p = 2 * (size_t)P.nzx * 16 + (2 * (kAdjointThreads / 64) + 2) * sizeof(double);
     if (pair && lds2p > limit[5]) return hipErrorInvalidValue;
     const size_t lds2 =
-        (nd2 ? 3 : 2) * (size_t)P.nzx * sizeof(cplx<T>) + 2 * (kAdjointThreads / 64) * sizeof(double);
+        (nd2 ? 3 : 2) * (size_t)P.nzx * sizeof(cplx<T>) + (2 * (kAdjointThreads / 64) + 2) * sizeof(double);
     const size_t lds3 = (size_t)P.nz * sizeof(cplx<T>);
     if (lds1 > limit[0] || lds2 > limit[nd2 ? 3 : 1] || lds3 > limit[q1 ? (r4 ? 6 : 2) : 4])
         return hipErrorInvalidValue;
@@ -2277,7 +2293,7 @@ static hipError_t run_class(const DevParams<T>& P, int natom, int atom_base, con
     if constexpr (sizeof(T) == 4) {
         if (pair)
             hipLaunchKernelGGL(k_adjoint_pair<ACC>, dim3((natom + 1) / 2), dim3(kAdjointThreads), lds2p, s, P, atom_base,
-                               P.crec16, (const cplx<T>*)B.Z, (cplx<T>*)B.W, (char*)B.Pt, B.ehc, B.rho1, eout, B.dE,
+                               ACC ? P.crec16 : P.crec16u, (const cplx<T>*)B.Z, (cplx<T>*)B.W, (char*)B.Pt, B.ehc, B.rho1, eout, B.dE,
                                B.edf, B.rho, natom);
     }
     if (pair) {

@@ -36,6 +36,11 @@ struct DevParams {
     const uint4* crec = nullptr;
     const uint4* crec16 = nullptr;   // FP32 only: the records with offsets of 16-byte entries (two
                                      // atoms' Z interleaved, k_adjoint_pair)
+    // FP32 without the accumulation recipe: the same records with the three offsets in x, y, z
+    // as they are (no unpacking) and the FP32 weight in w; 8-byte entries (crecu) and 16-byte
+    // entries (crec16u, k_adjoint_pair)
+    const uint4* crecu = nullptr;
+    const uint4* crec16u = nullptr;
     const int2* chunks = nullptr;    // (start, target | count << 12 | sole << 18 | slot << 19), count = its
                                      // group's length; chunk c runs on thread c % kAdjointThreads
     // partial-sum combine (TermList::comb, comb2): groups of <= kCombGroup slots
