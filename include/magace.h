@@ -238,11 +238,17 @@ int magace_model_terms(unsigned short *idx, double *w, int cap);
  * x, the gather form of src/forces.F90:84-349): chunks[2c] = first record, chunks[2c+1] =
  * x | count << 12 | sole << 18 | slot << 19 (unsigned; count = the length of the chunk's
  * group of 64, sole = the target's only chunk, slot = the partial-sum slot of the chunk's run
- * of consecutive same-target chunks on one adjoint thread); record r holds the
- * factors idx[4r..4r+2] and the weight w[r] (density 1); a chunk contributes
- * sum_j w Z[a_j] Z[b_j] Z[c_j] over its count records, 64 apart.  Copies when the caps
- * suffice; returns the record count (chunk count in *nchunks).  Host-side introspection. */
+ * of consecutive same-target chunks on one adjoint thread); record r holds the factors
+ * idx[4r..4r+2], indices into the extended vector Z' (magace_model_products), and the weight
+ * w[r] (density 1); a chunk contributes sum_j w Z'[a_j] Z'[b_j] Z'[c_j] over its count
+ * records, 64 apart.  Copies when the caps suffice; returns the record count (chunk count in
+ * *nchunks).  Host-side introspection. */
 int magace_model_records(int *chunks, int cap_chunks, unsigned short *idx, double *w, int cap, int *nchunks);
+/* The products Y of the extended vector Z' = [Z | 1 | Y] that the monomials of five and six
+ * factors (body orders 5, 6) and the gradient records address: entry nz + 1 + i =
+ * Z[pairs[2i]] Z[pairs[2i+1]].  Copies when cap >= the count; returns the count.  Host-side
+ * introspection. */
+int magace_model_products(int *pairs, int cap);
 
 #ifdef __cplusplus
 }

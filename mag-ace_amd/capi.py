@@ -29,7 +29,7 @@ EXPORTS = ("acelibini_", "acelib_", "magace_last_error", "magace_error_string", 
            "magace_md_read", "magace_md_destroy", "magace_fit_ncoef", "magace_fit_get_coefficients",
            "magace_fit_set_coefficients", "magace_fit_forward", "magace_fit_backward", "magace_write_potential",
            "magace_model_records", "magace_compute_device_noalloc", "magace_set_tuning", "magace_get_tuning",
-           "magace_tune")
+           "magace_tune", "magace_model_products")
 
 ERR_CAPACITY = 4
 
@@ -282,6 +282,19 @@ def model_records():
     w = np.zeros(max(n, 1))
     L.magace_model_records(ch.ctypes.data, nch.value, idx.ctypes.data, w.ctypes.data, n, C.byref(nch))
     return ch[:nch.value], idx[:n].astype(np.int64), w[:n]
+
+
+def model_products():
+    """Products Y of the extended vector Z' (body orders 5, 6): (n, 2) factor pairs, entry
+    nz + 1 + i = Z[pairs[i, 0]] Z[pairs[i, 1]]."""
+    L = lib()
+    L.magace_model_products.argtypes = [C.c_void_p, C.c_int]
+    n = L.magace_model_products(None, 0)
+    if n < 0:
+        _check(n)
+    p = np.zeros((max(n, 1), 2), np.int32)
+    L.magace_model_products(p.ctypes.data, n)
+    return p[:n].astype(np.int64)
 
 
 def profile(enable=True):

@@ -1226,6 +1226,16 @@ int magace_model_records(int* chunks, int cap_chunks, uint16_t* idx, double* w, 
     return n;
 }
 
+int magace_model_products(int* pairs, int cap) {
+    if (!g_ctx) {
+        set_error(MAGACE_ERR_STATE, "library not initialised");
+        return -MAGACE_ERR_STATE;
+    }
+    const TermList& T = g_ctx->model.terms;
+    if (pairs && cap >= T.ny && T.ny) std::memcpy(pairs, T.ypair.data(), (size_t)T.ny * 2 * sizeof(int32_t));
+    return T.ny;
+}
+
 int magace_fit_ncoef(void) {
     if (!g_ctx) {
         set_error(MAGACE_ERR_STATE, "library not initialised");

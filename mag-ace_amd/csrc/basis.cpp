@@ -417,6 +417,16 @@ struct KeyHash {
 // LDS passes of one read instruction for one 16-lane group: lanes that read the same entry are
 // served by one broadcast, distinct entries of one class x mod 16 (the same banks) take one
 // pass each, so the cost is the largest number of distinct entries in one class.
+//
+// The 16-lane groups of a 16-byte read (ds_read_b128) on gfx950, measured with the LDS
+// bank-conflict counter (a pair of lanes reading entries 0 and 16, every other lane one shared
+// entry, conflicts exactly when the two lanes share a group): in each half of the wave, the
+// quads (lanes 4q..4q+3, q < 8) with an even number of set bits in q form one group
+// ({0-3, 12-15, 20-23, 24-27}) and those with an odd number the other.
+static inline int lds_group(int lane) {
+    const int q = (lane >> 2) & 7;
+    return ((lane >> 5) << 1) | ((q ^ (q >> 1) ^ (q >> 2)) & 1);
+}
 struct BankCell {
     int n = 0;
     int val[16], mult[16];
@@ -455,21 +465,20 @@ struct BankCell {
 // base): swaps two records of one lane between steps while that lowers the group's LDS passes
 // (BankCell, for every step, 16-lane group and factor slot; padding, index `one`, is a
 // broadcast and not counted).  The greedy's last steps have few records left to choose from;
-// the swaps repair them.  One sweep over the lanes: on the lmax-6 header it takes the modelled
-// passes per read from 1.53 to 1.41 (a second sweep: 1.405, for twice the init time, 0.8 s).
+// the swaps repair them (one sweep over the lanes).
 static void refine_banks(TermList& T, size_t base, int len, int nl, int one) {
-    const int ng = (nl + 15) / 16;
+    const int ng = 4;
     std::vector<BankCell> cell((size_t)len * ng * 3);  // [step][group][factor]
     auto at = [&](int k, int g, int f) -> BankCell& { return cell[((size_t)k * ng + g) * 3 + f]; };
     for (int k = 0; k < len; ++k)
         for (int L = 0; L < nl; ++L)
             for (int f = 0; f < 3; ++f) {
                 const int v = T.cidx[4 * (base + (size_t)k * 64 + L) + f];
-                if (v != one) at(k, L >> 4, f).add(v);
+                if (v != one) at(k, lds_group(L), f).add(v);
             }
     {
         for (int L = 0; L < nl; ++L) {
-            const int g = L >> 4;
+            const int g = lds_group(L);
             for (int k1 = 0; k1 < len; ++k1)
                 for (int k2 = k1 + 1; k2 < len; ++k2) {
                     const size_t r1 = base + (size_t)k1 * 64 + L, r2 = base + (size_t)k2 * 64 + L;
@@ -918,12 +927,14 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
     // Z[a], Z[b], Z[c] of their k-th records from LDS (three reads, random addresses), and
     // LDS serves a wave in lane groups whose accesses must fall in distinct banks.  The
     // order inside each chunk is therefore chosen greedily, step by step, so that the
-    // distinct factor indices of each group of 16 lanes spread over the 16 classes x mod 16
-    // (BankCell): a 16-byte entry covers 4 of the 64 four-byte banks, so one pass serves 16
-    // lanes, and lanes reading the same entry share a broadcast.  The FP64 and the two-atom
-    // FP32 adjoints read such entries.  Of the lane groupings tried (x mod 8 over 8 lanes,
-    // mod 16 over 16, mod 32 over 32, all three summed, none), 16 was the fastest.  A local
-    // search (refine_banks) then repairs the last steps.  Padding (index nz) is a broadcast.
+    // distinct factor indices of each 16-lane group (lds_group) spread over the 16 classes
+    // x mod 16 (BankCell): a 16-byte entry covers 4 of the 64 four-byte banks, so one pass
+    // serves 16 lanes, and lanes reading the same entry share a broadcast.  The FP64 and the
+    // two-atom FP32 adjoints read such entries.  With the measured groups in place of
+    // consecutive lanes, the passes per read of the lmax-6 header fell from 1.83 to 1.38
+    // (bank-conflict cycles per LDS instruction of the two-atom FP32 adjoint 3.2 -> 1.5;
+    // adjoint 4-6% faster).  A local search (refine_banks) then repairs the last steps.
+    // Padding (index nz) is a broadcast.
     for (size_t g0 = 0; g0 < order.size(); g0 += 64) {
         const size_t g1 = std::min(order.size(), g0 + 64);
         int len = 0;
@@ -948,7 +959,7 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
                     int cost = 0;
                     for (int f = 0; f < 3; ++f) {
                         if (e.first[f] == one) continue;
-                        const BankCell& bc = sp[f][L >> 4];
+                        const BankCell& bc = sp[f][lds_group(L)];
                         if (!bc.has(e.first[f])) cost += bc.load(e.first[f] & 15);
                     }
                     if (cost < bestc) { bestc = cost; best = i; }
@@ -956,7 +967,7 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
                 const auto& e = by[ch.x][ch.off + rem[best]];
                 for (int f = 0; f < 3; ++f)
                     if (e.first[f] != one)
-                        sp[f][L >> 4].add(e.first[f]);
+                        sp[f][lds_group(L)].add(e.first[f]);
                 const size_t r = base + (size_t)k * 64 + L;
                 T.cidx[4 * r] = (uint16_t)e.first[0];
                 T.cidx[4 * r + 1] = (uint16_t)e.first[1];

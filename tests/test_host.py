@@ -207,28 +207,39 @@ print("counts", capi.basis_counts())
     assert "counts [2, 36, 30, 186, 0, 0]" in p.stdout
 
 
-@pytest.mark.parametrize("hname", ["HTINY", "H500", "H2000", "H500ND2"])
+@pytest.mark.parametrize("hname", ["HTINY", "H500", "H2000", "H500ND2", "H56"])
 def test_gradient_records_equal_the_monomial_derivative(hname, tmp_path):
     """The adjoint's transposed table reproduces dRho/dZ[x] of the monomial table for a
     random complex Z, entry by entry: the brute-force derivative differentiates every slot
-    of every monomial.  Each chunk header holds its group's length: the 64 chunks of a
-    group have equal lengths, and records past a chunk's end carry weight 0; a chunk is flagged
-    sole exactly when it is its target's only one (the kernel then stores instead of adding)."""
+    of every monomial (products Y of body orders 5 and 6 are formed from Z, and a Y target's
+    gradient goes back to its two factors).  Each chunk header holds its group's length: the
+    64 chunks of a group have equal lengths, and records past a chunk's end carry weight 0; a
+    chunk is flagged sole exactly when it is its target's only one (the kernel then stores
+    instead of adding)."""
     pot = write_potential(str(tmp_path / "Fe.pbe.in"), hname, 31)
     capi.init_file(pot)
     nz = capi.model_info()["nz"]
     idx, w = capi.model_terms()
     ch, ridx, rw = capi.model_records()
+    prod = capi.model_products()
+    ny = len(prod)
     rng = np.random.default_rng(5)
-    Z = rng.normal(size=nz + 1) + 1j * rng.normal(size=nz + 1)
+    Z = np.zeros(nz + 1 + len(prod), complex)
+    Z[:nz] = rng.normal(size=nz) + 1j * rng.normal(size=nz)
     Z[nz] = 1.0
-    want = np.zeros(nz + 1, complex)
+    for i, (a, b) in enumerate(prod):
+        assert max(a, b) < nz + 1 + i  # factors come before the product
+        Z[nz + 1 + i] = Z[a] * Z[b]
+    nzx = nz + 1 + ny
+    assert (ny > 0) == (hname == "H56")
+    want = np.zeros(nzx, complex)
     F = Z[idx]
     for s in range(4):
         np.add.at(want, idx[:, s], w * np.prod(np.delete(F, s, axis=1), axis=1))
     y = ch[:, 1].astype(np.int64) & 0xFFFFFFFF
     first, x, cnt, sole, slot = ch[:, 0], y & 0xFFF, (y >> 12) & 63, (y >> 18) & 1, y >> 19
-    assert np.all(x < nz) and np.all(cnt % 4 == 0) and np.all((cnt >= 4) & (cnt <= 32))
+    assert np.all(x < nzx) and np.all(cnt % 4 == 0) and np.all((cnt >= 4) & (cnt <= 32))
+    assert np.all(ridx[:, :3] < nzx)
     nchunk_of = np.bincount(x, minlength=nz)
     assert np.array_equal(sole == 1, nchunk_of[x] == 1)  # flagged exactly when the target has one chunk
     # runs (a lane's chunks in consecutive rounds with one target) of targets with several
@@ -250,12 +261,40 @@ def test_gradient_records_equal_the_monomial_derivative(hname, tmp_path):
     assert np.all(np.diff(tg) >= 0)  # consecutive per target
     for g0 in range(0, len(ch), 64):
         assert np.all(cnt[g0:g0 + 64] == cnt[g0]) and np.all(first[g0:g0 + 64] == first[g0] + np.arange(len(ch[g0:g0 + 64])))
-    got = np.zeros(nz + 1, complex)
+    got = np.zeros(nzx, complex)
     for c in range(len(ch)):
         r = first[c] + 64 * np.arange(cnt[c])
         got[x[c]] += np.sum(rw[r] * Z[ridx[r, 0]] * Z[ridx[r, 1]] * Z[ridx[r, 2]])
+    for i in range(ny):  # a Y target's gradient goes back to its factors
+        a, b = prod[i]
+        for t in ((a, b), (b, a)):
+            want[t[0]] += want[nz + 1 + i] * Z[t[1]]
+            got[t[0]] += got[nz + 1 + i] * Z[t[1]]
     scale = np.abs(want[:nz]).max()
     assert np.abs(got[:nz] - want[:nz]).max() <= 1e-12 * scale
+    if hname == "H2000":
+        # LDS passes per 16-byte read that the record order gives the adjoint: per step of a
+        # group of 64 chunks, per 16-lane group of the read (gfx950: in each half wave, the
+        # quads 4q..4q+3 with an even number of set bits in q, and those with an odd number)
+        # and factor slot, the largest number of distinct entries in one bank class x mod 16
+        # (padding, index nz, is a broadcast and not counted)
+        lane = np.arange(64)
+        q = (lane >> 2) & 7
+        grp = ((lane >> 5) << 1) | ((q ^ (q >> 1) ^ (q >> 2)) & 1)
+        passes = reads = 0
+        for g0 in range(0, len(ch), 64):
+            nl = min(64, len(ch) - g0)
+            for k in range(cnt[g0]):
+                r = ridx[first[g0] + 64 * k + np.arange(nl), :3]
+                for g in range(4):
+                    sel = r[grp[:nl] == g]
+                    for f in range(3):
+                        e = np.unique(sel[:, f])
+                        e = e[e != nz]
+                        if len(e):
+                            passes += np.bincount(e % 16).max()
+                            reads += 1
+        assert passes / reads < 1.45, passes / reads
 
 
 def test_tuning_table_applied_at_init(tmp_path):
