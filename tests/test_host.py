@@ -294,7 +294,7 @@ def test_gradient_records_equal_the_monomial_derivative(hname, tmp_path):
                         if len(e):
                             passes += np.bincount(e % 16).max()
                             reads += 1
-        assert passes / reads < 1.45, passes / reads
+        assert passes / reads < 1.35, passes / reads
 
 
 def test_tuning_table_applied_at_init(tmp_path):
