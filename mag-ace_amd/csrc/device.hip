@@ -112,6 +112,36 @@ __device__ __forceinline__ void horner(const T* __restrict__ c, T w, T rs, T& v,
     dv = fma(w, fma(w, (T)3 * c3, (T)2 * c2), c1) * rs;
 }
 
+// The four coefficients of one spline function as one or two 16-byte loads (rows and functions
+// are 16-byte aligned: 4 coefficients per function)
+__device__ __forceinline__ void load_coef(const float* __restrict__ p, float (&c)[4]) {
+    const float4 v = *(const float4*)p;
+    c[0] = v.x; c[1] = v.y; c[2] = v.z; c[3] = v.w;
+}
+__device__ __forceinline__ void load_coef(const double* __restrict__ p, double (&c)[4]) {
+    const double2 a = *(const double2*)p, b = *(const double2*)(p + 2);
+    c[0] = a.x; c[1] = a.y; c[2] = b.x; c[3] = b.y;
+}
+
+// Values of the spline functions koff + k, k0 <= k < k1, of one table row at fraction w
+// (horner's value, in the same operation order), with the coefficient loads of eight functions
+// (four in FP64) issued together so that one trip to L2 serves them all instead of one; dflt for every
+// function when the row is absent.  put(k, v) stores each.  `safe` is any readable coefficient
+// address, read in place of the row's where a function is past k1 or the row is absent.
+template <typename T, typename F>
+__device__ __forceinline__ void spline_values(const T* __restrict__ row, const T* __restrict__ safe, T w, int k0,
+                                              int k1, int koff, T dflt, F&& put) {
+    constexpr int B = sizeof(T) == 4 ? 8 : 4;  // FP64: four functions (32 registers) at a time
+    for (int k = k0; k < k1; k += B) {
+        T c[B][4];
+#pragma unroll
+        for (int u = 0; u < B; ++u) load_coef(row && k + u < k1 ? row + 4 * (koff + k + u) : safe, c[u]);
+#pragma unroll
+        for (int u = 0; u < B; ++u)
+            if (k + u < k1) put(k + u, row ? fma(w, fma(w, fma(w, c[u][3], c[u][2]), c[u][1]), c[u][0]) : dflt);
+    }
+}
+
 // Spline row and fraction for the atomic table (src/radial.F90:68-84).
 // Returns nullptr when the pair is beyond the table (all functions zero).
 template <typename T>
@@ -418,26 +448,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                         const int nf = P.nb_a + nfa, per = (nf + nfm + nsplit - 1) / nsplit;
                         const int k0 = (role - 1) * per, k1 = min(nf + nfm, k0 + per);
                         if (k0 < nf) {
-                            T w;
+                            T w = 0;
                             const T* row = atomic_row(P, dlist[p], w, err);
-                            for (int k = k0; k < min(k1, nf); ++k) {
-                                T v = 0, dv;
-                                if (row) horner(row + 4 * k, w, (T)P.rs_a, v, dv);
+                            spline_values<T>(row, P.lut_a, w, k0, min(k1, nf), 0, (T)0, [&](int k, T v) {
                                 if (k < P.nb_a)
                                     sG[j * P.nb_a + k] = v;
                                 else
                                     sR[j * nfa + (k - P.nb_a)] = v;
-                            }
+                            });
                         }
                         if (k1 > nf) {
-                            T w;
+                            T w = 0;
                             bool unit;
                             const T* row = magnetic_row(P, dmom[p], w, unit, err);
-                            for (int k = max(k0, nf); k < k1; ++k) {
-                                T v = unit ? (T)1 : (T)0, dv;
-                                if (row) horner(row + 4 * (P.nb_m + k - nf), w, (T)P.rs_m, v, dv);
-                                sM[j * nfm + (k - nf)] = v;
-                            }
+                            spline_values<T>(row, P.lut_m, w, max(k0, nf), k1, P.nb_m - nf, unit ? (T)1 : (T)0,
+                                             [&](int k, T v) { sM[j * nfm + (k - nf)] = v; });
                         }
                     }
                 } else if (role == nsplit + 1) {  // magnetic side of pair c0+j
@@ -469,18 +494,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                     const int j = tid;
                     if (j < nj) {
                         const int p = p0 + c0 + j;
-                        T w;
+                        T w = 0;
                         const T* row = atomic_row(P, dlist[p], w, err);
-                        for (int k = 0; k < P.nb_a; ++k) {
-                            T v = 0, dv;
-                            if (row) horner(row + 4 * k, w, (T)P.rs_a, v, dv);
-                            sG[j * P.nb_a + k] = v;
-                        }
-                        for (int k = 0; k < nfa; ++k) {
-                            T v = 0, dv;
-                            if (row) horner(row + 4 * (P.nb_a + k), w, (T)P.rs_a, v, dv);
-                            sR[j * nfa + k] = v;
-                        }
+                        spline_values<T>(row, P.lut_a, w, 0, P.nb_a + nfa, 0, (T)0, [&](int k, T v) {
+                            if (k < P.nb_a)
+                                sG[j * P.nb_a + k] = v;
+                            else
+                                sR[j * nfa + (k - P.nb_a)] = v;
+                        });
                         T cr = 0, dcr;
                         if (row) horner(row + 4 * (P.nb_a + nfa), w, (T)P.rs_a, cr, dcr);
                         sCr[j] = cr;
@@ -493,14 +514,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                     const int j = tid - CH;
                     if (j < nj) {
                         const int p = p0 + c0 + j;
-                        T w;
+                        T w = 0;
                         bool unit;
                         const T* row = magnetic_row(P, dmom[p], w, unit, err);
-                        for (int k = 0; k < nfm; ++k) {
-                            T v = unit ? (T)1 : (T)0, dv;
-                            if (row) horner(row + 4 * (P.nb_m + k), w, (T)P.rs_m, v, dv);
-                            sM[j * nfm + k] = v;
-                        }
+                        spline_values<T>(row, P.lut_m, w, 0, nfm, P.nb_m, unit ? (T)1 : (T)0,
+                                         [&](int k, T v) { sM[j * nfm + k] = v; });
                         cplx<T>* yh = sYm + j * P.lhm;
                         ylm_half<T, L>((T)emom[3 * p], (T)emom[3 * p + 1], (T)emom[3 * p + 2], P.l2m, yh);
                         // H_q = M_{l',n'} Y_{l'm'} for the magnetic columns of A (l' <= l3m)
