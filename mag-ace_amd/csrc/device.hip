@@ -1389,6 +1389,19 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                 U[q] = {0, 0};
             }
             {
+                // M_{l',n'}(mu_j) of the columns' (l', n'), entry n' (l3m + 1) + l' (at most nm3
+                // of them), with their coefficient loads issued together
+                T Mh[NM3X];
+                {
+                    const int nl = P.l3m + 1, nh = nl * P.nr3m;
+                    T c[NM3X][4];
+#pragma unroll
+                    for (int i = 0; i < NM3X; ++i)
+                        load_coef(rowm && i < nh ? rowm + 4 * (P.nb_m + (i % nl) * P.nr2m + i / nl) : P.lut_m, c[i]);
+#pragma unroll
+                    for (int i = 0; i < NM3X; ++i)
+                        Mh[i] = rowm ? fma(wm, fma(wm, fma(wm, c[i][3], c[i][2]), c[i][1]), c[i][0]) : unit ? (T)1 : (T)0;
+                }
                 cplx<T> ph = {1, 0};
                 LegRows<T, LM> Lr;
                 for (int lp = 0; lp <= P.l3m; ++lp) {
@@ -1399,8 +1412,11 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                         if (am > lp) break;
                         const cplx<T> y = {Lr.P[am] * ph.re, Lr.P[am] * ph.im};
                         for (int np = 0; np < P.nr3m; ++np) {
-                            T v = unit ? (T)1 : (T)0, dv = 0;
-                            if (rowm) horner(rowm + 4 * (P.nb_m + lp * P.nr2m + np), wm, rsm, v, dv);
+                            T v = 0;
+                            const int ih = np * (P.l3m + 1) + lp;
+#pragma unroll
+                            for (int i = 0; i < NM3X; ++i)
+                                if (i == ih) v = Mh[i];
                             const int q = np * P.lsm3 + lp * lp + lp + am;
 #pragma unroll
                             for (int qq = 0; qq < NM3X; ++qq)
@@ -1421,9 +1437,11 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                 // from the table overlap the arithmetic of the previous l
                 T cq[NRAX][4];
 #pragma unroll
-                for (int n = 0; n < NRAX; ++n)
+                for (int n = 0; n < NRAX; ++n) {
+                    load_coef(n < P.nr3a ? rowa + 4 * (P.nb_a + n) : P.lut_a, cq[n]);
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) cq[n][k] = n < P.nr3a ? rowa[4 * (P.nb_a + n) + k] : (T)0;
+                    for (int k = 0; k < 4; ++k) cq[n][k] = n < P.nr3a ? cq[n][k] : (T)0;
+                }
                 for (int l = 0; l <= P.l3a; ++l) {
                     Lr.step(l, ez);
                     T R[NRAX], dR[NRAX];
@@ -1435,9 +1453,11 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                     if (l < P.l3a) {
                         const T* nrow = rowa + 4 * (P.nb_a + (l + 1) * P.nr3a);
 #pragma unroll
-                        for (int n = 0; n < NRAX; ++n)
+                        for (int n = 0; n < NRAX; ++n) {
+                            load_coef(n < P.nr3a ? nrow + 4 * n : P.lut_a, cq[n]);
 #pragma unroll
-                            for (int k = 0; k < 4; ++k) cq[n][k] = n < P.nr3a ? nrow[4 * n + k] : (T)0;
+                            for (int k = 0; k < 4; ++k) cq[n][k] = n < P.nr3a ? cq[n][k] : (T)0;
+                        }
                     }
 #pragma unroll
                     for (int am = 0; am <= LA; ++am) {
@@ -1531,11 +1551,17 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
             {
                 const T invmu = (T)(1.0 / mu_d);
                 T ga[NBAX], dga[NBAX];
+                T cc[4];  // the core term's coefficients, loaded with the g_n (used after this loop)
+                {
+                    T c[NBAX][4];  // one 16-byte load per function, issued together
 #pragma unroll
-                for (int n = 0; n < NBAX; ++n) {
-                    ga[n] = 0;
-                    dga[n] = 0;
-                    if (n < P.nb_a) horner(rowa + 4 * n, wa, rsa, ga[n], dga[n]);
+                    for (int n = 0; n < NBAX; ++n) load_coef(n < P.nb_a ? rowa + 4 * n : P.lut_a, c[n]);
+                    load_coef(rowa + 4 * (P.nb_a + nfa), cc);
+#pragma unroll
+                    for (int n = 0; n < NBAX; ++n) {
+                        ga[n] = n < P.nb_a ? fma(wa, fma(wa, fma(wa, c[n][3], c[n][2]), c[n][1]), c[n][0]) : (T)0;
+                        dga[n] = n < P.nb_a ? fma(wa, fma(wa, (T)3 * c[n][3], (T)2 * c[n][2]), c[n][1]) * rsa : (T)0;
+                    }
                 }
                 T sf = 0;  // sum_n g'_n Re(W2 H2)
                 cplx<T> phm[LM + 1];
@@ -1543,14 +1569,24 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
 #pragma unroll
                 for (int m = 1; m <= LM; ++m) phm[m] = cmul(phm[m - 1], cplx<T>{sx, sy});
                 LegRows<T, LM> Lr;
+                // the coefficients of row l' are loaded one iteration ahead, as in the atomic loop
+                T cm[NR2X][4];
+#pragma unroll
+                for (int np = 0; np < NR2X; ++np)
+                    load_coef(rowm && np < P.nr2m ? rowm + 4 * (P.nb_m + np) : P.lut_m, cm[np]);
                 for (int lp = 0; lp <= P.l2m; ++lp) {
                     Lr.step(lp, sz);
                     T Mv[NR2X], dMv[NR2X];
 #pragma unroll
                     for (int np = 0; np < NR2X; ++np) {
-                        Mv[np] = unit ? (T)1 : (T)0;
-                        dMv[np] = 0;
-                        if (rowm && np < P.nr2m) horner(rowm + 4 * (P.nb_m + lp * P.nr2m + np), wm, rsm, Mv[np], dMv[np]);
+                        const bool f = rowm && np < P.nr2m;
+                        Mv[np] = f ? fma(wm, fma(wm, fma(wm, cm[np][3], cm[np][2]), cm[np][1]), cm[np][0]) : unit ? (T)1 : (T)0;
+                        dMv[np] = f ? fma(wm, fma(wm, (T)3 * cm[np][3], (T)2 * cm[np][2]), cm[np][1]) * rsm : (T)0;
+                    }
+                    if (lp < P.l2m) {
+#pragma unroll
+                        for (int np = 0; np < NR2X; ++np)
+                            load_coef(rowm && np < P.nr2m ? rowm + 4 * (P.nb_m + (lp + 1) * P.nr2m + np) : P.lut_m, cm[np]);
                     }
 #pragma unroll
                     for (int am = 0; am <= LM; ++am) {
@@ -1592,8 +1628,7 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                     }
                 }
                 // F from A2 (g'_n e) and the core term (src/forces.F90:464-494)
-                T cr, dcr;
-                horner(rowa + 4 * (P.nb_a + nfa), wa, rsa, cr, dcr);
+                const T dcr = fma(wa, fma(wa, (T)3 * cc[3], (T)2 * cc[2]), cc[1]) * rsa;
                 const T s = sf + (T)edf * dcr;
                 Fx = fma(s, ex, Fx);
                 Fy = fma(s, ey, Fy);
