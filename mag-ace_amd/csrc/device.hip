@@ -142,6 +142,25 @@ __device__ __forceinline__ void spline_values(const T* __restrict__ row, const T
     }
 }
 
+// The same with horner's derivative as well: put(k, v, dv), dv = 0 where the row is absent
+template <typename T, typename F>
+__device__ __forceinline__ void spline_values_dv(const T* __restrict__ row, const T* __restrict__ safe, T w, T rs,
+                                                 int k0, int k1, int koff, T dflt, F&& put) {
+    constexpr int B = sizeof(T) == 4 ? 8 : 4;
+    for (int k = k0; k < k1; k += B) {
+        T c[B][4];
+#pragma unroll
+        for (int u = 0; u < B; ++u) load_coef(row && k + u < k1 ? row + 4 * (koff + k + u) : safe, c[u]);
+#pragma unroll
+        for (int u = 0; u < B; ++u)
+            if (k + u < k1) {
+                const T v = row ? fma(w, fma(w, fma(w, c[u][3], c[u][2]), c[u][1]), c[u][0]) : dflt;
+                const T dv = row ? fma(w, fma(w, (T)3 * c[u][3], (T)2 * c[u][2]), c[u][1]) * rs : (T)0;
+                put(k + u, v, dv);
+            }
+    }
+}
+
 // Spline row and fraction for the atomic table (src/radial.F90:68-84).
 // Returns nullptr when the pair is beyond the table (all functions zero).
 template <typename T>
@@ -1828,30 +1847,25 @@ k_density_jvp(DevParams<double> P, const int* __restrict__ off,
                     } else {
                         const int nf = P.nb_a + nfa, per = (nf + nsplit - 1) / nsplit;
                         const int k0 = (role - 1) * per, k1 = min(nf, k0 + per);
-                        for (int k = k0; k < k1; ++k) {
-                            T v = 0, dv = 0;
-                            if (row) horner(row + 4 * k, w, (T)P.rs_a, v, dv);
+                        spline_values_dv<T>(row, P.lut_a, w, (T)P.rs_a, k0, k1, 0, (T)0, [&](int k, T v, T dv) {
                             if (k < P.nb_a) {
                                 sGd[j * P.nb_a + k] = dv * rdot;
                             } else {
                                 sR[j * nfa + (k - P.nb_a)] = v;
                                 sRd[j * nfa + (k - P.nb_a)] = dv * rdot;
                             }
-                        }
+                        });
                     }
                 } else if (role == 0) {
                     sCrd[j] = 0;
                 }
             } else if (role == nsplit + 1) {  // magnetic side (does not move with positions)
                 if (live) {
-                    T w;
+                    T w = 0;
                     bool unit;
                     const T* row = magnetic_row(P, dmom[p], w, unit, err);
-                    for (int k = 0; k < nfm; ++k) {
-                        T v = unit ? (T)1 : (T)0, dv;
-                        if (row) horner(row + 4 * (P.nb_m + k), w, (T)P.rs_m, v, dv);
-                        sM[j * nfm + k] = v;
-                    }
+                    spline_values<T>(row, P.lut_m, w, 0, nfm, P.nb_m, unit ? (T)1 : (T)0,
+                                     [&](int k, T v) { sM[j * nfm + k] = v; });
                     cplx<T>* yh = sYm + j * P.lhm;
                     ylm_half<T, L>(emom[3 * p], emom[3 * p + 1], emom[3 * p + 2], P.l2m, yh);
                     for (int q = 0; q < P.nm3; ++q) {
