@@ -269,9 +269,9 @@ __device__ __forceinline__ cplx<T> yfull(const cplx<T>* half, int l, int m) {
 // ============================================================================ k_density
 // Shapes: NB n-block of A columns, NM3X max magnetic columns of A, NB2 n-block of A2,
 // NR2X max magnetic radial index of A2.
-// Shape class 0: at least 5 waves per SIMD in FP32 (the register budget drops to 102; measured
-// 0.91 -> 0.77 ms on the lmax-6 workload, 0.52 -> 0.46 on lmax 4) and 3 in FP64 (168 registers;
-// at 169 the occupancy halves and the kernel takes 1.85 instead of 1.39 ms).  Class 1 (27
+// Shape class 0: 4 waves per SIMD in FP32 (128 registers; with the eight-function spline
+// batches 127 are used, and 5 waves at 96 registers spilled and ran no faster) and 3 in FP64
+// (168 registers; at 169 the occupancy halves and the kernel takes 1.85 instead of 1.39 ms).  Class 1 (27
 // magnetic columns): 3 waves per SIMD in FP32 (168 registers, a 12-byte spill; on the shape
 // of the example potential 0.75 -> 0.57 ms) and 2 in FP64 (256 registers with a 192-byte
 // spill instead of 302 registers at 1 wave: 2.06 -> 1.33 ms).
@@ -335,15 +335,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
     const int atom = blockIdx.x;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int nfa = P.nr3a * (P.l3a + 1), nfm = P.nr2m * (P.l2m + 1);
-    // LDS carve-up (sizes mirror device_density_lds on the host)
-    cplx<T>* sY = (cplx<T>*)smem;                        // [CH][lha]
-    cplx<T>* sYm = sY + CH * P.lha;                       // [CH][lhm]
-    cplx<T>* sH = sYm + CH * P.lhm;                       // [CH][nm3]
-    cplx<T>* sYc = sH + CH * P.nm3;                       // [lhm] central moment
-    T* sR = (T*)(sYc + P.lhm);                            // [CH][nfa]  R_{l,n}
-    T* sG = sR + CH * nfa;                                // [CH][nb_a]
-    T* sM = sG + CH * P.nb_a;                             // [CH][nfm]  M_{l',n'}
-    T* sCr = sM + CH * nfm;                               // [CH]
+    // LDS carve-up (sizes mirror density_lds_bytes on the host).  The per-pair rows have odd
+    // lengths (one entry of padding where the count is even), so that the staging threads,
+    // which write the rows of consecutive pairs at once, fall in distinct banks
+    const int sYs = P.lha | 1, sYms = P.lhm | 1, sHs = P.nm3 | 1, sRs = nfa | 1, sGs = P.nb_a | 1, sMs = nfm | 1;
+    cplx<T>* sY = (cplx<T>*)smem;                        // [CH][sYs]
+    cplx<T>* sYm = sY + CH * sYs;                         // [CH][sYms]
+    cplx<T>* sH = sYm + CH * sYms;                        // [CH][sHs]
+    cplx<T>* sYc = sH + CH * sHs;                         // [lhm] central moment
+    T* sR = (T*)(sYc + P.lhm);                            // [CH][sRs]  R_{l,n}
+    T* sG = sR + CH * sRs;                                // [CH][sGs]
+    T* sM = sG + CH * sGs;                                // [CH][sMs]  M_{l',n'}
+    T* sCr = sM + CH * sMs;                               // [CH]
     T* sMc = sCr + CH;                                    // [nfm + nb_m] central M and g^m
     double* sEhc = (double*)(((uintptr_t)(sMc + nfm + P.nb_m) + 15) & ~(uintptr_t)15);
 
@@ -456,7 +459,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                         if (row) horner(row + 4 * (P.nb_a + nfa), w, (T)P.rs_a, cr, dcr);
                         sCr[j] = cr;
                         ylm_half<T, L>((T)elist[3 * p], (T)elist[3 * p + 1], (T)elist[3 * p + 2], P.l3a,
-                                       sY + j * P.lha);
+                                       sY + j * sYs);
                     } else {
                         sCr[j] = 0;
                     }
@@ -471,9 +474,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                             const T* row = atomic_row(P, dlist[p], w, err);
                             spline_values<T>(row, P.lut_a, w, k0, min(k1, nf), 0, (T)0, [&](int k, T v) {
                                 if (k < P.nb_a)
-                                    sG[j * P.nb_a + k] = v;
+                                    sG[j * sGs + k] = v;
                                 else
-                                    sR[j * nfa + (k - P.nb_a)] = v;
+                                    sR[j * sRs + (k - P.nb_a)] = v;
                             });
                         }
                         if (k1 > nf) {
@@ -481,7 +484,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                             bool unit;
                             const T* row = magnetic_row(P, dmom[p], w, unit, err);
                             spline_values<T>(row, P.lut_m, w, max(k0, nf), k1, P.nb_m - nf, unit ? (T)1 : (T)0,
-                                             [&](int k, T v) { sM[j * nfm + (k - nf)] = v; });
+                                             [&](int k, T v) { sM[j * sMs + (k - nf)] = v; });
                         }
                     }
                 } else if (role == nsplit + 1) {  // magnetic side of pair c0+j
@@ -489,7 +492,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                         T w;
                         bool unit;
                         const T* row = magnetic_row(P, dmom[p], w, unit, err);
-                        cplx<T>* yh = sYm + j * P.lhm;
+                        cplx<T>* yh = sYm + j * sYms;
                         ylm_half<T, L>((T)emom[3 * p], (T)emom[3 * p + 1], (T)emom[3 * p + 2], P.l2m, yh);
                         // H_q = M_{l',n'} Y_{l'm'} for the magnetic columns of A (l' <= l3m,
                         // q = n' lsm3 + l'^2 + l' + m'); the few M it needs are evaluated here
@@ -501,7 +504,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                                 for (int mp = -lp; mp <= lp; ++mp) {
                                     const int q = np * P.lsm3 + lp * lp + lp + mp;
                                     const cplx<T> y = yfull(yh, lp, mp);
-                                    sH[j * P.nm3 + q] = {mv * y.re, mv * y.im};
+                                    sH[j * sHs + q] = {mv * y.re, mv * y.im};
                                 }
                             }
                     }
@@ -517,15 +520,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                         const T* row = atomic_row(P, dlist[p], w, err);
                         spline_values<T>(row, P.lut_a, w, 0, P.nb_a + nfa, 0, (T)0, [&](int k, T v) {
                             if (k < P.nb_a)
-                                sG[j * P.nb_a + k] = v;
+                                sG[j * sGs + k] = v;
                             else
-                                sR[j * nfa + (k - P.nb_a)] = v;
+                                sR[j * sRs + (k - P.nb_a)] = v;
                         });
                         T cr = 0, dcr;
                         if (row) horner(row + 4 * (P.nb_a + nfa), w, (T)P.rs_a, cr, dcr);
                         sCr[j] = cr;
                         ylm_half<T, L>((T)elist[3 * p], (T)elist[3 * p + 1], (T)elist[3 * p + 2], P.l3a,
-                                       sY + j * P.lha);
+                                       sY + j * sYs);
                     } else {
                         sCr[j] = 0;
                     }
@@ -537,8 +540,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                         bool unit;
                         const T* row = magnetic_row(P, dmom[p], w, unit, err);
                         spline_values<T>(row, P.lut_m, w, 0, nfm, P.nb_m, unit ? (T)1 : (T)0,
-                                         [&](int k, T v) { sM[j * nfm + k] = v; });
-                        cplx<T>* yh = sYm + j * P.lhm;
+                                         [&](int k, T v) { sM[j * sMs + k] = v; });
+                        cplx<T>* yh = sYm + j * sYms;
                         ylm_half<T, L>((T)emom[3 * p], (T)emom[3 * p + 1], (T)emom[3 * p + 2], P.l2m, yh);
                         // H_q = M_{l',n'} Y_{l'm'} for the magnetic columns of A (l' <= l3m)
                         for (int q = 0; q < P.nm3; ++q) {
@@ -547,8 +550,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                             while ((lp + 1) * (lp + 1) <= lmp) ++lp;
                             const int mp = lmp - lp * lp - lp;
                             const cplx<T> y = yfull(yh, lp, mp);
-                            const T mv = sM[j * nfm + lp * P.nr2m + np];
-                            sH[j * P.nm3 + q] = {mv * y.re, mv * y.im};
+                            const T mv = sM[j * sMs + lp * P.nr2m + np];
+                            sH[j * sHs + q] = {mv * y.re, mv * y.im};
                         }
                     }
                 }
@@ -564,11 +567,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                 // and radial indices past nr3a are read without a test: those reads stay inside
                 // the staging area, and their sums land in slots that are never stored.
                 for (int j = part; j < nj; j += sa) {
-                    const cplx<T> y = sY[j * P.lha + lh];
+                    const cplx<T> y = sY[j * sYs + lh];
                     cplx<T> h[NM3X];
 #pragma unroll
-                    for (int q = 0; q < NM3X; ++q) h[q] = sH[j * P.nm3 + q];
-                    const T* rr = sR + j * nfa + l * P.nr3a + n0;
+                    for (int q = 0; q < NM3X; ++q) h[q] = sH[j * sHs + q];
+                    const T* rr = sR + j * sRs + l * P.nr3a + n0;
                     // Y H_q once per pair and column, then two real FMAs per radial index
                     cplx<T> yh[NM3X];
 #pragma unroll
@@ -592,12 +595,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                 // A2[n][n'][l'm'] += g_n M_{l',n'} Y_{l'm'}   (src/a_basis.F90:193-201); unused
                 // slots as above
                 for (int j = 0; j < nj; ++j) {
-                    const cplx<T> y = sYm[j * P.lhm + lh];
-                    const T* mr = sM + j * nfm + l * P.nr2m;
+                    const cplx<T> y = sYm[j * sYms + lh];
+                    const T* mr = sM + j * sMs + l * P.nr2m;
                     cplx<T> my[NR2X];
 #pragma unroll
                     for (int q = 0; q < NR2X; ++q) my[q] = {mr[q] * y.re, mr[q] * y.im};
-                    const T* gr = sG + j * P.nb_a + n0;
+                    const T* gr = sG + j * sGs + n0;
 #pragma unroll
                     for (int b = 0; b < NB2; ++b) {
                         const T g = gr[b];
@@ -657,8 +660,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
 size_t density_lds_bytes(const DevParams<double>& Pd, int CH, size_t tsize) {
     const int nfa = Pd.nr3a * (Pd.l3a + 1), nfm = Pd.nr2m * (Pd.l2m + 1);
     size_t cb = 2 * tsize;
-    size_t n = cb * ((size_t)CH * Pd.lha + (size_t)CH * Pd.lhm + (size_t)CH * Pd.nm3 + Pd.lhm) +
-               tsize * ((size_t)CH * nfa + (size_t)CH * Pd.nb_a + (size_t)CH * nfm + CH + nfm + Pd.nb_m);
+    // per-pair rows padded to odd lengths (k_density)
+    size_t n = cb * ((size_t)CH * (Pd.lha | 1) + (size_t)CH * (Pd.lhm | 1) + (size_t)CH * (Pd.nm3 | 1) + Pd.lhm) +
+               tsize * ((size_t)CH * (nfa | 1) + (size_t)CH * (Pd.nb_a | 1) + (size_t)CH * (nfm | 1) + CH + nfm + Pd.nb_m);
     return ((n + 15) & ~(size_t)15) + 16;
 }
 
