@@ -833,10 +833,25 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int
     double* s_scale = red + 2 * (kAdjointThreads / 64);
     const int atom = blockIdx.x, tid = threadIdx.x;
     Zin += (size_t)atom * P.nz;
-    for (int x = tid; x < P.nzx; x += blockDim.x) {
-        if (x < P.nz) sZ[x] = Zin[x];
-        sW[x] = {0, 0};
-        if (ND2) sW2[x] = {0, 0};
+    {
+        // Z' from Z, with all of a thread's loads issued before the stores
+        constexpr int U = 8;  // covers nzx <= 8 * kAdjointThreads = 4096 in one pass
+        static_assert(U * kAdjointThreads >= 4096, "one pass over the largest feature vector");
+        cplx<T> v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int x = tid + u * kAdjointThreads;
+            v[u] = x < P.nz ? Zin[x] : cplx<T>{0, 0};
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int x = tid + u * kAdjointThreads;
+            if (x < P.nz) sZ[x] = v[u];
+            if (x < P.nzx) {
+                sW[x] = {0, 0};
+                if (ND2) sW2[x] = {0, 0};
+            }
+        }
     }
     if (tid == 0) sZ[P.nz] = {1, 0};
     __syncthreads();
@@ -1076,12 +1091,22 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<floa
     const bool two = a0 + 1 < natom;
     const cplx<T>* Z0 = Zin + (size_t)a0 * P.nz;
     const cplx<T>* Z1 = two ? Z0 + P.nz : Z0;
-    for (int x = tid; x < P.nzx; x += blockDim.x) {
-        if (x < P.nz) {
-            const cplx<T> u = Z0[x], v = Z1[x];
-            sZ[x] = make_float4(u.re, u.im, v.re, v.im);
+    {
+        // Z' of both atoms, with all of a thread's loads issued before the stores
+        constexpr int U = 8;  // covers nzx <= 8 * kAdjointThreads = 4096 in one pass
+        cplx<T> a[U], b[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int x = tid + u * kAdjointThreads;
+            a[u] = x < P.nz ? Z0[x] : cplx<T>{0, 0};
+            b[u] = x < P.nz ? Z1[x] : cplx<T>{0, 0};
         }
-        sW[x] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int x = tid + u * kAdjointThreads;
+            if (x < P.nz) sZ[x] = make_float4(a[u].re, a[u].im, b[u].re, b[u].im);
+            if (x < P.nzx) sW[x] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
     }
     if (tid == 0) sZ[P.nz] = make_float4(1.f, 0.f, 1.f, 0.f);
     __syncthreads();
@@ -1363,8 +1388,23 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
     extern __shared__ __align__(16) unsigned char smem_f[];
     cplx<T>* sW = reinterpret_cast<cplx<T>*>(smem_f);
     {
+        // eight loads in flight per lane: one trip to memory serves 512 entries, where a load
+        // followed by its store had waited for every 64
         const cplx<T>* __restrict__ Wg = Win + (size_t)atom * P.nz;
-        for (int i = lane; i < P.nz; i += 64) sW[i] = Wg[i];
+        constexpr int U = 8;
+        for (int i0 = 0; i0 < P.nz; i0 += 64 * U) {
+            cplx<T> v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + 64 * u + lane;
+                v[u] = i < P.nz ? Wg[i] : cplx<T>{0, 0};
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + 64 * u + lane;
+                if (i < P.nz) sW[i] = v[u];
+            }
+        }
         __syncthreads();
     }
     const cplx<T>* W = sW;
