@@ -1434,12 +1434,15 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
         const int p = p0 + (active ? j : 0);
         T Fx = 0, Fy = 0, Fz = 0, Tx = 0, Ty = 0, Tz = 0;
         T wa = 0;
-        const T* rowa = active ? atomic_row(P, dlist[p], wa, err) : nullptr;
+        // the pair's inputs, all loads issued before the first is used
+        const double r_d = dlist[p], mu_d = dmom[p];
+        const double ex_d = elist[3 * p], ey_d = elist[3 * p + 1], ez_d = elist[3 * p + 2];
+        const double sx_d = emom[3 * p], sy_d = emom[3 * p + 1], sz_d = emom[3 * p + 2];
+        const T* rowa = active ? atomic_row(P, r_d, wa, err) : nullptr;
         if (rowa) {
-            const T invr = (T)(1.0 / dlist[p]);
-            const T ex = (T)elist[3 * p], ey = (T)elist[3 * p + 1], ez = (T)elist[3 * p + 2];
-            const double mu_d = dmom[p];
-            const T sx = (T)emom[3 * p], sy = (T)emom[3 * p + 1], sz = (T)emom[3 * p + 2];
+            const T invr = (T)(1.0 / r_d);
+            const T ex = (T)ex_d, ey = (T)ey_d, ez = (T)ez_d;
+            const T sx = (T)sx_d, sy = (T)sy_d, sz = (T)sz_d;
             T wm = 0;
             bool unit = false;
             const T* rowm = magnetic_row(P, mu_d, wm, unit, err);
