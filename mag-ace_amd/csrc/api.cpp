@@ -571,16 +571,17 @@ int run_device(Context& c, int natom, const int* off, const double* dlist, const
     return 0;
 }
 
-// pairs per acelib_ chunk: MAGACE_ACELIB_CHUNK_PAIRS if set (1024 .. 2^30), else 262144; read
+// pairs per acelib_ chunk: MAGACE_ACELIB_CHUNK_PAIRS if set (1024 .. 2^30), else 393216; read
 // at every call.  Each of the two slots holds, in pinned host memory and again on the
-// device, 112 bytes per pair and 68 per atom of one chunk (29 MB at the default).  On the
-// 16000-atom lmax-6 workload (917170 pairs) 262144 was the fastest of 131072 .. 1048576.
+// device, 112 bytes per pair and 68 per atom of one chunk (44 MB at the default).  On the
+// 16000-atom lmax-6 workload (917170 pairs) 393216 was the fastest of 131072 .. 524288
+// (2.80 ms per call against 2.91 at 262144 and 2.99 at 524288).
 int chunk_pairs() {
     if (const char* e = std::getenv("MAGACE_ACELIB_CHUNK_PAIRS")) {
         const long v = std::atol(e);
         if (v >= 1024 && v <= (1L << 30)) return (int)v;
     }
-    return 262144;
+    return 393216;
 }
 
 // Byte offsets of one acelib_ chunk of na atoms and np pairs.  Inputs: dlist, elist,

@@ -1,9 +1,9 @@
 """acelib_ on host arrays: the atoms go through the kernels in chunks of whole atoms, staged
 through pinned buffers so that copies overlap the kernels (MAGACE_ACELIB_CHUNK_PAIRS sets the
-chunk size and is read at every call).  Chunking changes nothing but summation order inside
-the adjoint's LDS atomics: every chunk size must agree with the device-pointer entry point on
-the whole list to 1e-12 (FP64), including lists with atoms that have no neighbours and a
-list with no pairs at all.  The device entry point's own partitions are checked the same way:
+chunk size and is read at every call).  Chunks cut at even atoms and the adjoint's walk
+depends only on an atom's index in the whole system, so chunking changes no arithmetic: every
+chunk size must agree with the device-pointer entry point on the whole list to 1e-12 (FP64),
+including lists with atoms that have no neighbours and a list with no pairs at all.  The device entry point's own partitions are checked the same way:
 evaluations of 2048+ atoms run as two halves on two streams, and the FP32 adjoint pairs atoms
 two per workgroup (odd counts included).
 """
