@@ -445,30 +445,52 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
             __syncthreads();
             if constexpr (SPLIT) {
                 // staging of pair c0 + j, spread over the block: thread tid works on pair
-                // j = tid % CH in role tid / CH.  Role 0: bond Y_lm and the core term; roles
-                // 1..nsplit: a share of the radial functions g_n and R_{l,n}; role nsplit + 1: the
-                // magnetic side (M, Y_l'm' of the neighbour moment, their products H_q).
+                // j = tid % CH in role tid / CH.  Roles 0 and 1, which share a wave: the bond's
+                // Y_lm and the core term, and the magnetic side (Y_l'm' of the neighbour moment,
+                // the M and the products H_q the A columns need), through one Y_lm call for both;
+                // roles 2..nsplit + 1: a share of the radial functions g_n, R_{l,n} and, after
+                // them, of the magnetic functions M_{l',n'}
                 const int j = tid % CH, role = tid / CH, nsplit = nthr / CH - 2;
                 const bool live = j < nj;
                 const int p = p0 + c0 + (live ? j : 0);
-                if (role == 0) {
+                if (role <= 1) {
+                    const bool bond = role == 0;
                     if (live) {
-                        T w;
-                        const T* row = atomic_row(P, dlist[p], w, err);
-                        T cr = 0, dcr;
-                        if (row) horner(row + 4 * (P.nb_a + nfa), w, (T)P.rs_a, cr, dcr);
-                        sCr[j] = cr;
-                        ylm_half<T, L>((T)elist[3 * p], (T)elist[3 * p + 1], (T)elist[3 * p + 2], P.l3a,
-                                       sY + j * sYs);
-                    } else {
+                        T w = 0;
+                        bool unit = false;
+                        const T* row = bond ? atomic_row(P, dlist[p], w, err) : magnetic_row(P, dmom[p], w, unit, err);
+                        if (bond) {
+                            T cr = 0, dcr;
+                            if (row) horner(row + 4 * (P.nb_a + nfa), w, (T)P.rs_a, cr, dcr);
+                            sCr[j] = cr;
+                        }
+                        const double* v3 = bond ? elist : emom;
+                        cplx<T>* yo = bond ? sY + j * sYs : sYm + j * sYms;
+                        ylm_half<T, L>((T)v3[3 * p], (T)v3[3 * p + 1], (T)v3[3 * p + 2], bond ? P.l3a : P.l2m, yo);
+                        if (!bond) {
+                            // H_q = M_{l',n'} Y_{l'm'} for the magnetic columns of A (l' <= l3m,
+                            // q = n' lsm3 + l'^2 + l' + m'); the few M it needs are evaluated
+                            // here rather than waiting for the spline roles
+                            for (int np = 0; np * P.lsm3 < P.nm3; ++np)
+                                for (int lp = 0; lp <= P.l3m; ++lp) {
+                                    T mv = unit ? (T)1 : (T)0, dv;
+                                    if (row) horner(row + 4 * (P.nb_m + lp * P.nr2m + np), w, (T)P.rs_m, mv, dv);
+                                    for (int mp = -lp; mp <= lp; ++mp) {
+                                        const int q = np * P.lsm3 + lp * lp + lp + mp;
+                                        const cplx<T> y = yfull(yo, lp, mp);
+                                        sH[j * sHs + q] = {mv * y.re, mv * y.im};
+                                    }
+                                }
+                        }
+                    } else if (bond) {
                         sCr[j] = 0;
                     }
-                } else if (role <= nsplit) {
+                } else {
                     // a share of the pair's radial functions g_n, R_{l,n} and, after them, of
                     // its magnetic functions M_{l',n'}
                     if (live) {
                         const int nf = P.nb_a + nfa, per = (nf + nfm + nsplit - 1) / nsplit;
-                        const int k0 = (role - 1) * per, k1 = min(nf + nfm, k0 + per);
+                        const int k0 = (role - 2) * per, k1 = min(nf + nfm, k0 + per);
                         if (k0 < nf) {
                             T w = 0;
                             const T* row = atomic_row(P, dlist[p], w, err);
@@ -486,27 +508,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                             spline_values<T>(row, P.lut_m, w, max(k0, nf), k1, P.nb_m - nf, unit ? (T)1 : (T)0,
                                              [&](int k, T v) { sM[j * sMs + (k - nf)] = v; });
                         }
-                    }
-                } else if (role == nsplit + 1) {  // magnetic side of pair c0+j
-                    if (live) {
-                        T w;
-                        bool unit;
-                        const T* row = magnetic_row(P, dmom[p], w, unit, err);
-                        cplx<T>* yh = sYm + j * sYms;
-                        ylm_half<T, L>((T)emom[3 * p], (T)emom[3 * p + 1], (T)emom[3 * p + 2], P.l2m, yh);
-                        // H_q = M_{l',n'} Y_{l'm'} for the magnetic columns of A (l' <= l3m,
-                        // q = n' lsm3 + l'^2 + l' + m'); the few M it needs are evaluated here
-                        // rather than waiting for the spline roles
-                        for (int np = 0; np * P.lsm3 < P.nm3; ++np)
-                            for (int lp = 0; lp <= P.l3m; ++lp) {
-                                T mv = unit ? (T)1 : (T)0, dv;
-                                if (row) horner(row + 4 * (P.nb_m + lp * P.nr2m + np), w, (T)P.rs_m, mv, dv);
-                                for (int mp = -lp; mp <= lp; ++mp) {
-                                    const int q = np * P.lsm3 + lp * lp + lp + mp;
-                                    const cplx<T> y = yfull(yh, lp, mp);
-                                    sH[j * sHs + q] = {mv * y.re, mv * y.im};
-                                }
-                            }
                     }
                 }
             } else {
