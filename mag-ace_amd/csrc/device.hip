@@ -731,19 +731,19 @@ template <typename T> __device__ __forceinline__ cplx<T> lds_cplx(uint32_t off) 
 // B parts sum to tots[d]; writes eout, dE_i/drho_d = fc F'(rho_d) (also into scale[d]), the
 // e_hc factor and rho
 template <typename T>
-__device__ inline void atom_energy(const DevParams<T>& P, int nd, int atom, const double* tots,
-                                   const double* __restrict__ ehc, const double* __restrict__ rho1,
-                                   double* __restrict__ eout, double* __restrict__ dEout,
+__device__ inline void atom_energy(const DevParams<T>& P, int nd, int atom, const double* tots, double e,
+                                   const double* r1, double* __restrict__ eout, double* __restrict__ dEout,
                                    double* __restrict__ edfout, double* __restrict__ rho_out, double* scale) {
+    // e = e_hc of the atom, r1[d] its B1 parts of rho (loaded by the caller ahead of time)
     double Fs = 0.0, dF[2] = {0.0, 0.0};
     for (int d = 0; d < nd; ++d) {
-        const double r = tots[d] + rho1[(size_t)atom * nd + d];
+        const double r = tots[d] + r1[d];
         rho_out[(size_t)atom * nd + d] = r;
         double F;
         embed(P.nfemb, r, P.p1[d][0], P.p1[d][1], F, dF[d]);
         Fs += F;
     }
-    const double e = ehc[atom], elow = P.ecut - P.decut;
+    const double elow = P.ecut - P.decut;
     double fc = 1.0, dfc = 0.0;
     if (e >= P.ecut) { fc = 0.0; dfc = 0.0; }
     else if (e > elow) {
@@ -772,11 +772,13 @@ __device__ __forceinline__ cplx<T> vadd(cplx<T> a, cplx<T> b) {
 // target, its loads issued together, into sW[x] when the group is the target's only one and
 // into the group's first slot otherwise.  Second pass: a thread sums the group results of one
 // target with several groups.  Two round trips to L2 instead of one per slot.
+// cb, cb2: this thread's first descriptors (comb[tid], comb2[tid]), loaded ahead by the caller.
 template <typename P_t, typename V>
-__device__ __forceinline__ void combine_partials(const P_t& P, V* __restrict__ part, V* __restrict__ sW) {
+__device__ __forceinline__ void combine_partials(const P_t& P, V* __restrict__ part, V* __restrict__ sW, int2 cb,
+                                                 int2 cb2) {
     const int nthr = (int)blockDim.x, tid = (int)threadIdx.x;
     for (int k = tid; k < P.ncomb; k += nthr) {
-        const int2 e = P.comb[k];
+        const int2 e = k == tid ? cb : P.comb[k];
         const int x = e.y & 0xfff, n = (e.y >> 12) & 15;
         V v[kCombGroup];
 #pragma unroll
@@ -793,7 +795,7 @@ __device__ __forceinline__ void combine_partials(const P_t& P, V* __restrict__ p
     }
     __syncthreads();
     for (int k = tid; k < P.ncomb2; k += nthr) {
-        const int2 e = P.comb2[k];
+        const int2 e = k == tid ? cb2 : P.comb2[k];
         const int x = e.y & 0xfff, ng = e.y >> 12;
         V sum = part[e.x];
         for (int g0 = 1; g0 < ng; g0 += kCombGroup) {
@@ -853,6 +855,24 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int
                 if (ND2) sW2[x] = {0, 0};
             }
         }
+    }
+    // what the tail needs from global memory, loaded now so that its latency passes during the
+    // record walk: the fold words of this thread's entries, its first combine descriptors and
+    // (thread 0) the atom's e_hc and B1 part of rho
+    constexpr int kFold = 8;  // nz <= 8 * kAdjointThreads
+    uint32_t fw[kFold];
+#pragma unroll
+    for (int u = 0; u < kFold; ++u) {
+        const int x = tid + u * kAdjointThreads;
+        fw[u] = x < P.nz ? P.fold[x] : 0u;
+    }
+    const int2 cb = tid < P.ncomb ? P.comb[tid] : make_int2(0, 0);
+    const int2 cb2 = tid < P.ncomb2 ? P.comb2[tid] : make_int2(0, 0);
+    double e_hc = 0.0, r1v[2] = {0.0, 0.0};
+    if (tid == 0) {
+        e_hc = ehc[atom];
+        r1v[0] = rho1[(size_t)atom * P.nd];
+        if (ND2) r1v[1] = rho1[(size_t)atom * P.nd + 1];
     }
     if (tid == 0) sZ[P.nz] = {1, 0};
     __syncthreads();
@@ -993,10 +1013,10 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int
         j = jn;
     }
     __syncthreads();  // every run stored: sole targets in sW, the others in their slots
-    combine_partials(P, part, sW);
+    combine_partials(P, part, sW, cb, cb2);
     if (ND2) {
         __syncthreads();
-        combine_partials(P, part2, sW2);
+        combine_partials(P, part2, sW2, cb, cb2);
     }
     __syncthreads();
     // products Y: dRho/dZ[x] += dRho/dY Z[other] for the Y entries that hold Z[x], one thread
@@ -1031,7 +1051,7 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int
     }
     if (tid == 0) {
         const double tots[2] = {tot, tot2};
-        atom_energy<T>(P, ND2 ? 2 : 1, atom, tots, ehc, rho1, eout, dEout, edfout, rho_out, s_scale);
+        atom_energy<T>(P, ND2 ? 2 : 1, atom, tots, e_hc, r1v, eout, dEout, edfout, rho_out, s_scale);
     }
     __syncthreads();
     // W_eff on representatives of conjugate pairs: since Z_xbar = s conj(Z_x),
@@ -1042,8 +1062,11 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint(DevParams<T> P, int
     const T sc = (T)s_scale[0], sc2 = ND2 ? (T)s_scale[1] : (T)0;
     Wout += (size_t)atom * P.nz;
     cplx<T>* wd = ND2 && Wd ? Wd + (size_t)atom * 2 * P.nz : nullptr;
-    for (int x = tid; x < P.nz; x += blockDim.x) {
-        const uint32_t f = P.fold[x];
+#pragma unroll
+    for (int u = 0; u < kFold; ++u) {
+        const int x = tid + u * kAdjointThreads;
+        if (x >= P.nz) break;
+        const uint32_t f = fw[u];
         cplx<T> w = {0, 0}, w2 = {0, 0};
         if (f & (1u << 17)) {
             w = sW[x];
@@ -1108,6 +1131,23 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<floa
             if (x < P.nz) sZ[x] = make_float4(a[u].re, a[u].im, b[u].re, b[u].im);
             if (x < P.nzx) sW[x] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
+    }
+    // the tail's inputs from global memory, loaded now (as in k_adjoint); the two atoms'
+    // energies are evaluated by threads 0 and 64 (two waves) at the same time
+    constexpr int kFold = 8;
+    uint32_t fw[kFold];
+#pragma unroll
+    for (int u = 0; u < kFold; ++u) {
+        const int x = tid + u * kAdjointThreads;
+        fw[u] = x < P.nz ? P.fold[x] : 0u;
+    }
+    const int2 cb = tid < P.ncomb ? P.comb[tid] : make_int2(0, 0);
+    const int2 cb2 = tid < P.ncomb2 ? P.comb2[tid] : make_int2(0, 0);
+    const int eatom = tid == 0 ? a0 : tid == 64 && two ? a0 + 1 : -1;  // the atom whose energy this thread evaluates
+    double e_hc = 0.0, r1v = 0.0;
+    if (eatom >= 0) {
+        e_hc = ehc[eatom];
+        r1v = rho1[eatom];
     }
     if (tid == 0) sZ[P.nz] = make_float4(1.f, 0.f, 1.f, 0.f);
     __syncthreads();
@@ -1222,7 +1262,7 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<floa
         j = jn;
     }
     __syncthreads();
-    combine_partials(P, part, sW);
+    combine_partials(P, part, sW, cb, cb2);
     __syncthreads();
     if (P.nyt > 0) {  // products Y, as in k_adjoint
         for (int k = tid; k < P.nyt; k += nthr) {
@@ -1249,25 +1289,27 @@ __global__ void __launch_bounds__(kAdjointThreads) k_adjoint_pair(DevParams<floa
         }
     const double tot0 = block_sum<T>(rho0, red);
     __syncthreads();
-    const double tot1 = block_sum<T>(rho1s, red + kAdjointThreads / 64);
-    if (tid == 0) {
+    double tot1 = block_sum<T>(rho1s, red + kAdjointThreads / 64);  // valid on thread 0
+    if (tid == 64) {  // the second atom's sum, from the same partials in the same order
+        tot1 = 0.0;
+        for (int i = 0; i < kAdjointThreads / 64; ++i) tot1 += red[kAdjointThreads / 64 + i];
+    }
+    if (eatom >= 0) {
         double sc[1];
-        const double t0[1] = {tot0};
-        atom_energy<T>(P, 1, a0, t0, ehc, rho1, eout, dEout, edfout, rho_out, sc);
-        s_scale[0] = sc[0];
-        if (two) {
-            const double t1[1] = {tot1};
-            atom_energy<T>(P, 1, a0 + 1, t1, ehc, rho1, eout, dEout, edfout, rho_out, sc);
-            s_scale[1] = sc[0];
-        }
+        const double t[1] = {eatom == a0 ? tot0 : tot1};
+        atom_energy<T>(P, 1, eatom, t, e_hc, &r1v, eout, dEout, edfout, rho_out, sc);
+        s_scale[eatom - a0] = sc[0];
     }
     __syncthreads();
     // W_eff on representatives of conjugate pairs, as in k_adjoint
     const T sc0 = (T)s_scale[0], sc1 = two ? (T)s_scale[1] : (T)0;
     cplx<T>* W0 = Wout + (size_t)a0 * P.nz;
     cplx<T>* W1 = W0 + P.nz;
-    for (int x = tid; x < P.nz; x += blockDim.x) {
-        const uint32_t f = P.fold[x];
+#pragma unroll
+    for (int u = 0; u < kFold; ++u) {
+        const int x = tid + u * kAdjointThreads;
+        if (x >= P.nz) break;
+        const uint32_t f = fw[u];
         float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
         if (f & (1u << 17)) {
             w = sW[x];
