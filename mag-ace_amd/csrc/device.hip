@@ -1583,11 +1583,15 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                             // sum_n Re[V_n (R'_n Y e + R_n/r grad Y)], the torque
                             // sum_n W[n][lm][q] R_n Y: both are summed over n first
                             // (S1 = sum_n R'_n V_n, S2 = sum_n R_n V_n, Tq = sum_n R_n W[n][lm][q])
-                            // and meet Y and grad Y once per (l, m)
+                            // and meet Y and grad Y once per (l, m).  With the columns in
+                            // registers (NM3X <= 8) the sums over n run per column instead,
+                            // Tq and Dq = sum_n R'_n W[n][lm][q], and S1 = sum_q Dq H_q,
+                            // S2 = sum_q Tq H_q follow after the loop: four operations per
+                            // column and n in place of six, and none per n for S1, S2
                             constexpr int NTQ = NM3X <= 8 ? NM3X : 1;
-                            cplx<T> S1 = {0, 0}, S2 = {0, 0}, Tq[NTQ];
+                            cplx<T> S1 = {0, 0}, S2 = {0, 0}, Tq[NTQ], Dq[NTQ];
 #pragma unroll
-                            for (int q = 0; q < NTQ; ++q) Tq[q] = {0, 0};
+                            for (int q = 0; q < NTQ; ++q) Tq[q] = Dq[q] = {0, 0};
 #pragma unroll
                             for (int n = 0; n < NRAX; ++n) {
                                 // Q1 runs to the compile-time bound (past nr3a, R = R' = 0 and
@@ -1621,30 +1625,44 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                                     for (int q = 0; q < NM3X; ++q)
                                         if (!Q1 || used(q, sg)) wv[q] = Wn[q < P.nm3 ? q : 0];
                                 }
-                                cplx<T> V = {0, 0};
-                                const cplx<T> ry = {R[n] * Y.re, R[n] * Y.im};
+                                if constexpr (kHoist) {
 #pragma unroll
-                                for (int q = 0; q < NM3X; ++q) {
-                                    if (used(q, sg)) {
-                                        const cplx<T> wq = kHoist ? wv[kHoist ? q : 0] : Wn[q];
-                                        cfma(V, wq, H[q]);
-                                        if constexpr (kHoist) {
-                                            Tq[kHoist ? q : 0].re = fma(R[n], wq.re, Tq[kHoist ? q : 0].re);
-                                            Tq[kHoist ? q : 0].im = fma(R[n], wq.im, Tq[kHoist ? q : 0].im);
-                                        } else {
-                                            cfma(U[q], wq, ry);  // wide classes: Tq would not fit
+                                    for (int q = 0; q < NM3X; ++q) {
+                                        if (used(q, sg)) {
+                                            const cplx<T> wq = wv[kHoist ? q : 0];
+                                            const int qt = kHoist ? q : 0;
+                                            Tq[qt].re = fma(R[n], wq.re, Tq[qt].re);
+                                            Tq[qt].im = fma(R[n], wq.im, Tq[qt].im);
+                                            Dq[qt].re = fma(dR[n], wq.re, Dq[qt].re);
+                                            Dq[qt].im = fma(dR[n], wq.im, Dq[qt].im);
                                         }
                                     }
+                                } else {
+                                    // wide classes: per-column sums would not fit
+                                    cplx<T> V = {0, 0};
+                                    const cplx<T> ry = {R[n] * Y.re, R[n] * Y.im};
+#pragma unroll
+                                    for (int q = 0; q < NM3X; ++q) {
+                                        if (used(q, sg)) {
+                                            const cplx<T> wq = Wn[q];
+                                            cfma(V, wq, H[q]);
+                                            cfma(U[q], wq, ry);
+                                        }
+                                    }
+                                    S1.re = fma(dR[n], V.re, S1.re);
+                                    S1.im = fma(dR[n], V.im, S1.im);
+                                    S2.re = fma(R[n], V.re, S2.re);
+                                    S2.im = fma(R[n], V.im, S2.im);
                                 }
-                                S1.re = fma(dR[n], V.re, S1.re);
-                                S1.im = fma(dR[n], V.im, S1.im);
-                                S2.re = fma(R[n], V.re, S2.re);
-                                S2.im = fma(R[n], V.im, S2.im);
                             }
                             if constexpr (NM3X <= 8) {
 #pragma unroll
                                 for (int q = 0; q < NTQ; ++q)
-                                    if (used(q, sg)) cfma(U[q], Tq[q], Y);
+                                    if (used(q, sg)) {
+                                        cfma(S1, Dq[q], H[q]);
+                                        cfma(S2, Tq[q], H[q]);
+                                        cfma(U[q], Tq[q], Y);
+                                    }
                             }
                             // Re[S1 Y] e + Re[S2 grad Y] / r
                             const T a = re_mul(S1, Y);
@@ -1707,7 +1725,9 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                         for (int np = 0; np < NR2X; ++np) {
                             if (np >= P.nr2m) break;
                             const cplx<T> H2 = {Mv[np] * Y.re, Mv[np] * Y.im};
-                            cplx<T> U2 = {0, 0};
+                            // U2 = sum_n g_n W2[n], D2 = sum_n g'_n W2[n]; the force's
+                            // sum_n g'_n Re(W2[n] H2) is Re(D2 H2)
+                            cplx<T> U2 = {0, 0}, D2 = {0, 0};
                             const cplx<T>* W2 = W + P.oA2 + np * P.lsm + lmp;
 #pragma unroll
                             for (int n = 0; n < NBAX; ++n) {
@@ -1715,10 +1735,12 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                                 // loop bounded at run time was left rolled, its register
                                 // arrays indexed indirectly (FP32 forces 1.46 -> 1.30 ms)
                                 const cplx<T> wv = W2[(size_t)(n < P.nb_a ? n : 0) * P.nr2m * P.lsm];
-                                sf = fma(dga[n], re_mul(wv, H2), sf);
+                                D2.re = fma(wv.re, dga[n], D2.re);
+                                D2.im = fma(wv.im, dga[n], D2.im);
                                 U2.re = fma(wv.re, ga[n], U2.re);
                                 U2.im = fma(wv.im, ga[n], U2.im);
                             }
+                            sf += re_mul(D2, H2);
                             if (lp <= P.l3m && np < P.nr3m) {
                                 const int q = np * P.lsm3 + lmp;
 #pragma unroll
