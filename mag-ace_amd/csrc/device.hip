@@ -2410,47 +2410,48 @@ static hipError_t run_class(const DevParams<T>& P, int natom, int atom_base, con
     static_assert(!ACC || sizeof(T) == 4, "the accumulation recipe belongs to the FP32 kernels");
     auto k1 = k_density<T, L, CH, NB, NM3X, NB2, NR2X, SPLIT, TASKSPLIT, ACC>;
     const bool nd2 = P.nd == 2;
-    // the Q1 instantiation exists only for the four-column class
+    // the Q1 instantiation exists only for the four-column class.  Q1 visits every radial index
+    // up to its compile-time bound, so the bounds 4 (for nr3a <= 4), 5, 6, 7 and NRAX each have
+    // an instance; without Q1 all five name the general kernel
     constexpr bool kQ1 = NM3X == 4;
     const bool q1 = kQ1 && P.nm3 == 4 && P.lsm3 == 4;
-    // Q1 visits every radial index up to its bound: a bound of 4 for nr3a <= 4
-    constexpr int kNR4 = NRAX < 4 ? NRAX : 4;
-    const bool r4 = P.nr3a <= kNR4;
-    auto k3q = k_forces<T, L, L, NM3X, NBAX, NRAX, NR2X, kQ1>;
-    auto k3q4 = k_forces<T, L, L, NM3X, NBAX, kNR4, NR2X, kQ1>;
     auto k3g = k_forces<T, L, L, NM3X, NBAX, NRAX, NR2X, false>;
-    auto k3 = q1 ? (r4 ? k3q4 : k3q) : k3g;
+    constexpr int kB4 = kQ1 && NRAX > 4 ? 4 : NRAX, kB5 = kQ1 && NRAX > 5 ? 5 : NRAX,
+                  kB6 = kQ1 && NRAX > 6 ? 6 : NRAX, kB7 = kQ1 && NRAX > 7 ? 7 : NRAX;
+    constexpr int kQB[5] = {kB4, kB5, kB6, kB7, NRAX};
+    const decltype(k3g) k3qs[5] = {
+        k_forces<T, L, L, NM3X, NBAX, kB4, NR2X, kQ1>, k_forces<T, L, L, NM3X, NBAX, kB5, NR2X, kQ1>,
+        k_forces<T, L, L, NM3X, NBAX, kB6, NR2X, kQ1>, k_forces<T, L, L, NM3X, NBAX, kB7, NR2X, kQ1>,
+        k_forces<T, L, L, NM3X, NBAX, NRAX, NR2X, kQ1>};
+    int qi = 0;  // the smallest bound that covers nr3a
+    while (qi < 4 && kQB[qi] < P.nr3a) ++qi;
+    auto k3 = q1 ? k3qs[qi] : k3g;
     hipError_t e = hipSuccess;
     // the dynamic-LDS limit is raised once per kernel instantiation (to the workgroup
-    // maximum), so that evaluations can be captured into graphs without further API calls
+    // maximum), so that evaluations can be captured into graphs without further API calls.
+    // limit[]: 0 density, 1 adjoint, 2 adjoint with two densities, 3 two-atom adjoint,
+    // 4 general forces, 5 + i the Q1 forces of bound kQB[i]
     static bool attributes_set = false;
-    static size_t limit[7] = {0, 0, 0, 0, 0, 0, 0};
+    static size_t limit[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (!attributes_set) {
         int dev = 0, maxsh = 0;
         if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
         if ((e = hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev)) != hipSuccess)
             return e;
-        const void* ks[6] = {(const void*)k1, (const void*)k_adjoint<T, false, ACC>, (const void*)k3q,
-                             (const void*)k_adjoint<T, true, ACC>, (const void*)k3g, (const void*)k3q4};
-        for (int i = 0; i < 6; ++i) {
-            // limit[5] belongs to k_adjoint_pair below
-            const int li = i < 5 ? i : 6;
+        const void* ks[10] = {(const void*)k1, (const void*)k_adjoint<T, false, ACC>,
+                              (const void*)k_adjoint<T, true, ACC>, nullptr, (const void*)k3g,
+                              (const void*)k3qs[0], (const void*)k3qs[1], (const void*)k3qs[2],
+                              (const void*)k3qs[3], (const void*)k3qs[4]};
+        if constexpr (sizeof(T) == 4) ks[3] = (const void*)k_adjoint_pair<ACC>;
+        for (int i = 0; i < 10; ++i) {
+            if (!ks[i]) continue;
             hipFuncAttributes fa;
             if ((e = hipFuncGetAttributes(&fa, ks[i])) != hipSuccess) return e;
             // the adjoints address their dynamic LDS from 0 (lds_at)
-            if ((i == 1 || i == 3) && fa.sharedSizeBytes != 0) return hipErrorInvalidKernelFile;
-            limit[li] = (size_t)maxsh > fa.sharedSizeBytes ? (size_t)maxsh - fa.sharedSizeBytes : 0;
-            if ((e = hipFuncSetAttribute(ks[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)limit[li])) !=
+            if (i >= 1 && i <= 3 && fa.sharedSizeBytes != 0) return hipErrorInvalidKernelFile;
+            limit[i] = (size_t)maxsh > fa.sharedSizeBytes ? (size_t)maxsh - fa.sharedSizeBytes : 0;
+            if ((e = hipFuncSetAttribute(ks[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)limit[i])) !=
                 hipSuccess)
-                return e;
-        }
-        if constexpr (sizeof(T) == 4) {
-            hipFuncAttributes fa;
-            if ((e = hipFuncGetAttributes(&fa, (const void*)k_adjoint_pair<ACC>)) != hipSuccess) return e;
-            if (fa.sharedSizeBytes != 0) return hipErrorInvalidKernelFile;
-            limit[5] = (size_t)maxsh > fa.sharedSizeBytes ? (size_t)maxsh - fa.sharedSizeBytes : 0;
-            if ((e = hipFuncSetAttribute((const void*)k_adjoint_pair<ACC>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)limit[5])) != hipSuccess)
                 return e;
         }
         attributes_set = true;
@@ -2458,11 +2459,11 @@ static hipError_t run_class(const DevParams<T>& P, int natom, int atom_base, con
     // FP32 with one density: two atoms per adjoint block (their Z and W interleaved in LDS)
     const bool pair = sizeof(T) == 4 && !nd2 && P.crec16 != nullptr;
     const size_t lds2p = 2 * (size_t)P.nzx * 16 + (2 * (kAdjointThreads / 64) + 2) * sizeof(double);
-    if (pair && lds2p > limit[5]) return hipErrorInvalidValue;
+    if (pair && lds2p > limit[3]) return hipErrorInvalidValue;
     const size_t lds2 =
         (nd2 ? 3 : 2) * (size_t)P.nzx * sizeof(cplx<T>) + (2 * (kAdjointThreads / 64) + 2) * sizeof(double);
     const size_t lds3 = (size_t)P.nz * sizeof(cplx<T>);
-    if (lds1 > limit[0] || lds2 > limit[nd2 ? 3 : 1] || lds3 > limit[q1 ? (r4 ? 6 : 2) : 4])
+    if (lds1 > limit[0] || lds2 > limit[nd2 ? 2 : 1] || lds3 > limit[q1 ? 5 + qi : 4])
         return hipErrorInvalidValue;
     if (ev) (void)hipEventRecord(ev[0], s);
     hipLaunchKernelGGL(k1, dim3(natom), dim3(256), lds1, s, P, off, dlist, elist, dmomi, emomi, dmom,
