@@ -1431,10 +1431,10 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
     extern __shared__ __align__(16) unsigned char smem_f[];
     cplx<T>* sW = reinterpret_cast<cplx<T>*>(smem_f);
     {
-        // eight loads in flight per lane: one trip to memory serves 512 entries, where a load
-        // followed by its store had waited for every 64
+        // forty loads in flight per lane: one trip to memory serves 2560 entries (all of W for
+        // lmax 6), where a load followed by its store had waited for every 64
         const cplx<T>* __restrict__ Wg = Win + (size_t)atom * P.nz;
-        constexpr int U = 8;
+        constexpr int U = 40;
         for (int i0 = 0; i0 < P.nz; i0 += 64 * U) {
             cplx<T> v[U];
 #pragma unroll
