@@ -1592,6 +1592,40 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                             cplx<T> S1 = {0, 0}, S2 = {0, 0}, Tq[NTQ], Dq[NTQ];
 #pragma unroll
                             for (int q = 0; q < NTQ; ++q) Tq[q] = Dq[q] = {0, 0};
+                            if constexpr (Q1 && sizeof(T) == 4) {
+                                // FP32 rows are 32 bytes, 16-byte aligned: columns 2, 3 in one
+                                // ds_read_b128 and column 0 (m >= 0 only) in another.  Each row's
+                                // reads are issued kAhead rows before its sums, so that their
+                                // latency passes during the sums of the rows before it (read and
+                                // used at once, every row had waited for its own reads).  The
+                                // loop runs to the compile-time bound, as below.
+                                constexpr int kAhead = 2, kRing = kAhead + 1;
+                                float4 r23[kRing], r01[kRing];
+#pragma unroll
+                                for (int n = 0; n < NRAX + kAhead; ++n) {
+                                    if (n < NRAX) {
+                                        const int nw = n >= P.nr3a ? 0 : n;
+                                        const float4* Wn = (const float4*)(W + (nw * P.lsa + lm) * P.nm3);
+                                        r23[n % kRing] = Wn[1];
+                                        if (sg == 0) r01[n % kRing] = Wn[0];
+                                    }
+                                    const int k = n - kAhead;  // the row summed now
+                                    if (k < 0) continue;
+                                    cplx<T> wv[NM3X];
+                                    wv[2] = {r23[k % kRing].x, r23[k % kRing].y};
+                                    wv[3] = {r23[k % kRing].z, r23[k % kRing].w};
+                                    if (sg == 0) wv[0] = {r01[k % kRing].x, r01[k % kRing].y};
+#pragma unroll
+                                    for (int q = 0; q < NM3X; ++q) {
+                                        if (used(q, sg)) {
+                                            Tq[q].re = fma(R[k], wv[q].re, Tq[q].re);
+                                            Tq[q].im = fma(R[k], wv[q].im, Tq[q].im);
+                                            Dq[q].re = fma(dR[k], wv[q].re, Dq[q].re);
+                                            Dq[q].im = fma(dR[k], wv[q].im, Dq[q].im);
+                                        }
+                                    }
+                                }
+                            } else
 #pragma unroll
                             for (int n = 0; n < NRAX; ++n) {
                                 // Q1 runs to the compile-time bound (past nr3a, R = R' = 0 and
@@ -1610,17 +1644,7 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                                 // hoisted values would not fit in registers)
                                 constexpr bool kHoist = NM3X <= 8;
                                 cplx<T> wv[kHoist ? NM3X : 1];
-                                if constexpr (Q1 && sizeof(T) == 4) {
-                                    // FP32 rows are 32 bytes, 16-byte aligned: two ds_read_b128
-                                    // for columns 0, 2, 3 instead of three 8-byte reads
-                                    const float4 w23 = *(const float4*)(Wn + 2);
-                                    wv[2] = {w23.x, w23.y};
-                                    wv[3] = {w23.z, w23.w};
-                                    if (sg == 0) {
-                                        const float4 w01 = *(const float4*)Wn;
-                                        wv[0] = {w01.x, w01.y};
-                                    }
-                                } else if constexpr (kHoist) {
+                                if constexpr (kHoist) {
 #pragma unroll
                                     for (int q = 0; q < NM3X; ++q)
                                         if (!Q1 || used(q, sg)) wv[q] = Wn[q < P.nm3 ? q : 0];
