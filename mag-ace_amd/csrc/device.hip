@@ -1745,6 +1745,20 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                         cplx<T> Y, g[3];
                         ylm_grad<T>(am, Lr.P[am], Lr.dP[am], phm[am], phm[am > 0 ? am - 1 : 0], sx, sy, sz, Y, g);
                         const int lmp = lp * lp + lp + am;
+                        // Q1 in FP32: the W2 entries of every n' of this (l', m') are read
+                        // before the first sum, so that one LDS round trip serves all n' (read
+                        // per n', each n' had waited for its own reads)
+                        constexpr bool kW2B = Q1 && sizeof(T) == 4;
+                        cplx<T> w2b[kW2B ? NR2X : 1][kW2B ? NBAX : 1];
+                        if constexpr (kW2B) {
+#pragma unroll
+                            for (int np = 0; np < NR2X; ++np) {
+                                const cplx<T>* W2r = W + P.oA2 + (np < P.nr2m ? np : 0) * P.lsm + lmp;
+#pragma unroll
+                                for (int n = 0; n < NBAX; ++n)
+                                    w2b[np][n] = W2r[(size_t)(n < P.nb_a ? n : 0) * P.nr2m * P.lsm];
+                            }
+                        }
 #pragma unroll
                         for (int np = 0; np < NR2X; ++np) {
                             if (np >= P.nr2m) break;
@@ -1758,7 +1772,8 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                                 // no early exit: entries n >= nb_a have ga = dga = 0, and a
                                 // loop bounded at run time was left rolled, its register
                                 // arrays indexed indirectly (FP32 forces 1.46 -> 1.30 ms)
-                                const cplx<T> wv = W2[(size_t)(n < P.nb_a ? n : 0) * P.nr2m * P.lsm];
+                                const cplx<T> wv = kW2B ? w2b[kW2B ? np : 0][kW2B ? n : 0]
+                                                        : W2[(size_t)(n < P.nb_a ? n : 0) * P.nr2m * P.lsm];
                                 D2.re = fma(wv.re, dga[n], D2.re);
                                 D2.im = fma(wv.im, dga[n], D2.im);
                                 U2.re = fma(wv.re, ga[n], U2.re);
