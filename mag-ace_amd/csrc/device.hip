@@ -568,18 +568,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                 // and radial indices past nr3a are read without a test: those reads stay inside
                 // the staging area, and their sums land in slots that are never stored.
                 for (int j = part; j < nj; j += sa) {
+                    // the pair's Y, H and R reads issued together (R read after the products
+                    // Y H had waited for LDS a second time); the wide classes keep the later
+                    // reads, as they would spill
                     const cplx<T> y = sY[j * sYs + lh];
                     cplx<T> h[NM3X];
 #pragma unroll
                     for (int q = 0; q < NM3X; ++q) h[q] = sH[j * sHs + q];
                     const T* rr = sR + j * sRs + l * P.nr3a + n0;
+                    constexpr bool kEarly = NM3X <= 4;
+                    T rv[kEarly ? NB : 1];
+                    if constexpr (kEarly)
+#pragma unroll
+                        for (int b = 0; b < NB; ++b) rv[b] = rr[b];
                     // Y H_q once per pair and column, then two real FMAs per radial index
                     cplx<T> yh[NM3X];
 #pragma unroll
                     for (int q = 0; q < NM3X; ++q) yh[q] = cmul(y, h[q]);
 #pragma unroll
                     for (int b = 0; b < NB; ++b) {
-                        const T r = rr[b];
+                        const T r = kEarly ? rv[kEarly ? b : 0] : rr[b];
 #pragma unroll
                         for (int q = 0; q < NM3X; ++q) {
                             if constexpr (ACC) {
@@ -596,15 +604,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                 // A2[n][n'][l'm'] += g_n M_{l',n'} Y_{l'm'}   (src/a_basis.F90:193-201); unused
                 // slots as above
                 for (int j = 0; j < nj; ++j) {
+                    // the pair's g reads issued with its Y and M reads, as in the A loop
                     const cplx<T> y = sYm[j * sYms + lh];
                     const T* mr = sM + j * sMs + l * P.nr2m;
+                    const T* gr = sG + j * sGs + n0;
+                    constexpr bool kEarly = NM3X <= 4;
+                    T gv[kEarly ? NB2 : 1];
+                    if constexpr (kEarly)
+#pragma unroll
+                        for (int b = 0; b < NB2; ++b) gv[b] = gr[b];
                     cplx<T> my[NR2X];
 #pragma unroll
                     for (int q = 0; q < NR2X; ++q) my[q] = {mr[q] * y.re, mr[q] * y.im};
-                    const T* gr = sG + j * sGs + n0;
 #pragma unroll
                     for (int b = 0; b < NB2; ++b) {
-                        const T g = gr[b];
+                        const T g = kEarly ? gv[kEarly ? b : 0] : gr[b];
 #pragma unroll
                         for (int q = 0; q < NR2X; ++q) {
                             if constexpr (ACC) {
