@@ -1386,28 +1386,43 @@ struct LegRows {
     }
 };
 
-// one entry P_l^m, dP_l^m/dz of the rows above: P_m^m = prod_{k<=m} c(k), then
-// P_{m+1}^m = d(m+1) z P_m^m and the three-term recurrence up to l
-template <typename T>
+// one entry P_l^m, dP_l^m/dz of the rows above (l <= L): P_m^m = prod_{k<=m} c(k), then
+// P_{m+1}^m = d(m+1) z P_m^m and the three-term recurrence up to l.  The constants are loaded
+// first, to the compile-time bound (factors past m are 1 and steps past l are skipped), so
+// that one trip to memory serves them all instead of one per factor and step.
+template <typename T, int L>
 __device__ __forceinline__ void legendre_one(int l, int m, T z, T& p, T& dp) {
-    T pmm = (T)1;
-    for (int k = 1; k <= m; ++k) pmm *= LegC<T>::c(k);
-    p = pmm;
-    dp = (T)0;
-    if (l == m) return;
-    T p2 = pmm, dp2 = (T)0;
+    static_assert(L >= 1 && L < kLmaxC, "the constant tables cover l <= kLmaxC");
+    T cf[L], ca[L], cb[L];
+#pragma unroll
+    for (int k = 1; k <= L; ++k) cf[k - 1] = LegC<T>::c(k);  // every index is in the table
     const T dm = LegC<T>::d(m + 1);
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        const int ll = min(m + 2 + i, L), k = ll * (ll + 1) / 2 + m;
+        ca[i] = LegC<T>::a(k);
+        cb[i] = LegC<T>::b(k);
+    }
+    // every step is computed and kept or dropped by a select: with early exits the compiler
+    // moved each load into its branch, one trip to memory per step again
+    T pmm = (T)1;
+#pragma unroll
+    for (int k = 1; k <= L; ++k) pmm *= k <= m ? cf[k - 1] : (T)1;
+    T p2 = pmm, dp2 = (T)0;
     T p1 = dm * pmm * z, dp1 = dm * pmm;
-    for (int ll = m + 2; ll <= l; ++ll) {
-        const int k = ll * (ll + 1) / 2 + m;
-        const T a = LegC<T>::a(k), b = LegC<T>::b(k);
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        const T a = ca[i], b = cb[i];
         const T pn = a * fma(z, p1, b * p2);
         const T dpn = a * (p1 + fma(z, dp1, b * dp2));
-        p2 = p1; dp2 = dp1;
-        p1 = pn; dp1 = dpn;
+        const bool on = m + 2 + i <= l;
+        p2 = on ? p1 : p2;
+        dp2 = on ? dp1 : dp2;
+        p1 = on ? pn : p1;
+        dp1 = on ? dpn : dp1;
     }
-    p = p1;
-    dp = dp1;
+    p = l == m ? pmm : p1;
+    dp = l == m ? (T)0 : dp1;
 }
 
 template <typename T>
@@ -1838,34 +1853,45 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
         }
     }
     // ---- self term T0 = dE_i/dm_i over m' >= 0 (src/forces.F90:712-749), spread over the
-    // wave: lane k takes B1 function k and the (l', m' >= 0) entries k, k + 64, ... of the half
-    // set (each with its own Legendre recurrence), and the wave sums the three components in a
-    // fixed butterfly order
+    // wave: lane k takes B1 function k (nb_m <= 32) and entry k of the (l', m' >= 0) half set
+    // (at most 28 entries for l2m <= LM <= 6; each lane with its own Legendre recurrence), and
+    // the wave sums the three components in a fixed butterfly order.  The global loads come in
+    // two batches, the atom's moment, direction and dE first, then every coefficient a lane
+    // needs (its B1 function and c1, and the M rows of its (l', m')): loaded where each was
+    // used, they had taken seven trips to memory at the end of every workgroup.
+    static_assert((LM + 1) * (LM + 2) / 2 <= 64, "one half-set entry per lane");
     {
         const double mu_d = dmomi[atom];
+        const double sxd = emomi[3 * atom], syd = emomi[3 * atom + 1], szd = emomi[3 * atom + 2];
+        const double dE0 = dEin[(size_t)atom * P.nd], dE1 = P.nd > 1 ? dEin[(size_t)atom * P.nd + 1] : 0.0;
+        const bool b1 = lane < P.nb_m, hs = lane < P.lhm;
+        const double c10 = P.c1[0][b1 ? lane : 0], c11 = P.c1[1][b1 ? lane : 0];
         const T invmu = (T)(1.0 / mu_d);
-        const T sx = (T)emomi[3 * atom], sy = (T)emomi[3 * atom + 1], sz = (T)emomi[3 * atom + 2];
+        const T sx = (T)sxd, sy = (T)syd, sz = (T)szd;
         T wm = 0;
         bool unit = false;
         const T* rowm = magnetic_row(P, mu_d, wm, unit, err);
         const T rsm = (T)P.rs_m;
+        int lp = 0;
+        while ((lp + 1) * (lp + 2) / 2 <= lane) ++lp;
+        const int am = lane - lp * (lp + 1) / 2;
+        T cb[4], cm[NR2X][4];
+        load_coef(rowm && b1 ? rowm + 4 * lane : P.lut_m, cb);
+#pragma unroll
+        for (int np = 0; np < NR2X; ++np)
+            load_coef(rowm && hs && np < P.nr2m ? rowm + 4 * (P.nb_m + lp * P.nr2m + np) : P.lut_m, cm[np]);
         T Tx = 0, Ty = 0, Tz = 0;
-        // B1: dE/dg^m_n = dE/drho c1_n
-        const double dE0 = dEin[(size_t)atom * P.nd], dE1 = P.nd > 1 ? dEin[(size_t)atom * P.nd + 1] : 0.0;
-        for (int n = lane; n < P.nb_m; n += 64) {
+        if (b1) {  // B1: dE/dg^m_n = dE/drho c1_n
             T v = unit ? (T)1 : (T)0, dv = 0;
-            if (rowm) horner(rowm + 4 * n, wm, rsm, v, dv);
-            const T s1 = (T)(dE0 * P.c1[0][n] + dE1 * P.c1[1][n]) * dv;
+            if (rowm) horner(cb, wm, rsm, v, dv);
+            const T s1 = (T)(dE0 * c10 + dE1 * c11) * dv;
             Tx = fma(s1, sx, Tx);
             Ty = fma(s1, sy, Ty);
             Tz = fma(s1, sz, Tz);
         }
-        for (int it = lane; it < P.lhm; it += 64) {
-            int lp = 0;
-            while ((lp + 1) * (lp + 2) / 2 <= it) ++lp;
-            const int am = it - lp * (lp + 1) / 2;
+        if (hs) {
             T pl, dpl;
-            legendre_one<T>(lp, am, sz, pl, dpl);
+            legendre_one<T, LM>(lp, am, sz, pl, dpl);
             cplx<T> ph = {1, 0}, ph1 = {1, 0};
             for (int k = 0; k < am; ++k) {
                 ph1 = ph;
@@ -1873,9 +1899,11 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
             }
             cplx<T> Y, g[3];
             ylm_grad<T>(am, pl, dpl, ph, ph1, sx, sy, sz, Y, g);
-            for (int np = 0; np < P.nr2m; ++np) {
+#pragma unroll
+            for (int np = 0; np < NR2X; ++np) {
+                if (np >= P.nr2m) break;
                 T v = unit ? (T)1 : (T)0, dv = 0;
-                if (rowm) horner(rowm + 4 * (P.nb_m + lp * P.nr2m + np), wm, rsm, v, dv);
+                if (rowm) horner(cm[np], wm, rsm, v, dv);
                 const cplx<T> w0 = W[P.oA0 + np * P.lsm + lp * lp + lp + am];
                 const T a = dv * re_mul(w0, Y), b = v * invmu;
                 Tx += fma(a, sx, b * re_mul(w0, g[0]));
