@@ -1519,6 +1519,14 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
             bool unit = false;
             const T* rowm = magnetic_row(P, mu_d, wm, unit, err);
             const T rsa = (T)P.rs_a, rsm = (T)P.rs_m;
+            // the atomic loop's first spline coefficients (row l = 0); each later row is loaded
+            // one l ahead in the loop.  Q1 loads row 0 here, so that it comes in the same trip
+            // to memory as the M values below (the wide classes load it at the loop, where
+            // their registers allow)
+            T cq0[Q1 ? NRAX : 1][4];
+            if constexpr (Q1)
+#pragma unroll
+                for (int n = 0; n < NRAX; ++n) load_coef(n < P.nr3a ? rowa + 4 * (P.nb_a + n) : P.lut_a, cq0[n]);
             // H_q = M_{l',n'}(mu_j) Y_{l'm'}(s_j), m' >= 0, for the magnetic columns of A
             cplx<T> H[NM3X], U[NM3X];
 #pragma unroll
@@ -1576,7 +1584,12 @@ __global__ void __launch_bounds__(64) k_forces(DevParams<T> P, const int* __rest
                 T cq[NRAX][4];
 #pragma unroll
                 for (int n = 0; n < NRAX; ++n) {
-                    load_coef(n < P.nr3a ? rowa + 4 * (P.nb_a + n) : P.lut_a, cq[n]);
+                    if constexpr (Q1) {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) cq[n][k] = cq0[Q1 ? n : 0][k];
+                    } else {
+                        load_coef(n < P.nr3a ? rowa + 4 * (P.nb_a + n) : P.lut_a, cq[n]);
+                    }
 #pragma unroll
                     for (int k = 0; k < 4; ++k) cq[n][k] = n < P.nr3a ? cq[n][k] : (T)0;
                 }
