@@ -466,6 +466,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                         }
                         const double* v3 = bond ? elist : emom;
                         cplx<T>* yo = bond ? sY + j * sYs : sYm + j * sYms;
+                        // the moment's M coefficients for H (four-column class: l' <= 1, one n'),
+                        // loaded before the Y_lm so that their trip overlaps it
+                        constexpr bool kMh = NM3X == 4 && sizeof(T) == 4;
+                        T cmh[kMh ? 2 : 1][4];
+                        if constexpr (kMh)
+#pragma unroll
+                            for (int lp = 0; lp < 2; ++lp)
+                                load_coef(!bond && row && lp <= P.l3m ? row + 4 * (P.nb_m + lp * P.nr2m) : P.lut_m, cmh[lp]);
                         ylm_half<T, L>((T)v3[3 * p], (T)v3[3 * p + 1], (T)v3[3 * p + 2], bond ? P.l3a : P.l2m, yo);
                         if (!bond) {
                             // H_q = M_{l',n'} Y_{l'm'} for the magnetic columns of A (l' <= l3m,
@@ -474,7 +482,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 6
                             for (int np = 0; np * P.lsm3 < P.nm3; ++np)
                                 for (int lp = 0; lp <= P.l3m; ++lp) {
                                     T mv = unit ? (T)1 : (T)0, dv;
-                                    if (row) horner(row + 4 * (P.nb_m + lp * P.nr2m + np), w, (T)P.rs_m, mv, dv);
+                                    if (kMh && np == 0 && lp < 2) {
+                                        if (row) horner(cmh[kMh && lp < 2 ? lp : 0], w, (T)P.rs_m, mv, dv);
+                                    } else if (row) {
+                                        horner(row + 4 * (P.nb_m + lp * P.nr2m + np), w, (T)P.rs_m, mv, dv);
+                                    }
                                     for (int mp = -lp; mp <= lp; ++mp) {
                                         const int q = np * P.lsm3 + lp * lp + lp + mp;
                                         const cplx<T> y = yfull(yo, lp, mp);
